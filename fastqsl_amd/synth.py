@@ -1,0 +1,193 @@
+"""Synthetic magnetic fields and seed grids for the benchmark workloads.
+
+All fields are sampled at grid nodes and returned as three arrays of shape
+(nx, ny, nz) -- the layout the reference gets from
+``np.transpose(readsav(...), (2, 1, 0))`` (demo/Q_RKF45CuPy.ipynb cell [2]) --
+stored x-fastest.  Tracing happens in grid-index coordinates (cells).
+
+Workloads:
+
+* ``quadrupole_64``     4 alternating point sources, 64^3 float64
+* ``multipole_256``     40 random point sources (default_rng(seed)), 256^3 float32
+* ``rope_arcade``       Gold-Hoyle flux rope along x plus two bipoles, 512x256x256
+* ``multipole``         the random-multipole generator at any shape/seed
+
+Generation runs in torch (on the GPU when one is present) in float64 and is
+cast to the requested dtype at the end; the result is returned as NumPy.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _axes(shape, box, device, torch):
+    (x0, x1), (y0, y1), (z0, z1) = box
+    nx, ny, nz = shape
+    xs = torch.linspace(x0, x1, nx, dtype=torch.float64, device=device)
+    ys = torch.linspace(y0, y1, ny, dtype=torch.float64, device=device)
+    zs = torch.linspace(z0, z1, nz, dtype=torch.float64, device=device)
+    return xs, ys, zs
+
+
+def _to_numpy(bx, by, bz, dtype):
+    # torch tensors are (nz, ny, nx) C-order (x fastest); hand back (nx, ny, nz) views
+    out = []
+    for b in (bx, by, bz):
+        a = b.to("cpu").numpy().astype(dtype, copy=False)
+        out.append(a.transpose(2, 1, 0))
+    return tuple(out)
+
+
+def _default_device(device):
+    torch = _torch()
+    if device is None:
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+    return device
+
+
+def point_sources(shape, charges, positions, box=((-1, 1), (-1, 1), (0, 2)), dtype=np.float32,
+                  device=None, zchunk=16):
+    """B = sum_i q_i (r - r_i) / |r - r_i|^3 sampled at the grid nodes."""
+    torch = _torch()
+    device = _default_device(device)
+    xs, ys, zs = _axes(shape, box, device, torch)
+    nx, ny, nz = shape
+    bx = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+    by = torch.zeros_like(bx)
+    bz = torch.zeros_like(bx)
+    q = torch.as_tensor(np.asarray(charges, dtype=np.float64), device=device)
+    P = torch.as_tensor(np.asarray(positions, dtype=np.float64).reshape(-1, 3), device=device)
+    Y, X = torch.meshgrid(ys, xs, indexing="ij")  # (ny, nx)
+    for k0 in range(0, nz, zchunk):
+        k1 = min(nz, k0 + zchunk)
+        Z = zs[k0:k1].view(-1, 1, 1)
+        for i in range(q.shape[0]):
+            dx = X - P[i, 0]
+            dy = Y - P[i, 1]
+            dz = Z - P[i, 2]
+            r2 = dx * dx + dy * dy + dz * dz
+            w = q[i] / (r2 * torch.sqrt(r2))
+            bx[k0:k1] += w * dx
+            by[k0:k1] += w * dy
+            bz[k0:k1] += w * dz
+    return _to_numpy(bx, by, bz, dtype)
+
+
+def quadrupole_64(dtype=np.float64, device=None, n=64):
+    """Four sub-surface point sources q=(+1,-1,+1,-1) at (+.3,+.3), (-.3,+.3),
+    (-.3,-.3), (+.3,-.3), depth 0.15, in [-1,1]^2 x [0,2] on an n^3 grid."""
+    charges = [1.0, -1.0, 1.0, -1.0]
+    pos = [[0.3, 0.3, -0.15], [-0.3, 0.3, -0.15], [-0.3, -0.3, -0.15], [0.3, -0.3, -0.15]]
+    return point_sources((n, n, n), charges, pos, dtype=dtype, device=device)
+
+
+def multipole_sources(seed=0, count=40):
+    """Draw order: x ~ U[-1,1], y ~ U[-1,1], depth ~ U[0.05,0.3], q ~ N(0,1),
+    each as one vector of ``count`` draws from ``default_rng(seed)``."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1.0, 1.0, count)
+    y = rng.uniform(-1.0, 1.0, count)
+    depth = rng.uniform(0.05, 0.3, count)
+    q = rng.normal(0.0, 1.0, count)
+    return q, np.stack([x, y, -depth], axis=1)
+
+
+def multipole(shape=(256, 256, 256), seed=0, count=40, dtype=np.float32, device=None):
+    """Random multipolar potential field in [-1,1]^2 x [0,2]."""
+    q, pos = multipole_sources(seed, count)
+    return point_sources(shape, q, pos, dtype=dtype, device=device)
+
+
+def multipole_256(seed=0, dtype=np.float32, device=None):
+    return multipole((256, 256, 256), seed=seed, dtype=dtype, device=device)
+
+
+def rope_arcade(shape=(512, 256, 256), dtype=np.float32, device=None, turns=1.5, radius=0.2,
+                axis_z=0.4, b0=1.0, bipole_q=1.0):
+    """Uniform-twist Gold-Hoyle rope along x (B_x = B0/(1+T^2 r^2),
+    B_phi = B0 T r/(1+T^2 r^2), T = 2 pi turns / L, cut at r <= radius, axis at
+    (y=0, z=axis_z)) plus two point-source bipoles: +q at (-0.5, +-0.5, -0.1)
+    and -q at (+0.5, +-0.5, -0.1).  Box x in [-2,2], y in [-1,1], z in [0,2]."""
+    torch = _torch()
+    device = _default_device(device)
+    box = ((-2, 2), (-1, 1), (0, 2))
+    charges = [bipole_q, -bipole_q, bipole_q, -bipole_q]
+    pos = [[-0.5, 0.5, -0.1], [0.5, 0.5, -0.1], [-0.5, -0.5, -0.1], [0.5, -0.5, -0.1]]
+    bx, by, bz = point_sources(shape, charges, pos, box=box, dtype=np.float64, device=device)
+    xs, ys, zs = _axes(shape, box, device, torch)
+    L = box[0][1] - box[0][0]
+    T = 2 * math.pi * turns / L
+    Z, Y = torch.meshgrid(zs, ys, indexing="ij")  # (nz, ny)
+    dy = Y
+    dz = Z - axis_z
+    r = torch.sqrt(dy * dy + dz * dz)
+    inside = (r <= radius).to(torch.float64)
+    den = 1.0 + T * T * r * r
+    bxr = b0 / den * inside
+    bphi = b0 * T * r / den * inside
+    # phi direction about +x: (0, -dz, dy)/r
+    rs = torch.where(r > 0, r, torch.ones_like(r))
+    byr = -bphi * dz / rs
+    bzr = bphi * dy / rs
+    nx = shape[0]
+    add = [t.unsqueeze(2).expand(-1, -1, nx).to("cpu").numpy().transpose(2, 1, 0)
+           for t in (bxr, byr, bzr)]
+    return tuple((b + a).astype(dtype) for b, a in zip((bx, by, bz), add))
+
+
+def hyperbolic(shape, k=0.02, b0=1.0, dtype=np.float64, center=None):
+    """Linear X-point field B = (k (x-xc), -k (y-yc), b0) in index units.
+    Trilinear interpolation reproduces it exactly; lines obey
+    x - xc = (x0 - xc) exp(k z / b0), y - yc = (y0 - yc) exp(-k z / b0)."""
+    nx, ny, nz = shape
+    xc, yc = center if center is not None else ((nx - 1) / 2.0, (ny - 1) / 2.0)
+    i = np.arange(nx, dtype=np.float64).reshape(-1, 1, 1)
+    j = np.arange(ny, dtype=np.float64).reshape(1, -1, 1)
+    bx = np.broadcast_to(k * (i - xc), shape)
+    by = np.broadcast_to(-k * (j - yc), shape)
+    bz = np.full(shape, b0)
+    return tuple(np.asfortranarray(np.array(b, dtype=dtype)) for b in (bx, by, bz))
+
+
+def uniform(shape, b, dtype=np.float64):
+    return tuple(np.asfortranarray(np.full(shape, float(c), dtype=dtype)) for c in b)
+
+
+# ------------------------------------------------------------------ seeds
+def plane_seeds_z(nx_pix, ny_pix, x_range, y_range, z=0.0, dtype=np.float32):
+    """Seed grid on a constant-z plane: ``linspace`` x ``meshgrid`` ('xy'
+    indexing, rows = y), flattened row-major (demo/Q_RKF45CuPy.ipynb cell [6]).
+    Returns (seeds (n,3), (ny_pix, nx_pix))."""
+    xi = np.linspace(x_range[0], x_range[1], nx_pix, dtype=dtype)
+    yi = np.linspace(y_range[0], y_range[1], ny_pix, dtype=dtype)
+    X, Y = np.meshgrid(xi, yi)
+    Z = np.full_like(X, z)
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1), X.shape
+
+
+def plane_seeds_abc(A, B, C, n_u, n_v, dtype=np.float32):
+    """Seed grid spanning the parallelogram A + u (B-A) + v (C-A), u, v in
+    [0,1] (demo/Q_RKF45CuPyPlane.ipynb cell [4]); rows follow C-A.  Returns
+    (seeds, (n_v, n_u), unit normal (B-A)x(C-A)/|.|)."""
+    A = np.asarray(A, np.float64); B = np.asarray(B, np.float64); C = np.asarray(C, np.float64)
+    u = np.linspace(0.0, 1.0, n_u)
+    v = np.linspace(0.0, 1.0, n_v)
+    U, V = np.meshgrid(u, v)
+    P = A[None, None, :] + (B - A)[None, None, :] * U[..., None] + (C - A)[None, None, :] * V[..., None]
+    nrm = np.cross(B - A, C - A)
+    nrm = nrm / np.linalg.norm(nrm)
+    return P.reshape(-1, 3).astype(dtype), U.shape, nrm.astype(dtype)
+
+
+def volume_seeds(shape, dtype=np.float32):
+    """One seed per grid node of an (nx, ny, nz) cube, x fastest."""
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(dtype)
