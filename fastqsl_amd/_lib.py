@@ -1,0 +1,103 @@
+"""Loader and ctypes prototypes for ``libfqsl.so`` (declared in include/fqsl.h).
+
+The library is built in-tree by ``fastqsl_amd.build`` (hipcc, gfx950).  There
+is no fallback: if the library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfqsl.so")
+
+FQSL_OK = 0
+DTYPE_F32, DTYPE_F64 = 0, 1
+METHODS = {"rkf45": 0, "rk4": 1}
+PRECISIONS = {"fp32c": 0, "fp64": 1}
+SCHEDULES = {"persistent": 0, "per_line": 1}
+LAYOUT_SOA32, LAYOUT_SOA64, LAYOUT_CELL32, LAYOUT_ALL = 1, 2, 4, 7
+
+_lib = None
+
+vp = ctypes.c_void_p
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int
+
+
+class FqslError(RuntimeError):
+    pass
+
+
+class fqsl_scott_params(ctypes.Structure):
+    _fields_ = [("method", i32), ("precision", i32), ("schedule", i32), ("reserved", i32),
+                ("step", ctypes.c_double), ("tol", ctypes.c_double)]
+
+
+class fqsl_scott_out(ctypes.Structure):
+    _fields_ = [(n, vp) for n in ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
+                                  "flag_start", "flag_end", "b_this_x", "b_this_y", "b_this_z",
+                                  "q_0", "q_perp", "line_len", "evals")]
+
+
+class fqsl_plain_params(ctypes.Structure):
+    _fields_ = [("method", i32), ("precision", i32), ("schedule", i32), ("reserved", i32),
+                ("step", ctypes.c_double), ("tol_coef", ctypes.c_double),
+                ("ncross", ctypes.c_float * 3), ("reserved2", ctypes.c_float)]
+
+
+class fqsl_plain_out(ctypes.Structure):
+    _fields_ = [(n, vp) for n in ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
+                                  "flag_start", "flag_end", "b_this_x", "b_this_y", "b_this_z",
+                                  "b_flag", "b_start_x", "b_start_y", "b_start_z", "b_end_x",
+                                  "b_end_y", "b_end_z", "line_len", "evals")]
+
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "fqsl_last_error": (ctypes.c_char_p, []),
+    "fqsl_version": (ctypes.c_char_p, []),
+    "fqsl_field_create": (i32, [vp, vp, vp, i32, i32, i32, i32, i32, i32, ctypes.POINTER(vp)]),
+    "fqsl_field_destroy": (i32, [vp]),
+    "fqsl_field_bytes": (ctypes.c_size_t, [vp]),
+    "fqsl_field_shape": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+    "fqsl_trace_scott": (i32, [vp, ctypes.POINTER(fqsl_scott_params), u64, vp, vp, vp,
+                               ctypes.POINTER(fqsl_scott_out), vp]),
+    "fqsl_trace_plain": (i32, [vp, ctypes.POINTER(fqsl_plain_params), u64, vp, vp, vp,
+                               ctypes.POINTER(fqsl_plain_out), vp]),
+    "fqsl_qcalc_plane": (i32, [i32, i32] + [vp] * 15 + [ctypes.c_double, vp, vp, vp]),
+    "fqsl_pseudo_seeds": (i32, [u64, vp, vp, vp, vp, vp, vp, vp, ctypes.c_float, u64,
+                                vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64), vp]),
+    "fqsl_qcut_scatter": (i32, [u64] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),
+    "fqsl_TraceAllBlineScott": (i32, [vp] * 30 + [i32, i32, vp]),
+    "fqsl_TraceAllBline": (i32, [i32] + [vp] * 35 + [i32, i32, vp]),
+    "fqsl_last_kernel_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
+}
+
+
+def lib():
+    """The loaded library; raises FqslError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise FqslError(f"{LIBPATH} is missing: build it with fastqsl_amd.build.build() "
+                            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != FQSL_OK:
+        msg = lib().fqsl_last_error().decode(errors="replace")
+        raise FqslError(f"libfqsl error {rc}: {msg}")
+
+
+def last_kernel_ms() -> float:
+    v = ctypes.c_float()
+    check(lib().fqsl_last_kernel_ms(ctypes.byref(v)))
+    return float(v.value)

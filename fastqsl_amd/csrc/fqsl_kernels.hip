@@ -1,0 +1,937 @@
+// fqsl_kernels.hip -- gfx950 kernels and the C ABI of libfqsl.so.
+//
+// Kernels:
+//   k_build_cells          SoA float field -> 128-byte cell stencils
+//   k_scott_persistent     Scott tracer, persistent grid, per-lane refill
+//                          from a device work queue (one line per lane)
+//   k_scott_per_line       Scott tracer, one thread per line
+//   k_plain_persistent / k_plain_per_line   position-only tracer
+//   k_qcalc_plane          QCalcPlane (FastQSL.py:65-117)
+//   k_pseudo_*             CookPseudoLine compaction (FastQSL.py:4-57)
+//   k_qcut_scatter         Qcut + merge (demo/Q_RKF45CuPy.ipynb cells [10]-[11])
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "fqsl.h"
+#include "fqsl_tracer.h"
+
+using namespace fqsl;
+
+// ------------------------------------------------------------ error state
+static thread_local char g_err[512] = "";
+static thread_local hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+static thread_local bool g_ev_valid = false;
+
+static int set_err(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(FQSL_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+extern "C" const char* fqsl_last_error(void) { return g_err; }
+extern "C" const char* fqsl_version(void) { return "fastqsl_amd 0.1.0 (gfx950)"; }
+
+// ------------------------------------------------------------------- field
+struct fqsl_field {
+    int n[3];
+    int device;
+    float* soa32[3];
+    double* soa64[3];
+    float4* cell32;
+    size_t bytes;
+};
+
+__global__ void k_convert_to_f32(const double* in, float* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+__global__ void k_convert_to_f64(const float* in, double* out, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+// One thread per cell: gather the 8 corners of the 3 components into 8 float4.
+__global__ void k_build_cells(const float* __restrict__ bx, const float* __restrict__ by,
+                              const float* __restrict__ bz, int nx, int ny, int nz, float4* __restrict__ cells) {
+    const long cx = nx - 1, cy = ny - 1, cz = nz - 1;
+    const long ncell = cx * cy * cz;
+    const long nxy = (long)nx * ny;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+        const long i = t % cx, j = (t / cx) % cy, k = t / (cx * cy);
+        const long base = k * nxy + j * nx + i;
+        const float* comp[3] = {bx, by, bz};
+        float4* o = cells + 8 * t;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float* b = comp[q];
+            float v[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[c] = b[base + (c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy];
+            o[2 * q] = make_float4(v[0], v[1], v[2], v[3]);
+            o[2 * q + 1] = make_float4(v[4], v[5], v[6], v[7]);
+        }
+        o[6] = make_float4(0.f, 0.f, 0.f, 0.f);
+        o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+static int grid_for(long n, int block) {
+    long g = (n + block - 1) / block;
+    if (g > 65535L * 16) g = 65535L * 16;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+static void field_free(fqsl_field* f) {
+    for (int q = 0; q < 3; ++q) {
+        if (f->soa32[q]) (void)hipFree(f->soa32[q]);
+        if (f->soa64[q]) (void)hipFree(f->soa64[q]);
+    }
+    if (f->cell32) (void)hipFree(f->cell32);
+    delete f;
+}
+
+extern "C" int fqsl_field_create(const void* bx, const void* by, const void* bz, int nx, int ny, int nz,
+                                 int dtype, int on_device, int layouts, fqsl_field** out) {
+    if (!out) return set_err(FQSL_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!bx || !by || !bz) return set_err(FQSL_ERR_INVALID, "field component pointer is NULL");
+    if (nx < 2 || ny < 2 || nz < 2) return set_err(FQSL_ERR_INVALID, "each axis needs >= 2 nodes (%d,%d,%d)", nx, ny, nz);
+    if (dtype != FQSL_DTYPE_F32 && dtype != FQSL_DTYPE_F64) return set_err(FQSL_ERR_INVALID, "dtype %d", dtype);
+    if ((layouts & FQSL_LAYOUT_ALL) == 0) return set_err(FQSL_ERR_INVALID, "no layout requested");
+    const long nn = (long)nx * ny * nz;
+    if (nn > (1L << 31) - 1) return set_err(FQSL_ERR_INVALID, "field of %ld nodes exceeds int32 indexing", nn);
+    fqsl_field* f = new fqsl_field();
+    memset(f, 0, sizeof(*f));
+    f->n[0] = nx; f->n[1] = ny; f->n[2] = nz;
+    HIP_TRY(hipGetDevice(&f->device));
+    const void* src[3] = {bx, by, bz};
+    const size_t esz = dtype == FQSL_DTYPE_F64 ? 8 : 4;
+    // stage the input on the device in its own dtype
+    void* staged[3] = {nullptr, nullptr, nullptr};
+    int rc = FQSL_OK;
+    auto fail = [&](int code, const char* what, hipError_t e) {
+        for (int q = 0; q < 3; ++q)
+            if (staged[q] && !on_device) (void)hipFree(staged[q]);
+        field_free(f);
+        return set_err(code, "%s: %s", what, hipGetErrorString(e));
+    };
+    for (int q = 0; q < 3; ++q) {
+        if (on_device) { staged[q] = const_cast<void*>(src[q]); continue; }
+        hipError_t e = hipMalloc(&staged[q], nn * esz);
+        if (e != hipSuccess) return fail(FQSL_ERR_NOMEM, "hipMalloc(field staging)", e);
+        e = hipMemcpy(staged[q], src[q], nn * esz, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(FQSL_ERR_HIP, "hipMemcpy(field)", e);
+    }
+    const bool need32 = (layouts & (FQSL_LAYOUT_SOA32 | FQSL_LAYOUT_CELL32)) != 0;
+    for (int q = 0; q < 3; ++q) {
+        if (need32) {
+            hipError_t e = hipMalloc(&f->soa32[q], nn * 4);
+            if (e != hipSuccess) return fail(FQSL_ERR_NOMEM, "hipMalloc(soa32)", e);
+            if (dtype == FQSL_DTYPE_F32) e = hipMemcpy(f->soa32[q], staged[q], nn * 4, hipMemcpyDeviceToDevice);
+            else {
+                k_convert_to_f32<<<grid_for(nn, 256), 256>>>((const double*)staged[q], f->soa32[q], nn);
+                e = hipGetLastError();
+            }
+            if (e != hipSuccess) return fail(FQSL_ERR_HIP, "soa32 conversion", e);
+            f->bytes += nn * 4;
+        }
+        if (layouts & FQSL_LAYOUT_SOA64) {
+            hipError_t e = hipMalloc(&f->soa64[q], nn * 8);
+            if (e != hipSuccess) return fail(FQSL_ERR_NOMEM, "hipMalloc(soa64)", e);
+            if (dtype == FQSL_DTYPE_F64) e = hipMemcpy(f->soa64[q], staged[q], nn * 8, hipMemcpyDeviceToDevice);
+            else {
+                k_convert_to_f64<<<grid_for(nn, 256), 256>>>((const float*)staged[q], f->soa64[q], nn);
+                e = hipGetLastError();
+            }
+            if (e != hipSuccess) return fail(FQSL_ERR_HIP, "soa64 conversion", e);
+            f->bytes += nn * 8;
+        }
+    }
+    if (layouts & FQSL_LAYOUT_CELL32) {
+        const long ncell = (long)(nx - 1) * (ny - 1) * (nz - 1);
+        hipError_t e = hipMalloc(&f->cell32, ncell * 128);
+        if (e != hipSuccess) return fail(FQSL_ERR_NOMEM, "hipMalloc(cell32)", e);
+        k_build_cells<<<grid_for(ncell, 256), 256>>>(f->soa32[0], f->soa32[1], f->soa32[2], nx, ny, nz, f->cell32);
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(FQSL_ERR_HIP, "k_build_cells", e);
+        f->bytes += ncell * 128;
+    }
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(FQSL_ERR_HIP, "field build", e);
+    for (int q = 0; q < 3; ++q)
+        if (!on_device && staged[q]) (void)hipFree(staged[q]);
+    if (!(layouts & FQSL_LAYOUT_SOA32) && (layouts & FQSL_LAYOUT_CELL32)) {
+        // the float SoA copy was only needed to build the cells
+        for (int q = 0; q < 3; ++q) { (void)hipFree(f->soa32[q]); f->soa32[q] = nullptr; f->bytes -= nn * 4; }
+    }
+    (void)rc;
+    *out = f;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_field_destroy(fqsl_field* f) {
+    if (!f) return FQSL_OK;
+    field_free(f);
+    return FQSL_OK;
+}
+
+extern "C" size_t fqsl_field_bytes(const fqsl_field* f) { return f ? f->bytes : 0; }
+
+extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) {
+    if (!f) return set_err(FQSL_ERR_INVALID, "field is NULL");
+    if (nx) *nx = f->n[0];
+    if (ny) *ny = f->n[1];
+    if (nz) *nz = f->n[2];
+    return FQSL_OK;
+}
+
+// ------------------------------------------------------- device work queue
+// A ring of 64-bit queue counters; each trace call takes the next slot and
+// clears it with a fill kernel on its stream before the persistent launch.
+// Calls that are more than kQueueSlots launches apart never share a slot.
+static constexpr int kQueueSlots = 4096;
+struct DeviceQueues {
+    unsigned long long* slots = nullptr;
+    unsigned next = 0;
+};
+static std::mutex g_q_mu;
+static std::vector<DeviceQueues> g_queues;
+static std::vector<int> g_cu_count;
+
+static int queue_slot(unsigned long long** slot) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_q_mu);
+    if ((int)g_queues.size() <= dev) g_queues.resize(dev + 1);
+    DeviceQueues& d = g_queues[dev];
+    if (!d.slots) {
+        HIP_TRY(hipMalloc(&d.slots, kQueueSlots * sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(d.slots, 0, kQueueSlots * sizeof(unsigned long long)));
+    }
+    *slot = d.slots + (d.next++ % kQueueSlots);
+    return FQSL_OK;
+}
+
+static int cu_count() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(g_q_mu);
+    if ((int)g_cu_count.size() <= dev) g_cu_count.resize(dev + 1, 0);
+    if (!g_cu_count[dev]) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        g_cu_count[dev] = c;
+    }
+    return g_cu_count[dev];
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// Scalars the reference passes as one-element device arrays.  When `shape`
+// is non-NULL the kernel reads them on the device.
+struct DevScalars {
+    const int* shape;
+    const float* s_len;
+    const unsigned long long* N;
+    const float* tol_coef;
+    const bool* twist_flag;
+    double* twist;
+};
+
+struct Seeds {
+    const float* x;
+    const float* y;
+    const float* z;
+};
+
+struct ScottOut {
+    float *sx, *sy, *sz, *ex, *ey, *ez;
+    int32_t *fs, *fe;
+    float *btx, *bty, *btz, *q0, *qp;
+    double* len;
+    uint32_t* evals;
+};
+
+struct PlainOut {
+    float *sx, *sy, *sz, *ex, *ey, *ez;
+    int32_t *fs, *fe;
+    float *btx, *bty, *btz;
+    int32_t* bflag;
+    float *bsx, *bsy, *bsz, *bex, *bey, *bez;
+    double* len;
+    uint32_t* evals;
+};
+
+template <typename R, class F>
+__device__ __forceinline__ void scott_write(const ScottLine<R>& L, const F& f, unsigned long long i,
+                                            const ScottOut& o, const DevScalars& ds) {
+    double q0, qp;
+    scott_q<R>(L, f, &q0, &qp);
+    if (o.sx) { o.sx[i] = (float)L.ss.x[0]; o.sy[i] = (float)L.ss.x[1]; o.sz[i] = (float)L.ss.x[2]; }
+    if (o.ex) { o.ex[i] = (float)L.se.x[0]; o.ey[i] = (float)L.se.x[1]; o.ez[i] = (float)L.se.x[2]; }
+    if (o.fs) o.fs[i] = L.flag_s;
+    if (o.fe) o.fe[i] = L.flag_e;
+    if (o.q0) o.q0[i] = (float)q0;
+    if (o.qp) o.qp[i] = (float)qp;
+    if (o.len) o.len[i] = L.len;
+    if (o.evals) o.evals[i] = L.evals;
+    if (o.btx) {
+        float bt[3];
+        b_clamped<R>(f, L.p0, bt);
+        o.btx[i] = bt[0]; o.bty[i] = bt[1]; o.btz[i] = bt[2];
+    }
+    if (ds.twist_flag && ds.twist && ds.twist_flag[0]) ds.twist[i] = NAN;
+}
+
+template <class F>
+__device__ __forceinline__ void apply_scalars(F& f, ScottParams& P, unsigned long long& n, const DevScalars& ds) {
+    if (!ds.shape) return;
+    f.n[0] = ds.shape[0]; f.n[1] = ds.shape[1]; f.n[2] = ds.shape[2];
+    P.s0 = (double)ds.s_len[0];
+    P.tol = 1e-3 * (double)ds.tol_coef[0];
+    n = ds.N[0];
+}
+
+__device__ __forceinline__ unsigned long long bcast64(unsigned long long v, int src) {
+    const int lo = __shfl((int)(unsigned)(v & 0xffffffffull), src);
+    const int hi = __shfl((int)(unsigned)(v >> 32), src);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// Persistent Scott tracer: every lane owns one line; lanes whose line is done
+// take the next queue entries (one atomic per wavefront refill).  A wave exits
+// once the queue is drained and all its lanes are idle.
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                          ScottOut out, DevScalars ds, unsigned long long* queue) {
+    apply_scalars(f, P, n, ds);
+    ScottLine<R> L;
+    bool active = false;
+    bool drained = false;
+    unsigned long long idx = 0;
+    const int lane = __lane_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (;;) {
+        if (!drained) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                const int cnt = __popcll(need);
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(queue, (unsigned long long)cnt);
+                base = bcast64(base, leader);
+                if (base + (unsigned long long)cnt >= n) drained = true;
+                if (!active) {
+                    idx = base + (unsigned long long)__popcll(need & below);
+                    if (idx < n) {
+                        scott_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx]);
+                        if (L.done) scott_write<R>(L, f, idx, out, ds);
+                        else active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            scott_attempt<R>(L, f, P);
+            if (L.done) {
+                scott_write<R>(L, f, idx, out, ds);
+                active = false;
+            }
+        }
+    }
+}
+
+// One thread per line (grid-stride), the reference's mapping
+// (TraceBlineScott.cu:530-553).
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                        ScottOut out, DevScalars ds) {
+    apply_scalars(f, P, n, ds);
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
+        ScottLine<R> L;
+        scott_init<R>(L, f, P, s.x[i], s.y[i], s.z[i]);
+        while (!L.done) scott_attempt<R>(L, f, P);
+        scott_write<R>(L, f, i, out, ds);
+    }
+}
+
+// ------------------------------------------------------------ plain kernels
+template <typename R, class F>
+__device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, const PlainParams& P,
+                                            unsigned long long i, const PlainOut& o, const DevScalars& ds) {
+    const float xs[3] = {(float)L.xs[0], (float)L.xs[1], (float)L.xs[2]};
+    const float xe[3] = {(float)L.xe[0], (float)L.xe[1], (float)L.xe[2]};
+    if (o.sx) { o.sx[i] = xs[0]; o.sy[i] = xs[1]; o.sz[i] = xs[2]; }
+    if (o.ex) { o.ex[i] = xe[0]; o.ey[i] = xe[1]; o.ez[i] = xe[2]; }
+    if (o.fs) o.fs[i] = L.flag_s;
+    if (o.fe) o.fe[i] = L.flag_e;
+    if (o.len) o.len[i] = L.len_total;
+    if (o.evals) o.evals[i] = L.evals;
+    float b[3];
+    if (o.bsx) { b_clamped<R>(f, xs, b); o.bsx[i] = b[0]; o.bsy[i] = b[1]; o.bsz[i] = b[2]; }
+    if (o.bex) { b_clamped<R>(f, xe, b); o.bex[i] = b[0]; o.bey[i] = b[1]; o.bez[i] = b[2]; }
+    b_clamped<R>(f, L.p0, b);
+    if (o.btx) { o.btx[i] = b[0]; o.bty[i] = b[1]; o.btz[i] = b[2]; }
+    if (o.bflag) {
+        // TraceBlineAdaptive.cu:498-501, in float as the reference
+        const float bn = fabsf(b[0] * P.ncross[0] + b[1] * P.ncross[1] + b[2] * P.ncross[2]) * 100.f;
+        o.bflag[i] = bn < sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) ? 1 : 0;
+    }
+    if (ds.twist_flag && ds.twist && ds.twist_flag[0]) ds.twist[i] = NAN;
+}
+
+template <class F>
+__device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsigned long long& n,
+                                                    const DevScalars& ds, const float* ncross) {
+    if (!ds.shape) return;
+    f.n[0] = ds.shape[0]; f.n[1] = ds.shape[1]; f.n[2] = ds.shape[2];
+    P.s0 = (double)ds.s_len[0];
+    P.tol_coef = ds.tol_coef ? (double)ds.tol_coef[0] : 1.0;
+    n = ds.N[0];
+    P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];
+}
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Seeds s, unsigned long long n,
+                                                          PlainOut out, DevScalars ds, const float* ncross,
+                                                          unsigned long long* queue) {
+    apply_plain_scalars(f, P, n, ds, ncross);
+    PlainLine<R> L;
+    bool active = false;
+    bool drained = false;
+    unsigned long long idx = 0;
+    const int lane = __lane_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (;;) {
+        if (!drained) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const int leader = __ffsll((long long)need) - 1;
+                const int cnt = __popcll(need);
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(queue, (unsigned long long)cnt);
+                base = bcast64(base, leader);
+                if (base + (unsigned long long)cnt >= n) drained = true;
+                if (!active) {
+                    idx = base + (unsigned long long)__popcll(need & below);
+                    if (idx < n) {
+                        plain_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx]);
+                        if (L.done) plain_write<R>(L, f, P, idx, out, ds);
+                        else active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            plain_attempt<R>(L, f, P);
+            if (L.done) {
+                plain_write<R>(L, f, P, idx, out, ds);
+                active = false;
+            }
+        }
+    }
+}
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_plain_per_line(F f, PlainParams P, Seeds s, unsigned long long n,
+                                                        PlainOut out, DevScalars ds, const float* ncross) {
+    apply_plain_scalars(f, P, n, ds, ncross);
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
+        PlainLine<R> L;
+        plain_init<R>(L, f, P, s.x[i], s.y[i], s.z[i]);
+        while (!L.done) plain_attempt<R>(L, f, P);
+        plain_write<R>(L, f, P, i, out, ds);
+    }
+}
+
+// ------------------------------------------------------------- launching
+static int record_start(hipStream_t st) {
+    if (!g_ev0) {
+        HIP_TRY(hipEventCreate(&g_ev0));
+        HIP_TRY(hipEventCreate(&g_ev1));
+    }
+    HIP_TRY(hipEventRecord(g_ev0, st));
+    return FQSL_OK;
+}
+static int record_end(hipStream_t st) {
+    HIP_TRY(hipEventRecord(g_ev1, st));
+    g_ev_valid = true;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_last_kernel_ms(float* ms) {
+    if (!ms) return set_err(FQSL_ERR_INVALID, "ms is NULL");
+    if (!g_ev_valid) return set_err(FQSL_ERR_INVALID, "no trace kernel has been launched on this thread");
+    HIP_TRY(hipEventSynchronize(g_ev1));
+    HIP_TRY(hipEventElapsedTime(ms, g_ev0, g_ev1));
+    return FQSL_OK;
+}
+
+template <typename K>
+static int persistent_grid(K kernel, int* grid) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    if (per_cu < 1) per_cu = 1;
+    *grid = per_cu * cu_count();
+    return FQSL_OK;
+}
+
+template <typename R, class F>
+static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsigned long long n, const ScottOut& o,
+                        const DevScalars& ds, int schedule, hipStream_t st, unsigned long long n_hint) {
+    if (schedule == FQSL_SCHED_PERSISTENT) {
+        unsigned long long* q = nullptr;
+        int rc = queue_slot(&q);
+        if (rc) return rc;
+        int grid = 0;
+        rc = persistent_grid(k_scott_persistent<R, F>, &grid);
+        if (rc) return rc;
+        k_zero_u64<<<1, 1, 0, st>>>(q);
+        HIP_TRY(hipGetLastError());
+        rc = record_start(st);
+        if (rc) return rc;
+        k_scott_persistent<R, F><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, q);
+        HIP_TRY(hipGetLastError());
+        return record_end(st);
+    }
+    if (schedule == FQSL_SCHED_PER_LINE) {
+        int rc = record_start(st);
+        if (rc) return rc;
+        k_scott_per_line<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds);
+        HIP_TRY(hipGetLastError());
+        return record_end(st);
+    }
+    return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
+}
+
+template <typename R, class F>
+static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsigned long long n, const PlainOut& o,
+                        const DevScalars& ds, const float* ncross, int schedule, hipStream_t st,
+                        unsigned long long n_hint) {
+    if (schedule == FQSL_SCHED_PERSISTENT) {
+        unsigned long long* q = nullptr;
+        int rc = queue_slot(&q);
+        if (rc) return rc;
+        int grid = 0;
+        rc = persistent_grid(k_plain_persistent<R, F>, &grid);
+        if (rc) return rc;
+        k_zero_u64<<<1, 1, 0, st>>>(q);
+        HIP_TRY(hipGetLastError());
+        rc = record_start(st);
+        if (rc) return rc;
+        k_plain_persistent<R, F><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q);
+        HIP_TRY(hipGetLastError());
+        return record_end(st);
+    }
+    if (schedule == FQSL_SCHED_PER_LINE) {
+        int rc = record_start(st);
+        if (rc) return rc;
+        k_plain_per_line<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, ncross);
+        HIP_TRY(hipGetLastError());
+        return record_end(st);
+    }
+    return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
+}
+
+static int check_common(int method, int precision, int schedule) {
+    if (method != FQSL_METHOD_RKF45 && method != FQSL_METHOD_RK4) return set_err(FQSL_ERR_INVALID, "method %d", method);
+    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64)
+        return set_err(FQSL_ERR_INVALID, "precision %d", precision);
+    if (schedule != FQSL_SCHED_PERSISTENT && schedule != FQSL_SCHED_PER_LINE)
+        return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params* p, uint64_t n, const float* x,
+                                const float* y, const float* z, const fqsl_scott_out* out, void* stream) {
+    if (!field || !p || !out) return set_err(FQSL_ERR_INVALID, "NULL field/params/out");
+    int rc = check_common(p->method, p->precision, p->schedule);
+    if (rc) return rc;
+    if (n == 0) return FQSL_OK;
+    if (!x || !y || !z) return set_err(FQSL_ERR_INVALID, "NULL seed array");
+    if (!(p->step > 0) || !(p->step <= 100.0)) return set_err(FQSL_ERR_INVALID, "step %g outside (0, 100]", p->step);
+    if (p->method == FQSL_METHOD_RKF45 && !(p->tol > 0)) return set_err(FQSL_ERR_INVALID, "tol %g must be > 0", p->tol);
+    ScottParams P{p->method, p->step, p->tol};
+    Seeds s{x, y, z};
+    ScottOut o{out->start_x, out->start_y, out->start_z, out->end_x, out->end_y, out->end_z,
+               out->flag_start, out->flag_end, out->b_this_x, out->b_this_y, out->b_this_z,
+               out->q_0, out->q_perp, out->line_len, out->evals};
+    if ((o.sx && (!o.sy || !o.sz)) || (o.ex && (!o.ey || !o.ez)) || (o.btx && (!o.bty || !o.btz)))
+        return set_err(FQSL_ERR_INVALID, "x/y/z output arrays must be all set or all NULL");
+    DevScalars ds{};
+    hipStream_t st = (hipStream_t)stream;
+    if (p->precision == FQSL_PREC_FP64) {
+        if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
+        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {field->n[0], field->n[1], field->n[2]}};
+        return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n);
+    }
+    if (field->cell32) {
+        FieldCell32 f{field->cell32, {field->n[0], field->n[1], field->n[2]}};
+        return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
+    }
+    if (field->soa32[0]) {
+        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {field->n[0], field->n[1], field->n[2]}};
+        return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
+    }
+    return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
+extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n, const float* x,
+                                const float* y, const float* z, const fqsl_plain_out* out, void* stream) {
+    if (!field || !p || !out) return set_err(FQSL_ERR_INVALID, "NULL field/params/out");
+    int rc = check_common(p->method, p->precision, p->schedule);
+    if (rc) return rc;
+    if (n == 0) return FQSL_OK;
+    if (!x || !y || !z) return set_err(FQSL_ERR_INVALID, "NULL seed array");
+    if (!(p->step > 0) || !(p->step <= 100.0)) return set_err(FQSL_ERR_INVALID, "step %g outside (0, 100]", p->step);
+    PlainParams P;
+    P.method = p->method; P.s0 = p->step; P.tol_coef = p->tol_coef;
+    P.ncross[0] = p->ncross[0]; P.ncross[1] = p->ncross[1]; P.ncross[2] = p->ncross[2];
+    Seeds s{x, y, z};
+    PlainOut o{out->start_x, out->start_y, out->start_z, out->end_x, out->end_y, out->end_z,
+               out->flag_start, out->flag_end, out->b_this_x, out->b_this_y, out->b_this_z, out->b_flag,
+               out->b_start_x, out->b_start_y, out->b_start_z, out->b_end_x, out->b_end_y, out->b_end_z,
+               out->line_len, out->evals};
+    if ((o.sx && (!o.sy || !o.sz)) || (o.ex && (!o.ey || !o.ez)) || (o.btx && (!o.bty || !o.btz)) ||
+        (o.bsx && (!o.bsy || !o.bsz)) || (o.bex && (!o.bey || !o.bez)))
+        return set_err(FQSL_ERR_INVALID, "x/y/z output arrays must be all set or all NULL");
+    DevScalars ds{};
+    hipStream_t st = (hipStream_t)stream;
+    if (p->precision == FQSL_PREC_FP64) {
+        if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
+        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {field->n[0], field->n[1], field->n[2]}};
+        return launch_plain<double>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
+    }
+    if (field->cell32) {
+        FieldCell32 f{field->cell32, {field->n[0], field->n[1], field->n[2]}};
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
+    }
+    if (field->soa32[0]) {
+        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {field->n[0], field->n[1], field->n[2]}};
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
+    }
+    return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
+// ------------------------------------------- reference-shaped entry points
+// The seed count N lives on the device; the per-line launch needs a grid on
+// the host, so it covers up to kPerLineMaxBlocks * 256 threads with a
+// grid-stride loop (as the reference's fixed grids do).
+static constexpr int kPerLineMaxBlocks = 65536;
+
+extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* BshapeN, float* curB_x, float* curB_y,
+                                       float* curB_z, double* twist, bool* curB_flag, float* inp_x, float* inp_y,
+                                       float* inp_z, float* inp_cross_dir, float* start_x, float* start_y,
+                                       float* start_z, int* flag_start, float* end_x, float* end_y, float* end_z,
+                                       int* flag_end, float* B_this_x, float* B_this_y, float* B_this_z, float* q_0,
+                                       float* q_perp, float* s_len, unsigned long long* N, double* LineLen,
+                                       float* tol_coef, int precision, int schedule, void* stream) {
+    (void)curB_x; (void)curB_y; (void)curB_z; (void)inp_cross_dir;
+    if (!Bx || !By || !Bz || !BshapeN || !inp_x || !inp_y || !inp_z || !s_len || !N || !tol_coef)
+        return set_err(FQSL_ERR_INVALID, "NULL required argument");
+    int rc = check_common(FQSL_METHOD_RKF45, precision, schedule);
+    if (rc) return rc;
+    if (precision != FQSL_PREC_FP32C)
+        return set_err(FQSL_ERR_UNSUPPORTED, "the in-place float field supports FQSL_PREC_FP32C only");
+    ScottParams P{FQSL_METHOD_RKF45, 0.0, 0.0};
+    Seeds s{inp_x, inp_y, inp_z};
+    ScottOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
+               q_0, q_perp, LineLen, nullptr};
+    DevScalars ds{BshapeN, s_len, N, tol_coef, curB_flag, twist};
+    FieldSoA<float> f{Bx, By, Bz, {2, 2, 2}};
+    return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream,
+                               (unsigned long long)kPerLineMaxBlocks * 256);
+}
+
+extern "C" int fqsl_TraceAllBline(int method, float* Bx, float* By, float* Bz, int* BshapeN, float* curB_x,
+                                  float* curB_y, float* curB_z, double* twist, bool* curB_flag, float* inp_x,
+                                  float* inp_y, float* inp_z, float* inp_cross_dir, float* start_x, float* start_y,
+                                  float* start_z, int* flag_start, float* end_x, float* end_y, float* end_z,
+                                  int* flag_end, float* B_this_x, float* B_this_y, float* B_this_z, int* B_flag,
+                                  float* B_start_x, float* B_start_y, float* B_start_z, float* B_end_x,
+                                  float* B_end_y, float* B_end_z, float* s_len, unsigned long long* N,
+                                  double* LineLen, float* tol_coef, int precision, int schedule, void* stream) {
+    (void)curB_x; (void)curB_y; (void)curB_z;
+    if (!Bx || !By || !Bz || !BshapeN || !inp_x || !inp_y || !inp_z || !inp_cross_dir || !s_len || !N)
+        return set_err(FQSL_ERR_INVALID, "NULL required argument");
+    if (method == FQSL_METHOD_RKF45 && !tol_coef) return set_err(FQSL_ERR_INVALID, "RKF45 needs tol_coef");
+    int rc = check_common(method, precision, schedule);
+    if (rc) return rc;
+    if (precision != FQSL_PREC_FP32C)
+        return set_err(FQSL_ERR_UNSUPPORTED, "the in-place float field supports FQSL_PREC_FP32C only");
+    PlainParams P;
+    P.method = method; P.s0 = 0.0; P.tol_coef = 1.0;
+    P.ncross[0] = P.ncross[1] = P.ncross[2] = 0.f;
+    Seeds s{inp_x, inp_y, inp_z};
+    PlainOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
+               B_flag, B_start_x, B_start_y, B_start_z, B_end_x, B_end_y, B_end_z, LineLen, nullptr};
+    DevScalars ds{BshapeN, s_len, N, method == FQSL_METHOD_RKF45 ? tol_coef : nullptr, curB_flag, twist};
+    FieldSoA<float> f{Bx, By, Bz, {2, 2, 2}};
+    return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, (hipStream_t)stream,
+                               (unsigned long long)kPerLineMaxBlocks * 256);
+}
+
+// ---------------------------------------------- neighbour-difference assembly
+// Exit-face -> 2-D footpoint coordinates (FastQSL.py:76-84).
+__device__ __forceinline__ void footpoint2d(int flag, float x, float y, float z, float* X, float* Y) {
+    const int dim = (flag - 1) >= 0 ? (flag - 1) / 2 : -1;  // Python floor division
+    if (dim == 2) { *X = x; *Y = y; }
+    else if (dim == 1) { *X = z; *Y = x; }
+    else if (dim == 0) { *X = y; *Y = z; }
+    else { *X = 0.f; *Y = 0.f; }
+}
+
+#pragma clang fp contract(off)
+__global__ void k_qcalc_plane(int ny, int nx, const float* __restrict__ xe, const float* __restrict__ ye,
+                              const float* __restrict__ ze, const int32_t* __restrict__ fe,
+                              const float* __restrict__ xs, const float* __restrict__ ys,
+                              const float* __restrict__ zs, const int32_t* __restrict__ fs,
+                              const float* __restrict__ bzin, const float* __restrict__ bzout,
+                              const float* __restrict__ b0x, const float* __restrict__ b0y,
+                              const float* __restrict__ b0z, float cd0, float cd1, float cd2, float scale,
+                              float* __restrict__ q) {
+    const long inner = (long)(ny - 2) * (nx - 2);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < inner; t += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(t / (nx - 2)) + 1, c = (int)(t % (nx - 2)) + 1;
+        const long p = (long)r * nx + c;
+        const long pn[4] = {p + nx, p - nx, p + 1, p - 1};  // rows +-1 (axis 0), columns +-1
+        float X1[4], Y1[4], X2[4], Y2[4];
+        bool diff = false;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const long o = pn[m];
+            footpoint2d(fs[o], xs[o], ys[o], zs[o], &X1[m], &Y1[m]);
+            footpoint2d(fe[o], xe[o], ye[o], ze[o], &X2[m], &Y2[m]);
+            diff |= (fs[o] != fs[p]) | (fe[o] != fe[p]);
+        }
+        const float dx2xc = X2[0] - X2[1], dx2yc = X2[2] - X2[3];
+        const float dy2xc = Y2[0] - Y2[1], dy2yc = Y2[2] - Y2[3];
+        const float dx1xc = X1[0] - X1[1], dx1yc = X1[2] - X1[3];
+        const float dy1xc = Y1[0] - Y1[1], dy1yc = Y1[2] - Y1[3];
+        const float a = dx2xc * dy1yc - dx2yc * dy1xc;
+        const float b = dx2yc * dx1xc - dx2xc * dx1yc;
+        const float cc = dy2xc * dy1yc - dy2yc * dy1xc;
+        const float d = dy2yc * dx1xc - dy2xc * dx1yc;
+        const float bnorm = b0x[p] * cd0 + b0y[p] * cd1 + b0z[p] * cd2;
+        const float bn = fabsf(bnorm);
+        const float bnr = fabsf(bzin[p]) / (bn * bn) * fabsf(bzout[p]) * scale;
+        float Q = (a * a + b * b + cc * cc + d * d) * bnr;
+        if (Q < 1.0f) Q = 1.0f;
+        if (diff) Q = NAN;
+        q[t] = Q;
+    }
+}
+
+__global__ void k_qcut_scatter(unsigned long long m, const float* __restrict__ sx, const float* __restrict__ sy,
+                               const float* __restrict__ sz, const int32_t* __restrict__ fs,
+                               const float* __restrict__ ex, const float* __restrict__ ey,
+                               const float* __restrict__ ez, const int32_t* __restrict__ fe,
+                               const float* __restrict__ bzs, const float* __restrict__ bze, float scale,
+                               const uint64_t* __restrict__ pixel, float* __restrict__ qall) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < m;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        float X1[4], Y1[4], X2[4], Y2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long o = 4 * i + k;
+            footpoint2d(fs[o], sx[o], sy[o], sz[o], &X1[k], &Y1[k]);
+            footpoint2d(fe[o], ex[o], ey[o], ez[o], &X2[k], &Y2[k]);
+        }
+        const float dx2xc = X2[0] - X2[2], dx2yc = X2[1] - X2[3];
+        const float dy2xc = Y2[0] - Y2[2], dy2yc = Y2[1] - Y2[3];
+        const float dx1xc = X1[0] - X1[2], dx1yc = X1[1] - X1[3];
+        const float dy1xc = Y1[0] - Y1[2], dy1yc = Y1[1] - Y1[3];
+        const float a = dx2xc * dy1yc - dx2yc * dy1xc;
+        const float b = dx2yc * dx1xc - dx2xc * dx1yc;
+        const float c = dy2xc * dy1yc - dy2yc * dy1xc;
+        const float d = dy2yc * dx1xc - dy2xc * dx1yc;
+        const float bnr = fabsf(bze[i]) / fabsf(bzs[i]) * scale;
+        float Q = (a * a + b * b + c * c + d * d) * bnr;
+        if (Q < 1.0f) Q = 1.0f;
+        qall[pixel[i]] = Q;
+    }
+}
+#pragma clang fp contract(on)
+
+// CookPseudoLine compaction, order-preserving: per-block counts, one-block
+// exclusive scan, per-block write.
+static constexpr int kCompactBlock = 256;
+
+__global__ void k_pseudo_count(unsigned long long n, const int32_t* __restrict__ bflag, unsigned* __restrict__ counts) {
+    __shared__ unsigned s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const unsigned long long i = blockIdx.x * (unsigned long long)kCompactBlock + threadIdx.x;
+    const bool sel = i < n && bflag[i] == 1;
+    const unsigned long long m = __ballot(sel);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&s, (unsigned)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s;
+}
+
+__global__ void k_pseudo_scan(unsigned nblocks, unsigned* __restrict__ counts, unsigned long long* __restrict__ total) {
+    // single block of 1024 threads, each scans a contiguous chunk
+    __shared__ unsigned long long part[1024];
+    const unsigned per = (nblocks + 1023) / 1024;
+    const unsigned b0 = threadIdx.x * per;
+    unsigned long long sum = 0;
+    for (unsigned b = b0; b < b0 + per && b < nblocks; ++b) sum += counts[b];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int t = 0; t < 1024; ++t) { const unsigned long long v = part[t]; part[t] = run; run += v; }
+        *total = run;
+    }
+    __syncthreads();
+    unsigned long long run = part[threadIdx.x];
+    for (unsigned b = b0; b < b0 + per && b < nblocks; ++b) {
+        const unsigned v = counts[b];
+        counts[b] = (unsigned)run;  // exclusive offset (fits: offsets < 2^32 checked on the host)
+        run += v;
+    }
+}
+
+__global__ void k_pseudo_write(unsigned long long n, const int32_t* __restrict__ bflag, const float* __restrict__ xs,
+                               const float* __restrict__ ys, const float* __restrict__ zs,
+                               const int32_t* __restrict__ fls, const float* __restrict__ bzs,
+                               const float* __restrict__ bze, float st, const unsigned* __restrict__ offs,
+                               unsigned long long capacity, float* __restrict__ ox, float* __restrict__ oy,
+                               float* __restrict__ oz, float* __restrict__ obzs, float* __restrict__ obze,
+                               uint64_t* __restrict__ pix) {
+    __shared__ unsigned wave_base[kCompactBlock / 64];
+    const unsigned long long i = blockIdx.x * (unsigned long long)kCompactBlock + threadIdx.x;
+    const bool sel = i < n && bflag[i] == 1;
+    const unsigned long long m = __ballot(sel);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wave_base[w] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned run = 0;
+        for (int k = 0; k < kCompactBlock / 64; ++k) { const unsigned v = wave_base[k]; wave_base[k] = run; run += v; }
+    }
+    __syncthreads();
+    if (!sel) return;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long j = (unsigned long long)offs[blockIdx.x] + wave_base[w] + __popcll(m & below);
+    if (j >= capacity) return;
+    const float x = xs[i], y = ys[i], z = zs[i];
+    float cx[4] = {x, x, x, x}, cy[4] = {y, y, y, y}, cz[4] = {z, z, z, z};
+    const int f = fls[i];
+    const int dim = (f - 1) >= 0 ? (f - 1) / 2 : -1;
+    if (dim == 2) { cx[0] = x + st; cx[2] = x - st; cy[1] = y + st; cy[3] = y - st; }   // FastQSL.py:38-41
+    else if (dim == 1) { cx[0] = x + st; cx[2] = x - st; cz[1] = z + st; cz[3] = z - st; }  // :42-45
+    else if (dim == 0) { cy[0] = y + st; cy[2] = y - st; cz[1] = z + st; cz[3] = z - st; }  // :46-49
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { ox[4 * j + k] = cx[k]; oy[4 * j + k] = cy[k]; oz[4 * j + k] = cz[k]; }
+    obzs[j] = bzs[i];
+    obze[j] = bze[i];
+    pix[j] = i;
+}
+
+extern "C" int fqsl_qcalc_plane(int ny, int nx, const float* x_end, const float* y_end, const float* z_end,
+                                const int32_t* flag_end, const float* x_start, const float* y_start,
+                                const float* z_start, const int32_t* flag_start, const float* b_in_z,
+                                const float* b_out_z, const float* b0_x, const float* b0_y, const float* b0_z,
+                                double stride_step, const float* cross_dir_host, float* q, void* stream) {
+    if (ny < 3 || nx < 3) return set_err(FQSL_ERR_INVALID, "QCalcPlane needs at least 3x3 pixels (%d x %d)", ny, nx);
+    if (!x_end || !y_end || !z_end || !flag_end || !x_start || !y_start || !z_start || !flag_start || !b_in_z ||
+        !b_out_z || !b0_x || !b0_y || !b0_z || !q)
+        return set_err(FQSL_ERR_INVALID, "NULL array argument");
+    if (!(stride_step > 0)) return set_err(FQSL_ERR_INVALID, "stride_step must be > 0");
+    const float cd[3] = {cross_dir_host ? cross_dir_host[0] : 0.f, cross_dir_host ? cross_dir_host[1] : 0.f,
+                         cross_dir_host ? cross_dir_host[2] : 1.f};
+    // ((1./(2.*stride_step))**4.) in double, then float32 (FastQSL.py:113)
+    const float scale = (float)pow(1.0 / (2.0 * stride_step), 4.0);
+    const long inner = (long)(ny - 2) * (nx - 2);
+    k_qcalc_plane<<<grid_for(inner, 256), 256, 0, (hipStream_t)stream>>>(
+        ny, nx, x_end, y_end, z_end, flag_end, x_start, y_start, z_start, flag_start, b_in_z, b_out_z, b0_x, b0_y,
+        b0_z, cd[0], cd[1], cd[2], scale, q);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_pseudo_seeds(uint64_t n, const int32_t* b_flag, const float* x_start, const float* y_start,
+                                 const float* z_start, const int32_t* flag_start, const float* bz_start,
+                                 const float* bz_end, float stride, uint64_t capacity, float* seed_x, float* seed_y,
+                                 float* seed_z, float* bz0_start, float* bz0_end, uint64_t* pixel,
+                                 uint64_t* count_host, void* stream) {
+    if (!count_host) return set_err(FQSL_ERR_INVALID, "count_host is NULL");
+    *count_host = 0;
+    if (n == 0) return FQSL_OK;
+    if (!b_flag || !x_start || !y_start || !z_start || !flag_start || !bz_start || !bz_end)
+        return set_err(FQSL_ERR_INVALID, "NULL input array");
+    if (capacity && (!seed_x || !seed_y || !seed_z || !bz0_start || !bz0_end || !pixel))
+        return set_err(FQSL_ERR_INVALID, "NULL output array");
+    if (n > 0xffffffffull) return set_err(FQSL_ERR_INVALID, "pseudo-line compaction supports < 2^32 pixels");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned nblocks = (unsigned)((n + kCompactBlock - 1) / kCompactBlock);
+    unsigned* counts = nullptr;
+    unsigned long long* total = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&counts, nblocks * sizeof(unsigned) + 16, st));
+    total = (unsigned long long*)nullptr;
+    HIP_TRY(hipMallocAsync((void**)&total, sizeof(unsigned long long), st));
+    k_pseudo_count<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, counts);
+    k_pseudo_scan<<<1, 1024, 0, st>>>(nblocks, counts, total);
+    k_pseudo_write<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, x_start, y_start, z_start, flag_start, bz_start,
+                                                      bz_end, stride, counts, capacity, seed_x, seed_y, seed_z,
+                                                      bz0_start, bz0_end, pixel);
+    hipError_t e = hipGetLastError();
+    unsigned long long cnt = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, total, sizeof(cnt), hipMemcpyDeviceToHost, st);
+    (void)hipFreeAsync(counts, st);
+    (void)hipFreeAsync(total, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "pseudo-line compaction: %s", hipGetErrorString(e));
+    *count_host = cnt;
+    if (cnt > capacity)
+        return set_err(FQSL_ERR_INVALID, "%llu pseudo pixels exceed capacity %llu (outputs hold the first %llu)",
+                       (unsigned long long)cnt, (unsigned long long)capacity, (unsigned long long)capacity);
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_qcut_scatter(uint64_t m, const float* start_x, const float* start_y, const float* start_z,
+                                 const int32_t* flag_start, const float* end_x, const float* end_y,
+                                 const float* end_z, const int32_t* flag_end, const float* bz0_start,
+                                 const float* bz0_end, float stride, const uint64_t* pixel, float* q_all,
+                                 void* stream) {
+    if (m == 0) return FQSL_OK;
+    if (!start_x || !start_y || !start_z || !flag_start || !end_x || !end_y || !end_z || !flag_end || !bz0_start ||
+        !bz0_end || !pixel || !q_all)
+        return set_err(FQSL_ERR_INVALID, "NULL array argument");
+    if (!(stride > 0)) return set_err(FQSL_ERR_INVALID, "stride must be > 0");
+    // (1/stride/2)**4 in float32 (demo/Q_RKF45CuPy.ipynb cell [10])
+    const float scale = powf((1.0f / stride) / 2.0f, 4.0f);
+    k_qcut_scatter<<<grid_for((long)m, 256), 256, 0, (hipStream_t)stream>>>(
+        m, start_x, start_y, start_z, flag_start, end_x, end_y, end_z, flag_end, bz0_start, bz0_end, scale, pixel,
+        q_all);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}

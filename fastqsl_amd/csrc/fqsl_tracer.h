@@ -1,0 +1,973 @@
+// fqsl_tracer.h -- per-line field-line tracing math for gfx950.
+//
+// One struct per traced line holds its whole state; `attempt()` performs one
+// integrator attempt (step, controller, exit test, face landing).  The same
+// code runs under two schedulers in fqsl_kernels.hip: a persistent kernel in
+// which each lane of a wavefront refills itself from a device work queue, and
+// a one-thread-per-line kernel.
+//
+// Precision variants (template parameter R):
+//   float  -- field, stages, Jacobian products in FP32; the position is a
+//             compensated (hi, lo) float pair and every stage point is formed
+//             relative to its cell so interpolation weights keep ~1e-7 cell
+//             accuracy at any distance from the origin; arc length, the RKF45
+//             error norm, the step controller, landing and Q assembly in FP64.
+//   double -- everything in FP64.
+//
+// Reference behaviour reproduced here (file:line in peijin94/FastQSL):
+//   field indexing Arr[k*ny*nx + j*nx + i]         TraceBlineScott.cu:36-39
+//   Interp3dxyzn clamp rule (<=0 -> cell 0, w 0;
+//     >=N-1 -> cell N-2, w 1)                     TraceBlineScott.cu:120-131
+//   Interp3d clamp rule (into [1e-4, N-1-1e-4])    TraceBlineScott.cu:57-62
+//   grad_unit_vec_B difference quotient, h = 1e-3  TraceBlineScott.cu:204-226
+//   f_scott right-hand side                        TraceBlineScott.cu:230-249
+//   RKF45_Scott (Fehlberg, 5th-order advance,
+//     position-only error)                         TraceBlineScott.cu:256-290
+//   step controller                                TraceBlineScott.cu:410-419
+//   checkFlag                                      TraceBlineScott.cu:310-332
+//   TraceBlineScott driver and face landing        TraceBlineScott.cu:334-471
+//   Q assembly (q_0, q_perp)                       TraceBlineScott.cu:475-508
+//   plain RKF45 / RK4 / RK4_boundary / driver      TraceBlineAdaptive.cu:215-426,
+//                                                  TraceBline.cu:226-378
+//
+// The difference quotient of the reference, (b(x+h e_j) - b(x-h e_j)) / 2h on
+// the clamped piecewise-trilinear interpolant, is evaluated exactly but from
+// one 8-corner stencil: along each axis the interpolant is piecewise linear,
+// so B(x +- h e_j) = B(x) + a+ and B(x) - a- with a+, a- sums of the cell's
+// edge derivative over the covered length (plus the neighbour cell's when the
+// +-h sample crosses a grid plane, zero outside the box).  The quotient is
+// then formed without cancellation from D = a+ + a- and S = B+ + B-:
+//   b+ - b- = D (1/n+ + 1/n-)/2 - S (S.D) / (2 n+ n- (n+ + n-)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define FQSL_HD __host__ __device__ __forceinline__
+
+namespace fqsl {
+
+// ---------------------------------------------------------------- constants
+constexpr double kFdH = 1e-3;                // TraceBlineScott.cu:210
+constexpr int kScottStepLim = 10000;         // N_STEP_LIM, TraceBlineScott.cu:13
+constexpr double kScottLenRatio = 2.0;       // MAX_STEP_RATIO, TraceBlineScott.cu:11
+constexpr int kPlainRkf45StepLim = 500000;   // MAX_NUM_STEP, TraceBlineAdaptive.cu:13
+constexpr double kPlainRkf45LenRatio = 3.0;  // MAX_STEP_RATIO, TraceBlineAdaptive.cu:11
+constexpr int kPlainRk4StepLim = 100000;     // MAX_NUM_STEP, TraceBline.cu:12
+constexpr double kPlainRk4LenRatio = 2.0;    // MAX_STEP_RATIO, TraceBline.cu:10
+constexpr double kTolBase = 1e-3;            // TOL, TraceBlineAdaptive.cu:12
+constexpr int kAttemptCap = 1000000;         // hard bound on integrator attempts per line
+
+enum Method : int { kRKF45 = 0, kRK4 = 1 };
+
+// class codes written to flag_start / flag_end
+constexpr int kClassLimit = 0;
+constexpr int kClassBelow = -1;
+constexpr int kClassOther = 42;
+
+// Fehlberg 4(5), TraceBlineScott.cu:267-275
+struct Tab {
+    static constexpr double a21 = 1. / 4.;
+    static constexpr double a31 = 3. / 32., a32 = 9. / 32.;
+    static constexpr double a41 = 1932. / 2197., a42 = -7200. / 2197., a43 = 7296. / 2197.;
+    static constexpr double a51 = 439. / 216., a52 = -8., a53 = 3680. / 513., a54 = -845. / 4104.;
+    static constexpr double a61 = -8. / 27., a62 = 2., a63 = -3544. / 2565., a64 = 1859. / 4104.,
+                            a65 = -11. / 40.;
+    static constexpr double b1 = 16. / 135., b3 = 6656. / 12825., b4 = 28561. / 56430.,
+                            b5 = -9. / 50., b6 = 2. / 55.;
+    static constexpr double e1 = 1. / 360., e3 = -128. / 4275., e4 = -2197. / 75240.,
+                            e5 = 1. / 50., e6 = 2. / 55.;
+};
+
+// ------------------------------------------------------------ scalar helpers
+FQSL_HD float rsq(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __frsqrt_rn(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+FQSL_HD double rsq(double x) { return 1.0 / sqrt(x); }
+FQSL_HD float flr(float x) { return floorf(x); }
+FQSL_HD double flr(double x) { return floor(x); }
+
+// float -> int that keeps NaN and huge values in range (value semantics are
+// restored by the NaN branch of axis_setup)
+FQSL_HD int to_int(float v) { return (int)fminf(fmaxf(v, -1.0e8f), 1.0e8f); }
+FQSL_HD int to_int(double v) { return (int)fmin(fmax(v, -1.0e8), 1.0e8); }
+
+template <typename R> FQSL_HD R dot3(const R* a, const R* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// ----------------------------------------------------------------- fields
+// Plain SoA layout, x fastest (the reference's layout).
+template <typename T>
+struct FieldSoA {
+    const T* bx;
+    const T* by;
+    const T* bz;
+    int n[3];
+    // c[comp][a + 2b + 4c] = B_comp at node (i+a, j+b, k+c)
+    template <typename R>
+    FQSL_HD void load(int i, int j, int k, R c[3][8]) const {
+        const long nx = n[0], nxy = (long)n[0] * n[1];
+        const long base = (long)k * nxy + (long)j * nx + i;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long o = base + (q & 1) + ((q >> 1) & 1) * nx + ((q >> 2) & 1) * nxy;
+            c[0][q] = (R)bx[o];
+            c[1][q] = (R)by[o];
+            c[2][q] = (R)bz[o];
+        }
+    }
+};
+
+// Cell-stencil layout: each cell (i, j, k), i < nx-1 ..., stores its eight
+// corners for the three components contiguously as 8 float4 (128 B, one cache
+// line): [Bx c0-3][Bx c4-7][By c0-3][By c4-7][Bz c0-3][Bz c4-7][pad][pad],
+// corner q = a + 2b + 4c.  One interpolation is six 16-byte loads from one line.
+struct FieldCell32 {
+    const float4* cells;
+    int n[3];
+    template <typename R>
+    FQSL_HD void load(int i, int j, int k, R c[3][8]) const {
+        const long cx = n[0] - 1, cxy = (long)(n[0] - 1) * (n[1] - 1);
+        const float4* p = cells + 8 * ((long)k * cxy + (long)j * cx + i);
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) {
+            const float4 lo = p[2 * comp], hi = p[2 * comp + 1];
+            c[comp][0] = lo.x; c[comp][1] = lo.y; c[comp][2] = lo.z; c[comp][3] = lo.w;
+            c[comp][4] = hi.x; c[comp][5] = hi.y; c[comp][6] = hi.z; c[comp][7] = hi.w;
+        }
+    }
+};
+
+// ------------------------------------------------------ axis / stencil setup
+// Per-axis view of the point x = i + r (i = floor(x)): the interpolation cell
+// c and weight w after the Interp3dxyzn clamp rule, and the lengths that make
+// up a+ = B(x+h) - B(x) = ap d(c) + apn d(c+1) and
+//     a- = B(x) - B(x-h) = am d(c) + amp d(c-1)
+// on the clamped piecewise-linear interpolant.
+template <typename R>
+struct Axis {
+    int c;
+    R w, ap, apn, am, amp;
+};
+
+template <typename R>
+FQSL_HD Axis<R> axis_setup(int i, R r, int n) {
+    const R h = (R)kFdH;
+    Axis<R> a;
+    a.apn = 0;
+    a.amp = 0;
+    if (r != r) {  // NaN position: the value stays NaN
+        a.c = 0; a.w = r; a.ap = h; a.am = h;
+        return a;
+    }
+    if (i < 0 || (i == 0 && r == (R)0)) {  // x <= 0: node-0 plane
+        a.c = 0; a.w = 0; a.am = 0;
+        if (i == 0) a.ap = h;                                  // x == 0
+        else if (i == -1 && r > (R)1 - h) a.ap = (r - (R)1) + h;  // -h < x < 0
+        else a.ap = 0;                                         // x <= -h
+        return a;
+    }
+    if (i >= n - 1) {  // x >= N-1: node-(N-1) plane
+        a.c = n - 2; a.w = 1; a.ap = 0;
+        const R e = (R)(i - (n - 1)) + r;  // x - (N-1)
+        a.am = (e < h) ? (h - e) : (R)0;
+        return a;
+    }
+    a.c = i; a.w = r;  // 0 < x < N-1
+    if (r >= h) a.am = h;
+    else { a.am = r; if (i >= 1) a.amp = h - r; }
+    if (r + h <= (R)1) a.ap = h;
+    else { a.ap = (R)1 - r; if (i + 1 <= n - 2) a.apn = (r + h) - (R)1; }
+    return a;
+}
+
+// Trilinear value and the three edge derivatives (B(c+e_j) plane minus B(c)
+// plane, bilinear in the other two axes) of a loaded cell.
+template <typename R>
+FQSL_HD void cell_eval(const R c[3][8], R wx, R wy, R wz, R B[3], R d[3][3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const R dx00 = c[q][1] - c[q][0], dx10 = c[q][3] - c[q][2];
+        const R dx01 = c[q][5] - c[q][4], dx11 = c[q][7] - c[q][6];
+        const R e00 = c[q][0] + wx * dx00, e10 = c[q][2] + wx * dx10;
+        const R e01 = c[q][4] + wx * dx01, e11 = c[q][6] + wx * dx11;
+        const R dy0 = e10 - e00, dy1 = e11 - e01;
+        const R f0 = e00 + wy * dy0, f1 = e01 + wy * dy1;
+        const R gx0 = dx00 + wy * (dx10 - dx00), gx1 = dx01 + wy * (dx11 - dx01);
+        const R dz = f1 - f0;
+        B[q] = f0 + wz * dz;
+        d[2][q] = dz;
+        d[1][q] = dy0 + wz * (dy1 - dy0);
+        d[0][q] = gx0 + wz * (gx1 - gx0);
+    }
+}
+
+// Value of the trilinear interpolant only.
+template <typename R>
+FQSL_HD void cell_value(const R c[3][8], R wx, R wy, R wz, R B[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const R e00 = c[q][0] + wx * (c[q][1] - c[q][0]), e10 = c[q][2] + wx * (c[q][3] - c[q][2]);
+        const R e01 = c[q][4] + wx * (c[q][5] - c[q][4]), e11 = c[q][6] + wx * (c[q][7] - c[q][6]);
+        const R f0 = e00 + wy * (e10 - e00), f1 = e01 + wy * (e11 - e01);
+        B[q] = f0 + wz * (f1 - f0);
+    }
+}
+
+// (b(x+h e_j) - b(x-h e_j)) / 2h from B and the covered increments.
+template <typename R>
+FQSL_HD void fd_quotient(const R B[3], const R ap[3], const R am[3], R out[3]) {
+    R Bp[3], Bm[3], D[3], S[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        Bp[q] = B[q] + ap[q];
+        Bm[q] = B[q] - am[q];
+        D[q] = ap[q] + am[q];
+        S[q] = Bp[q] + Bm[q];
+    }
+    const R np2 = dot3(Bp, Bp), nm2 = dot3(Bm, Bm);
+    const R ip = rsq(np2), im = rsq(nm2);
+    const R np = np2 * ip, nm = nm2 * im;
+    const R inv2h = (R)(0.5 / kFdH);
+    const R t1 = (R)0.5 * (ip + im) * inv2h;
+    const R t2 = (R)0.5 * dot3(S, D) * ip * im / (np + nm) * inv2h;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[q] = D[q] * t1 - S[q] * t2;
+}
+
+// Raw stage position: x = i + r per axis (i = floor(x)).
+template <typename R>
+struct Pt {
+    int i[3];
+    R r[3];
+};
+
+template <typename R>
+FQSL_HD Pt<R> pt_from_double(const double* x) {
+    Pt<R> p;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double f = floor(x[a]);
+        p.i[a] = to_int(f);
+        p.r[a] = (R)(x[a] - f);
+    }
+    return p;
+}
+
+// Scott right-hand side at p: b = B/|B| and J[j][q] = (difference quotient of
+// b_q along axis j), i.e. grad_unit_vec_B (TraceBlineScott.cu:204-226).
+template <typename R, class F>
+FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
+    Axis<R> ax[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ax[a] = axis_setup<R>(p.i[a], p.r[a], f.n[a]);
+    R c[3][8];
+    f.template load<R>(ax[0].c, ax[1].c, ax[2].c, c);
+    R B[3], d[3][3];
+    cell_eval<R>(c, ax[0].w, ax[1].w, ax[2].w, B, d);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        R ap[3], am[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            ap[q] = ax[j].ap * d[j][q];
+            am[q] = ax[j].am * d[j][q];
+        }
+        if (ax[j].apn != (R)0 || ax[j].amp != (R)0) {  // the +-h sample crosses a grid plane
+            const int dj = ax[j].apn != (R)0 ? 1 : -1;
+            const R coef = ax[j].apn != (R)0 ? ax[j].apn : ax[j].amp;
+            int ci[3] = {ax[0].c, ax[1].c, ax[2].c};
+            ci[j] += dj;
+            R cn[3][8], Bn[3], dn[3][3];
+            f.template load<R>(ci[0], ci[1], ci[2], cn);
+            cell_eval<R>(cn, ax[0].w, ax[1].w, ax[2].w, Bn, dn);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (dj > 0) ap[q] += coef * dn[j][q];
+                else am[q] += coef * dn[j][q];
+            }
+            // a sample may cross on both sides only if 2h > 1 cell: impossible for h = 1e-3
+        }
+        fd_quotient<R>(B, ap, am, J[j]);
+    }
+    const R inb = rsq(dot3(B, B));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
+}
+
+// Unit vector b only (plain tracer RHS: Interp3dxyzn(..., true)).
+template <typename R, class F>
+FQSL_HD void unit_b(const F& f, const Pt<R>& p, R b[3]) {
+    int ci[3];
+    R w[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const Axis<R> ax = axis_setup<R>(p.i[a], p.r[a], f.n[a]);
+        ci[a] = ax.c; w[a] = ax.w;
+    }
+    R c[3][8], B[3];
+    f.template load<R>(ci[0], ci[1], ci[2], c);
+    cell_value<R>(c, w[0], w[1], w[2], B);
+    const R inb = rsq(dot3(B, B));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
+}
+
+// Raw B (Interp3dxyzn(..., false)) at a double position.
+template <typename R, class F>
+FQSL_HD void raw_b(const F& f, const double* x, double B[3]) {
+    const Pt<R> p = pt_from_double<R>(x);
+    int ci[3];
+    R w[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const Axis<R> ax = axis_setup<R>(p.i[a], p.r[a], f.n[a]);
+        ci[a] = ax.c; w[a] = ax.w;
+    }
+    R c[3][8], Bv[3];
+    f.template load<R>(ci[0], ci[1], ci[2], c);
+    cell_value<R>(c, w[0], w[1], w[2], Bv);
+    B[0] = Bv[0]; B[1] = Bv[1]; B[2] = Bv[2];
+}
+
+// Interp3d (TraceBlineScott.cu:47-93) for the three components: the point is
+// first pulled into [1e-4, N-1-1e-4] (NaN -> 1e-4), in float as the reference.
+template <typename R, class F>
+FQSL_HD void b_clamped(const F& f, const float* x, float out[3]) {
+    double q[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float v = x[a];
+        v = (v > 0.f) ? v : 0.0001f;
+        v = (v < (float)(f.n[a] - 1)) ? v : (float)((double)f.n[a] - 1 - 0.0001);
+        q[a] = v;
+    }
+    double B[3];
+    raw_b<R>(f, q, B);
+    out[0] = (float)B[0]; out[1] = (float)B[1]; out[2] = (float)B[2];
+}
+
+// checkFlag (TraceBlineScott.cu:310-332) in double.
+FQSL_HD int check_flag(const int* n, const double* p) {
+    const double X = p[0], Y = p[1], Z = p[2];
+    const double mx = n[0] - 1., my = n[1] - 1., mz = n[2] - 1.;
+    if (X >= 0. && Y > 0. && Z >= 0. && X <= mx && Y <= my && Z <= mz) return 0;
+    int fl = kClassOther;
+    if (X < 0. && Y >= 0. && Z >= 0. && X < mx && Y < my && Z < mz) fl = 1;
+    if (X >= 0. && Y >= 0. && Z >= 0. && X >= mx && Y < my && Z < mz) fl = 2;
+    if (X >= 0. && Y < 0. && Z >= 0. && X < mx && Y < my && Z < mz) fl = 3;
+    if (X >= 0. && Y >= 0. && Z >= 0. && X < mx && Y >= my && Z < mz) fl = 4;
+    if (X >= 0. && Y >= 0. && Z < 0. && X < mx && Y < my && Z < mz) fl = 5;
+    if (X >= 0. && Y >= 0. && Z >= 0. && X < mx && Y < my && Z >= mz) fl = 6;
+    return fl;
+}
+
+// ----------------------------------------------------------------- positions
+// Position of a line: compensated float pair or plain double.
+template <typename R> struct Pos;
+
+template <>
+struct Pos<float> {
+    float hi[3], lo[3];
+    FQSL_HD void set(const double* x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            hi[a] = (float)x[a];
+            lo[a] = (float)(x[a] - (double)hi[a]);
+        }
+    }
+    FQSL_HD void set_f(const float* x) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { hi[a] = x[a]; lo[a] = 0.f; }
+    }
+    FQSL_HD double get(int a) const { return (double)hi[a] + (double)lo[a]; }
+    // stage point x + delta, formed relative to floor(hi) so the weight keeps
+    // full float resolution
+    FQSL_HD Pt<float> stage(const float* delta) const {
+        Pt<float> p;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float fh = floorf(hi[a]);
+            const float u = (hi[a] - fh) + (lo[a] + delta[a]);
+            const float fu = floorf(u);
+            p.i[a] = to_int(fh) + to_int(fu);
+            p.r[a] = u - fu;
+        }
+        return p;
+    }
+    // x += inc with a compensated (two-sum) update
+    FQSL_HD void add(const float* inc) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float t = lo[a] + inc[a];
+            const float s = hi[a] + t;
+            const float bb = s - hi[a];
+            const float err = (hi[a] - (s - bb)) + (t - bb);
+            hi[a] = s;
+            lo[a] = err;
+        }
+    }
+};
+
+template <>
+struct Pos<double> {
+    double x[3];
+    FQSL_HD void set(const double* v) { x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; }
+    FQSL_HD void set_f(const float* v) { x[0] = v[0]; x[1] = v[1]; x[2] = v[2]; }
+    FQSL_HD double get(int a) const { return x[a]; }
+    FQSL_HD Pt<double> stage(const double* delta) const {
+        Pt<double> p;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double v = x[a] + delta[a];
+            const double f = floor(v);
+            p.i[a] = to_int(f);
+            p.r[a] = v - f;
+        }
+        return p;
+    }
+    FQSL_HD void add(const double* inc) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x[a] += inc[a];
+    }
+};
+
+// ------------------------------------------------------------- Scott tracer
+struct ScottParams {
+    int method;     // kRKF45 or kRK4
+    double s0;      // initial (RK4: fixed) step, cells
+    double tol;     // RKF45 absolute per-step tolerance, cells
+};
+
+template <typename R>
+struct ScottSide {
+    double x[3];
+    R u[3], v[3];
+};
+
+// State of one Scott line (TraceBlineScott, TraceBlineScott.cu:334-519).
+template <typename R>
+struct ScottLine {
+    // seed
+    float p0[3];
+    R u0[3], v0[3];
+    double nb0;
+    float b0z;
+    // current trace
+    Pos<R> x;
+    R u[3], v[3];
+    double s, len, len_lim;
+    int steps, attempts, dir;  // dir: -1 backward (first), +1 forward
+    int flag_s, flag_e;
+    bool done, aborted;
+    uint32_t evals;
+    ScottSide<R> ss, se;  // start (backward) and end (forward) footpoint states
+};
+
+// Record the current state as the footpoint of the active side.
+template <typename R>
+FQSL_HD void scott_record(ScottLine<R>& L, const double* x, const R* u, const R* v, int flag) {
+    ScottSide<R>& S = L.dir < 0 ? L.ss : L.se;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { S.x[a] = x[a]; S.u[a] = u[a]; S.v[a] = v[a]; }
+    if (L.dir < 0) L.flag_s = flag; else L.flag_e = flag;
+}
+
+template <typename R>
+FQSL_HD bool scott_can_step(const ScottLine<R>& L) {
+    return L.len < L.len_lim && L.steps < kScottStepLim && L.attempts < kAttemptCap;
+}
+
+// Start direction L.dir from the seed; handles the z0 shortcut
+// (TraceBlineScott.cu:396-401) and directions that cannot take a step
+// because the shared length/step budget is spent.  Advances to the next
+// direction or marks the line done as needed.
+template <typename R>
+FQSL_HD void scott_begin_dir(ScottLine<R>& L, const ScottParams& P) {
+    for (;;) {
+        const bool z0 = L.p0[2] < 1e-6f;
+        L.x.set_f(L.p0);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { L.u[a] = L.u0[a]; L.v[a] = L.v0[a]; }
+        const double px[3] = {L.p0[0], L.p0[1], L.p0[2]};
+        if (z0 && (double)L.b0z * L.dir < 0) {
+            scott_record(L, px, L.u0, L.v0, 5);
+        } else {
+            L.s = fabs(L.s);
+            if (scott_can_step(L)) return;  // trace this direction
+            scott_record(L, px, L.u0, L.v0, kClassLimit);
+        }
+        if (L.dir < 0) { L.dir = 1; continue; }
+        L.done = true;
+        return;
+    }
+}
+
+template <typename R, class F>
+FQSL_HD void scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float px, float py, float pz) {
+    L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
+    L.len = 0.0;
+    L.len_lim = kScottLenRatio * (f.n[0] + f.n[1] + f.n[2]);
+    L.steps = 0; L.attempts = 0; L.evals = 0;
+    L.s = P.s0;
+    L.flag_s = L.flag_e = kClassLimit;
+    L.done = false; L.aborted = false;
+    L.dir = -1;
+    const double p0d[3] = {px, py, pz};
+    if (pz < -1e-8f) {  // TraceBlineScott.cu:366
+        L.flag_s = L.flag_e = kClassBelow;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { L.ss.x[a] = p0d[a]; L.se.x[a] = p0d[a]; }
+        L.done = true;
+        return;
+    }
+    double B0[3];
+    raw_b<R>(f, p0d, B0);
+    R Bv[3] = {(R)B0[0], (R)B0[1], (R)B0[2]};
+    L.nb0 = sqrt(B0[0] * B0[0] + B0[1] * B0[1] + B0[2] * B0[2]);
+    const R inb = rsq(dot3(Bv, Bv));
+    R b0[3] = {Bv[0] * inb, Bv[1] * inb, Bv[2] * inb};
+    L.b0z = (float)b0[2];
+    // v0 = normalize((b.y, -b.x, 0) - ((b.y,-b.x,0).b) b); u0 = normalize(b x v0)
+    // (TraceBlineScott.cu:380-386)
+    R v0[3] = {b0[1], -b0[0], (R)0};
+    const R dv = dot3(v0, b0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v0[a] -= dv * b0[a];
+    const R iv = rsq(dot3(v0, v0));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v0[a] *= iv;
+    R u0[3] = {b0[1] * v0[2] - b0[2] * v0[1], b0[2] * v0[0] - b0[0] * v0[2], b0[0] * v0[1] - b0[1] * v0[0]};
+    const R iu = rsq(dot3(u0, u0));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { L.u0[a] = u0[a] * iu; L.v0[a] = v0[a]; }
+    scott_begin_dir(L, P);
+}
+
+// Right-hand side for a stage: k = s * f(y + delta), returns the 9-vector.
+template <typename R, class F>
+FQSL_HD void scott_stage(const F& f, const ScottLine<R>& L, const R* dx, const R* du, const R* dv,
+                         R sc, R k[9]) {
+    const Pt<R> p = L.x.stage(dx);
+    R b[3], J[3][3];
+    scott_eval<R>(f, p, b, J);
+    R uu[3], vv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { uu[a] = L.u[a] + du[a]; vv[a] = L.v[a] + dv[a]; }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        k[q] = sc * b[q];
+        k[3 + q] = sc * (uu[0] * J[0][q] + uu[1] * J[1][q] + uu[2] * J[2][q]);
+        k[6 + q] = sc * (vv[0] * J[0][q] + vv[1] * J[1][q] + vv[2] * J[2][q]);
+    }
+}
+
+// One integrator attempt (one pass of the while loop at TraceBlineScott.cu:407-470).
+template <typename R, class F>
+FQSL_HD void scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P) {
+    const R sc = (R)(L.s * L.dir);
+    R inc[9];
+    double err = 0.0;
+    L.attempts += 1;
+    if (P.method == kRKF45) {
+        R k1[9], k2[9], k3[9], k4[9], k5[9], k6[9], t[9];
+        const R zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        scott_stage<R>(f, L, zero, zero + 3, zero + 6, sc, k1);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = (R)Tab::a21 * k1[c];
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k2);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = (R)Tab::a31 * k1[c] + (R)Tab::a32 * k2[c];
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k3);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) t[c] = (R)Tab::a41 * k1[c] + (R)Tab::a42 * k2[c] + (R)Tab::a43 * k3[c];
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k4);
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            t[c] = (R)Tab::a51 * k1[c] + (R)Tab::a52 * k2[c] + (R)Tab::a53 * k3[c] + (R)Tab::a54 * k4[c];
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k5);
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            t[c] = (R)Tab::a61 * k1[c] + (R)Tab::a62 * k2[c] + (R)Tab::a63 * k3[c] + (R)Tab::a64 * k4[c] +
+                   (R)Tab::a65 * k5[c];
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k6);
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            inc[c] = (R)Tab::b1 * k1[c] + (R)Tab::b3 * k3[c] + (R)Tab::b4 * k4[c] + (R)Tab::b5 * k5[c] +
+                     (R)Tab::b6 * k6[c];
+        double e2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double e = Tab::e1 * (double)k1[c] + Tab::e3 * (double)k3[c] + Tab::e4 * (double)k4[c] +
+                             Tab::e5 * (double)k5[c] + Tab::e6 * (double)k6[c];
+            e2 += e * e;
+        }
+        err = sqrt(e2);
+        L.evals += 6;
+        // step controller (TraceBlineScott.cu:413-419), FP64
+        const double scale = pow(P.tol / err / 11.09, 0.2);
+        if (scale < 0.618) {
+            L.s = L.s * 0.618;
+            if (!scott_can_step(L)) {  // attempt budget exhausted: limit exit
+                double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
+                scott_record(L, xa, L.u, L.v, kClassLimit);
+                if (L.dir < 0) { L.dir = 1; scott_begin_dir(L, P); } else L.done = true;
+            }
+            return;
+        }
+        L.s = L.s * scale;
+        if (L.s > 100.) L.s = 100.;
+        if (L.s < 1. / 10.) L.s = 1. / 10.;
+    } else {
+        // RK4 (TraceBline.cu:226-234) on the 9-vector, fixed step
+        R k[9], t[9];
+        const R zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const R half = (R)0.5;
+        scott_stage<R>(f, L, zero, zero + 3, zero + 6, sc, k);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) { inc[c] = k[c]; t[c] = half * k[c]; }
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) { inc[c] += (R)2 * k[c]; t[c] = half * k[c]; }
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) { inc[c] += (R)2 * k[c]; t[c] = k[c]; }
+        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) inc[c] = (inc[c] + k[c]) * (R)(1.0 / 6.0);
+        L.evals += 4;
+    }
+
+    // accepted step: a = current state, b = a + inc
+    const double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
+    Pos<R> xb = L.x;
+    xb.add(inc);
+    const double xbv[3] = {xb.get(0), xb.get(1), xb.get(2)};
+    R ub[3], vb[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { ub[a] = L.u[a] + inc[3 + a]; vb[a] = L.v[a] + inc[6 + a]; }
+    const double dL = sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
+    L.len += dL;
+    const int flag = check_flag(f.n, xbv);
+    if (flag > 0) {
+        L.len -= dL;
+        if (flag <= 6) {
+            const int dim = (flag - 1) / 2;
+            const double p1 = xa[dim], p2 = xbv[dim];
+            if (fabs(p1 - p2) > 0) {
+                // linear landing of the 9-vector on the face (TraceBlineScott.cu:442-448);
+                // max faces land on index N
+                const double pm = (flag % 2 == 1) ? 0.0 : (double)f.n[dim];
+                const double wa = (p2 - pm) / (p2 - p1), wb = (pm - p1) / (p2 - p1);
+                double xl[3];
+                R ul[3], vl[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    xl[a] = (xa[a] * (p2 - pm) + xbv[a] * (pm - p1)) / (p2 - p1);
+                    ul[a] = (R)((double)L.u[a] * wa + (double)ub[a] * wb);
+                    vl[a] = (R)((double)L.v[a] * wa + (double)vb[a] * wb);
+                }
+                scott_record(L, xl, ul, vl, flag);
+                // arc length to the face with b at the last in-box state
+                R ba[3];
+                const R zero3[3] = {0, 0, 0};
+                unit_b<R>(f, L.x.stage(zero3), ba);
+                L.len += fabs(pm - p1) / (1e-4 + fabs((double)ba[dim]));
+            } else {
+                scott_record(L, xa, L.u, L.v, flag);
+            }
+            L.steps += 1;
+            if (L.dir < 0) { L.dir = 1; scott_begin_dir(L, P); } else L.done = true;
+        } else {
+            // class 42 (TraceBlineScott.cu:460-465): q = 0, the line stops here
+            scott_record(L, xa, L.u, L.v, kClassOther);
+            if (L.dir < 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) L.se.x[a] = L.p0[a];
+                L.flag_e = kClassOther;
+            }
+            L.aborted = true;
+            L.done = true;
+        }
+        return;
+    }
+    L.x = xb;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { L.u[a] = ub[a]; L.v[a] = vb[a]; }
+    L.steps += 1;
+    if (!scott_can_step(L)) {  // length / step limit: class 0, last state
+        scott_record(L, xbv, ub, vb, kClassLimit);
+        if (L.dir < 0) { L.dir = 1; scott_begin_dir(L, P); } else L.done = true;
+    }
+}
+
+// Q assembly (TraceBlineScott.cu:475-508) in FP64.
+template <typename R, class F>
+FQSL_HD void scott_q(const ScottLine<R>& L, const F& f, double* q0, double* qperp) {
+    if (L.aborted) { *q0 = 0.0; *qperp = 0.0; return; }
+    if (L.flag_s < 1 || L.flag_s > 6 || L.flag_e < 1 || L.flag_e > 6) {
+        *q0 = NAN; *qperp = NAN; return;
+    }
+    double Bs[3], Be[3];
+    raw_b<R>(f, L.ss.x, Bs);
+    raw_b<R>(f, L.se.x, Be);
+    const int ds = (L.flag_s - 1) / 2, de = (L.flag_e - 1) / 2;
+    double us[3], vs[3], ue[3], ve[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        us[a] = L.ss.u[a]; vs[a] = L.ss.v[a]; ue[a] = L.se.u[a]; ve[a] = L.se.v[a];
+    }
+    double us1[3], vs1[3], ue1[3], ve1[3];
+    const double BNs = Bs[ds], BNe = Be[de];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        us1[a] = us[a] - us[ds] / BNs * Bs[a];
+        vs1[a] = vs[a] - vs[ds] / BNs * Bs[a];
+        ue1[a] = ue[a] - ue[de] / BNe * Be[a];
+        ve1[a] = ve[a] - ve[de] / BNe * Be[a];
+    }
+    const double nb2 = L.nb0 * L.nb0;
+    double num = dot3(ue1, ue1) * dot3(vs1, vs1) + dot3(us1, us1) * dot3(ve1, ve1) -
+                 2.0 * dot3(ue1, ve1) * dot3(us1, vs1);
+    *q0 = fabs(num) / fabs(nb2 / (BNs * BNe));
+    const double nbs = sqrt(dot3(Bs, Bs)), nbe = sqrt(dot3(Be, Be));
+    const double due = dot3(ue, Be) / nbe, dve = dot3(ve, Be) / nbe;
+    const double dus = dot3(us, Bs) / nbs, dvs = dot3(vs, Bs) / nbs;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ue1[a] = ue[a] - due * (Be[a] / nbe);
+        ve1[a] = ve[a] - dve * (Be[a] / nbe);
+        us1[a] = us[a] - dus * (Bs[a] / nbs);
+        vs1[a] = vs[a] - dvs * (Bs[a] / nbs);
+    }
+    num = dot3(ue1, ue1) * dot3(vs1, vs1) + dot3(us1, us1) * dot3(ve1, ve1) -
+          2.0 * dot3(ue1, ve1) * dot3(us1, vs1);
+    *qperp = fabs(num) / (nb2 / (nbs * nbe));
+}
+
+// ------------------------------------------------------------- plain tracer
+struct PlainParams {
+    int method;        // kRKF45 or kRK4
+    double s0;         // initial step (each direction starts from it)
+    double tol_coef;   // RKF45 tolerance coefficient
+    float ncross[3];   // plane normal for the tolerance rule and B_flag
+};
+
+// State of one plain line (TraceBlineAdap / TraceBline, TraceBlineAdaptive.cu:333-426).
+template <typename R>
+struct PlainLine {
+    float p0[3];
+    Pos<R> x;
+    double s, len, len_lim, tol, len_total;
+    int steps, attempts, dir, step_lim;  // dir +1 forward (first), -1 backward
+    int flag_e, flag_s;
+    double xe[3], xs[3];
+    bool done;
+    uint32_t evals;
+};
+
+template <typename R>
+FQSL_HD bool plain_can_step(const PlainLine<R>& L) {
+    return L.len < L.len_lim && L.steps < L.step_lim && L.attempts < kAttemptCap;
+}
+
+template <typename R>
+FQSL_HD void plain_record(PlainLine<R>& L, const double* x, int flag) {
+    double* d = L.dir > 0 ? L.xe : L.xs;
+    d[0] = x[0]; d[1] = x[1]; d[2] = x[2];
+    if (L.dir > 0) L.flag_e = flag; else L.flag_s = flag;
+}
+
+// Begin the direction L.dir from the seed with a fresh step and budget.
+template <typename R, class F>
+FQSL_HD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P) {
+    for (;;) {
+        L.x.set_f(L.p0);
+        L.s = P.s0;
+        L.len = 0.0;
+        L.steps = 0;
+        L.attempts = 0;
+        if (plain_can_step(L)) return;
+        const double px[3] = {L.p0[0], L.p0[1], L.p0[2]};
+        plain_record(L, px, kClassLimit);
+        if (L.dir > 0) { L.dir = -1; continue; }
+        L.done = true;
+        return;
+    }
+}
+
+template <typename R, class F>
+FQSL_HD void plain_init(PlainLine<R>& L, const F& f, const PlainParams& P, float px, float py, float pz) {
+    L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
+    const bool rkf = P.method == kRKF45;
+    L.len_lim = (rkf ? kPlainRkf45LenRatio : kPlainRk4LenRatio) * (f.n[0] + f.n[1] + f.n[2]);
+    L.step_lim = rkf ? kPlainRkf45StepLim : kPlainRk4StepLim;
+    L.len_total = 0.0;
+    L.evals = 0;
+    L.done = false;
+    L.flag_e = L.flag_s = kClassLimit;
+    L.tol = 0.0;
+    if (rkf) {  // TraceBlineAdaptive.cu:354-359
+        const double p0d[3] = {px, py, pz};
+        R b0[3];
+        unit_b<R>(f, pt_from_double<R>(p0d), b0);
+        const double bn = fabs((double)b0[0] * P.ncross[0] + (double)b0[1] * P.ncross[1] +
+                               (double)b0[2] * P.ncross[2]);
+        L.tol = (bn <= 0.05 ? kTolBase / 8e3 : kTolBase * pow(bn, 1.5)) * P.tol_coef;
+    }
+    L.dir = 1;
+    plain_begin_dir(L, f, P);
+}
+
+template <typename R, class F>
+FQSL_HD void plain_finish_dir(PlainLine<R>& L, const F& f, const PlainParams& P) {
+    L.len_total += L.len;
+    if (L.dir > 0) { L.dir = -1; plain_begin_dir(L, f, P); } else L.done = true;
+}
+
+// RK4_boundary (TraceBline.cu:253-274): integrate dx/dx_dim = b / b_dim over s.
+template <typename R, class F>
+FQSL_HD void rk4_boundary(const F& f, const double* p, double s, int dim, double out[3]) {
+    double k[4][3], t[3];
+    const double cf[4] = {0.0, 0.5, 0.5, 1.0};
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) t[a] = st == 0 ? p[a] : p[a] + s * k[st - 1][a] * cf[st];
+        R b[3];
+        unit_b<R>(f, pt_from_double<R>(t), b);
+        const double bd = (double)b[dim];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) k[st][a] = (double)b[a] / bd;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = p[a] + (1. / 6.) * s * (k[0][a] + 2.0 * k[1][a] + 2.0 * k[2][a] + k[3][a]);
+}
+
+template <typename R, class F>
+FQSL_HD void plain_attempt(PlainLine<R>& L, const F& f, const PlainParams& P) {
+    const R sc = (R)(L.s * L.dir);
+    R inc[3];
+    L.attempts += 1;
+    if (P.method == kRKF45) {
+        R k[6][3], t[3];
+        const R zero[3] = {0, 0, 0};
+        unit_b<R>(f, L.x.stage(zero), k[0]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) k[0][a] *= sc;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) t[a] = (R)Tab::a21 * k[0][a];
+        unit_b<R>(f, L.x.stage(t), k[1]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { k[1][a] *= sc; t[a] = (R)Tab::a31 * k[0][a] + (R)Tab::a32 * k[1][a]; }
+        unit_b<R>(f, L.x.stage(t), k[2]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            k[2][a] *= sc;
+            t[a] = (R)Tab::a41 * k[0][a] + (R)Tab::a42 * k[1][a] + (R)Tab::a43 * k[2][a];
+        }
+        unit_b<R>(f, L.x.stage(t), k[3]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            k[3][a] *= sc;
+            t[a] = (R)Tab::a51 * k[0][a] + (R)Tab::a52 * k[1][a] + (R)Tab::a53 * k[2][a] + (R)Tab::a54 * k[3][a];
+        }
+        unit_b<R>(f, L.x.stage(t), k[4]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            k[4][a] *= sc;
+            t[a] = (R)Tab::a61 * k[0][a] + (R)Tab::a62 * k[1][a] + (R)Tab::a63 * k[2][a] + (R)Tab::a64 * k[3][a] +
+                   (R)Tab::a65 * k[4][a];
+        }
+        unit_b<R>(f, L.x.stage(t), k[5]);
+        double e2 = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            k[5][a] *= sc;
+            inc[a] = (R)Tab::b1 * k[0][a] + (R)Tab::b3 * k[2][a] + (R)Tab::b4 * k[3][a] + (R)Tab::b5 * k[4][a] +
+                     (R)Tab::b6 * k[5][a];
+            const double e = Tab::e1 * (double)k[0][a] + Tab::e3 * (double)k[2][a] + Tab::e4 * (double)k[3][a] +
+                             Tab::e5 * (double)k[4][a] + Tab::e6 * (double)k[5][a];
+            e2 += e * e;
+        }
+        L.evals += 6;
+        const double scale = pow(L.tol / sqrt(e2) / 11.09, 0.2);  // TraceBlineAdaptive.cu:365-370
+        if (scale < 0.618) {
+            L.s = L.s * 0.618;
+            if (!plain_can_step(L)) {
+                const double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
+                plain_record(L, xa, kClassLimit);
+                plain_finish_dir(L, f, P);
+            }
+            return;
+        }
+        L.s = L.s * scale;
+        if (L.s > 100.) L.s = 100.;
+        if (L.s < 1. / 10.) L.s = 1. / 10.;
+    } else {
+        R k[3], t[3];
+        const R zero[3] = {0, 0, 0};
+        unit_b<R>(f, L.x.stage(zero), k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { inc[a] = k[a]; t[a] = sc * k[a] * (R)0.5; }
+        unit_b<R>(f, L.x.stage(t), k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { inc[a] += (R)2 * k[a]; t[a] = sc * k[a] * (R)0.5; }
+        unit_b<R>(f, L.x.stage(t), k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { inc[a] += (R)2 * k[a]; t[a] = sc * k[a]; }
+        unit_b<R>(f, L.x.stage(t), k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) inc[a] = (R)(1.0 / 6.0) * sc * (inc[a] + k[a]);
+        L.evals += 4;
+    }
+    const double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
+    Pos<R> xb = L.x;
+    xb.add(inc);
+    const double xbv[3] = {xb.get(0), xb.get(1), xb.get(2)};
+    const double dl = sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
+    L.len += dl;
+    const int flag = check_flag(f.n, xbv);
+    if (flag > 0) {
+        L.len -= dl;
+        if (flag <= 6) {
+            const int dim = (flag - 1) / 2;
+            const double pa = xa[dim], pb = xbv[dim];
+            if (fabs(pa - pb) > 1e-3) {  // TraceBlineAdaptive.cu:391-405
+                const double pm = (flag % 2 == 1) ? 0.0 : (double)f.n[dim];
+                R b1[3], b2[3];
+                unit_b<R>(f, pt_from_double<R>(xa), b1);
+                unit_b<R>(f, pt_from_double<R>(xbv), b2);
+                double out[3];
+                if (fabs((double)b1[dim]) < 0.2 || fabs((double)b2[dim]) < 0.2) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) out[a] = (xa[a] * (pb - pm) + xbv[a] * (pm - pa)) / (pb - pa);
+                } else {
+                    rk4_boundary<R>(f, xa, pm - pa, dim, out);
+                    L.evals += 4;
+                }
+                plain_record(L, out, flag);
+                L.len += fabs(pm - pa) / (1e-4 + fabs((double)b1[dim]));
+            } else {
+                plain_record(L, xa, flag);
+            }
+        } else {
+            plain_record(L, xbv, flag);  // "ignore": the outside point itself
+        }
+        L.steps += 1;
+        plain_finish_dir(L, f, P);
+        return;
+    }
+    L.x = xb;
+    L.steps += 1;
+    if (!plain_can_step(L)) {
+        plain_record(L, xbv, kClassLimit);
+        plain_finish_dir(L, f, P);
+    }
+}
+
+}  // namespace fqsl

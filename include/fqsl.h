@@ -1,0 +1,205 @@
+/*
+ * fqsl.h -- C ABI of libfqsl.so, the MI355X (gfx950) squashing-factor engine.
+ *
+ * Plain pointers, sizes and status codes; no C++ or torch types.  Every entry
+ * point returns FQSL_OK (0) or an error code and leaves a message for
+ * fqsl_last_error() (thread-local).  Array arguments are device pointers
+ * unless a parameter says otherwise; all work is enqueued on the given HIP
+ * stream (NULL = the default stream) and nothing synchronises unless stated.
+ *
+ * Grid-index coordinates are used throughout: node (i, j, k) sits at
+ * (x, y, z) = (i, j, k); fields are three arrays Bx, By, Bz of nx*ny*nz values
+ * with x fastest, element (i, j, k) at k*ny*nx + j*nx + i
+ * (fastqslpy/cu/TraceBlineScott.cu:36-39).
+ *
+ * Line classes written to flag_start / flag_end (TraceBlineScott.cu:310-332):
+ *   1..6  left through x min, x max, y min, y max, z min, z max
+ *   0     stopped on the length / step limit (its side's endpoint is the last
+ *         state reached; Q = NaN)
+ *   42    any other exit (edge, corner, NaN); the line stops, Q = 0
+ *   -1    seed below z = -1e-8 (Scott tracer only; Q = NaN)
+ */
+#ifndef FQSL_H
+#define FQSL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FQSL_OK 0
+#define FQSL_ERR_INVALID 1     /* bad argument */
+#define FQSL_ERR_HIP 2         /* a HIP runtime call failed */
+#define FQSL_ERR_NOMEM 3       /* device allocation failed */
+#define FQSL_ERR_UNSUPPORTED 4 /* valid request this build does not implement */
+
+#define FQSL_DTYPE_F32 0
+#define FQSL_DTYPE_F64 1
+
+#define FQSL_METHOD_RKF45 0
+#define FQSL_METHOD_RK4 1
+
+/* precision of the tracing arithmetic */
+#define FQSL_PREC_FP32C 0 /* FP32 field/stages, compensated position, FP64 controller and Q */
+#define FQSL_PREC_FP64 1  /* FP64 throughout */
+
+/* how lines are mapped to lanes */
+#define FQSL_SCHED_PERSISTENT 0 /* persistent grid, each lane refills from a device queue */
+#define FQSL_SCHED_PER_LINE 1   /* one thread per line, one launch */
+
+/* field layouts held on the device (bit mask for fqsl_field_create) */
+#define FQSL_LAYOUT_SOA32 1  /* three float arrays, x fastest */
+#define FQSL_LAYOUT_SOA64 2  /* three double arrays, x fastest */
+#define FQSL_LAYOUT_CELL32 4 /* 128-byte cell stencils (8 corners x 3 components, float) */
+#define FQSL_LAYOUT_ALL 7
+
+const char *fqsl_last_error(void);
+const char *fqsl_version(void);
+
+/* ------------------------------------------------------------------ fields */
+typedef struct fqsl_field fqsl_field;
+
+/* Copy a field to the current device and build the requested layouts; the
+ * library owns the result.  bx/by/bz are host pointers when on_device == 0,
+ * device pointers otherwise.  Synchronous.  Replaces the notebooks' cupy.array
+ * upload of Bx/By/Bz (demo/Q_RKF45CuPyScott.ipynb cell [4]). */
+int fqsl_field_create(const void *bx, const void *by, const void *bz, int nx, int ny, int nz,
+                      int dtype, int on_device, int layouts, fqsl_field **out);
+int fqsl_field_destroy(fqsl_field *field);
+/* Device bytes held by the field's layouts. */
+size_t fqsl_field_bytes(const fqsl_field *field);
+int fqsl_field_shape(const fqsl_field *field, int *nx, int *ny, int *nz);
+
+/* ------------------------------------------------------------ Scott tracer */
+typedef struct {
+    int method;     /* FQSL_METHOD_RKF45 (TraceBlineScott.cu:256) or FQSL_METHOD_RK4 */
+    int precision;  /* FQSL_PREC_* */
+    int schedule;   /* FQSL_SCHED_* */
+    int reserved;
+    double step;    /* initial step (RKF45) or fixed step (RK4), cells */
+    double tol;     /* RKF45 absolute per-step tolerance, cells */
+} fqsl_scott_params;
+
+/* Per-line outputs (device arrays of n elements).  Any pointer may be NULL
+ * to skip that output.  Mirrors the outputs of TraceAllBlineScott
+ * (TraceBlineScott.cu:521-528) plus a stage-evaluation count. */
+typedef struct {
+    float *start_x, *start_y, *start_z; /* backward footpoint */
+    float *end_x, *end_y, *end_z;       /* forward footpoint */
+    int32_t *flag_start, *flag_end;
+    float *b_this_x, *b_this_y, *b_this_z; /* B at the seed, Interp3d rule */
+    float *q_0, *q_perp;                /* TraceBlineScott.cu:495-508 */
+    double *line_len;
+    uint32_t *evals;                    /* RHS evaluations spent on the line */
+} fqsl_scott_out;
+
+/* Trace n seeds (device float arrays x, y, z) with the Scott method and
+ * assemble q_0 and q_perp per line.  Replaces a launch of
+ * TraceAllBlineScott (fastqslpy/cu/TraceBlineScott.cu:521-595). */
+int fqsl_trace_scott(const fqsl_field *field, const fqsl_scott_params *params, uint64_t n,
+                     const float *x, const float *y, const float *z, const fqsl_scott_out *out,
+                     void *stream);
+
+/* ------------------------------------------------------------ plain tracer */
+typedef struct {
+    int method;      /* FQSL_METHOD_RKF45 (TraceBlineAdaptive.cu) or FQSL_METHOD_RK4 (TraceBline.cu) */
+    int precision;   /* FQSL_PREC_* */
+    int schedule;    /* FQSL_SCHED_* */
+    int reserved;
+    double step;     /* initial step, cells (each direction restarts from it) */
+    double tol_coef; /* tolerance coefficient, TraceBlineAdaptive.cu:354-359 */
+    float ncross[3]; /* plane normal: tolerance rule and B_flag */
+    float reserved2;
+} fqsl_plain_params;
+
+/* Mirrors the outputs of TraceAllBline (TraceBlineAdaptive.cu:429-437). */
+typedef struct {
+    float *start_x, *start_y, *start_z;
+    float *end_x, *end_y, *end_z;
+    int32_t *flag_start, *flag_end;
+    float *b_this_x, *b_this_y, *b_this_z;
+    int32_t *b_flag;
+    float *b_start_x, *b_start_y, *b_start_z;
+    float *b_end_x, *b_end_y, *b_end_z;
+    double *line_len;
+    uint32_t *evals;
+} fqsl_plain_out;
+
+/* Position-only tracing of n seeds, forward then backward.  Replaces a launch
+ * of TraceAllBline (TraceBlineAdaptive.cu:429-510, TraceBline.cu:383-464). */
+int fqsl_trace_plain(const fqsl_field *field, const fqsl_plain_params *params, uint64_t n,
+                     const float *x, const float *y, const float *z, const fqsl_plain_out *out,
+                     void *stream);
+
+/* ------------------------------------------- neighbour-difference assembly */
+/* QCalcPlane (fastqslpy/FastQSL.py:65-117) on a (ny, nx) row-major pixel grid:
+ * writes q of (ny-2)*(nx-2) floats.  Arithmetic follows the reference's
+ * float32 expression tree operation by operation.  b_in_z / b_out_z are Bz at
+ * the start / end footpoints; b0_* is B at the seed. */
+int fqsl_qcalc_plane(int ny, int nx, const float *x_end, const float *y_end, const float *z_end,
+                     const int32_t *flag_end, const float *x_start, const float *y_start,
+                     const float *z_start, const int32_t *flag_start, const float *b_in_z,
+                     const float *b_out_z, const float *b0_x, const float *b0_y, const float *b0_z,
+                     double stride_step, const float *cross_dir_host, float *q, void *stream);
+
+/* CookPseudoLine (FastQSL.py:4-57): for every pixel with b_flag == 1 (in
+ * pixel order) write four seeds at its start footpoint +/- stride along the
+ * two in-face axes, plus Bz at both footpoints and the pixel index.  The
+ * output arrays hold up to `capacity` pixels (4*capacity seeds); the number of
+ * selected pixels is written to *count_host after a stream synchronisation. */
+int fqsl_pseudo_seeds(uint64_t n, const int32_t *b_flag, const float *x_start, const float *y_start,
+                      const float *z_start, const int32_t *flag_start, const float *bz_start,
+                      const float *bz_end, float stride, uint64_t capacity, float *seed_x,
+                      float *seed_y, float *seed_z, float *bz0_start, float *bz0_end,
+                      uint64_t *pixel, uint64_t *count_host, void *stream);
+
+/* Qcut (demo/Q_RKF45CuPy.ipynb cell [10]) from the traces of the 4*m pseudo
+ * seeds, scattered into q_all[pixel[i]] (cell [11]). */
+int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, const float *start_z,
+                      const int32_t *flag_start, const float *end_x, const float *end_y,
+                      const float *end_z, const int32_t *flag_end, const float *bz0_start,
+                      const float *bz0_end, float stride, const uint64_t *pixel, float *q_all,
+                      void *stream);
+
+/* ------------------------------------------- reference-shaped entry points */
+/* Same parameters, order and meaning as the CUDA kernels the fastqslpy host
+ * code launches (all device pointers; scalars as one-element device arrays),
+ * plus the stream.  The field arrays are read in place (SoA float layout).
+ * A set curB_flag (twist) is not implemented: twist[] is filled with NaN. */
+
+/* TraceAllBlineScott (TraceBlineScott.cu:521-528).  tol_coef scales the
+ * per-step tolerance 1e-3 * tol_coef.  precision / schedule select the path. */
+int fqsl_TraceAllBlineScott(float *Bx, float *By, float *Bz, int *BshapeN, float *curB_x,
+                            float *curB_y, float *curB_z, double *twist, bool *curB_flag,
+                            float *inp_x, float *inp_y, float *inp_z, float *inp_cross_dir,
+                            float *start_x, float *start_y, float *start_z, int *flag_start,
+                            float *end_x, float *end_y, float *end_z, int *flag_end,
+                            float *B_this_x, float *B_this_y, float *B_this_z, float *q_0,
+                            float *q_perp, float *s_len, unsigned long long *N, double *LineLen,
+                            float *tol_coef, int precision, int schedule, void *stream);
+
+/* TraceAllBline of TraceBlineAdaptive.cu:429-437 (method RKF45) or of
+ * TraceBline.cu:383-391 (method RK4; tol_coef may be NULL). */
+int fqsl_TraceAllBline(int method, float *Bx, float *By, float *Bz, int *BshapeN, float *curB_x,
+                       float *curB_y, float *curB_z, double *twist, bool *curB_flag, float *inp_x,
+                       float *inp_y, float *inp_z, float *inp_cross_dir, float *start_x,
+                       float *start_y, float *start_z, int *flag_start, float *end_x, float *end_y,
+                       float *end_z, int *flag_end, float *B_this_x, float *B_this_y,
+                       float *B_this_z, int *B_flag, float *B_start_x, float *B_start_y,
+                       float *B_start_z, float *B_end_x, float *B_end_y, float *B_end_z,
+                       float *s_len, unsigned long long *N, double *LineLen, float *tol_coef,
+                       int precision, int schedule, void *stream);
+
+/* -------------------------------------------------------------- timing */
+/* Duration in milliseconds of the most recent trace kernel launched by
+ * fqsl_trace_scott / fqsl_trace_plain on this thread, measured with HIP
+ * events recorded around it on its stream (synchronises on the end event). */
+int fqsl_last_kernel_ms(float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FQSL_H */

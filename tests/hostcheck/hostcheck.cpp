@@ -1,0 +1,143 @@
+// Test-only CPU build of fastqsl_amd/csrc/fqsl_tracer.h: runs the per-line
+// tracer code of the HIP kernels on the host so that its logic (the exact
+// difference-quotient stencil, the compensated position, the driver) can be
+// compared with the FP64 oracle without a GPU.  Never part of the product.
+#include <string.h>
+
+#include <vector>
+
+#include "fqsl_tracer.h"
+
+using namespace fqsl;
+
+namespace {
+// host copy of the cell-stencil layout
+std::vector<float4> build_cells(const float* bx, const float* by, const float* bz, int nx, int ny, int nz) {
+    const long cx = nx - 1, cy = ny - 1, cz = nz - 1, nxy = (long)nx * ny;
+    std::vector<float4> cells(8 * cx * cy * cz);
+    const float* comp[3] = {bx, by, bz};
+    for (long t = 0; t < cx * cy * cz; ++t) {
+        const long i = t % cx, j = (t / cx) % cy, k = t / (cx * cy);
+        const long base = k * nxy + j * nx + i;
+        for (int q = 0; q < 3; ++q) {
+            float v[8];
+            for (int c = 0; c < 8; ++c) v[c] = comp[q][base + (c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy];
+            cells[8 * t + 2 * q] = make_float4(v[0], v[1], v[2], v[3]);
+            cells[8 * t + 2 * q + 1] = make_float4(v[4], v[5], v[6], v[7]);
+        }
+        cells[8 * t + 6] = make_float4(0, 0, 0, 0);
+        cells[8 * t + 7] = make_float4(0, 0, 0, 0);
+    }
+    return cells;
+}
+
+template <typename R, class F>
+void run_scott(const F& f, const ScottParams& P, long n, const float* seeds, double* out_start, double* out_end,
+               int* fs, int* fe, double* q0, double* qp, double* len, long* evals) {
+    for (long i = 0; i < n; ++i) {
+        ScottLine<R> L;
+        scott_init<R>(L, f, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+        while (!L.done) scott_attempt<R>(L, f, P);
+        scott_q<R>(L, f, &q0[i], &qp[i]);
+        for (int a = 0; a < 3; ++a) { out_start[3 * i + a] = L.ss.x[a]; out_end[3 * i + a] = L.se.x[a]; }
+        fs[i] = L.flag_s; fe[i] = L.flag_e; len[i] = L.len; evals[i] = L.evals;
+    }
+}
+}  // namespace
+
+extern "C" {
+// precision 0: float + cell layout from the float field; 1: double SoA.
+int hc_trace_scott(const double* bx, const double* by, const double* bz, int nx, int ny, int nz, int precision,
+                   int method, double s0, double tol, long n, const float* seeds, double* start, double* end,
+                   int* fs, int* fe, double* q0, double* qp, double* len, long* evals) {
+    ScottParams P{method, s0, tol};
+    const long nn = (long)nx * ny * nz;
+    if (precision == 1) {
+        FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
+        run_scott<double>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
+        return 0;
+    }
+    std::vector<float> fx(bx, bx + nn), fy(by, by + nn), fz(bz, bz + nn);
+    std::vector<float4> cells = build_cells(fx.data(), fy.data(), fz.data(), nx, ny, nz);
+    FieldCell32 f{cells.data(), {nx, ny, nz}};
+    run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
+    return 0;
+}
+
+// grad b by the stencil formula at n points (double SoA); out[9*i + 3*j + c].
+void hc_grad_b(const double* bx, const double* by, const double* bz, int nx, int ny, int nz, long n,
+               const double* pts, double* out) {
+    FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
+    for (long i = 0; i < n; ++i) {
+        const Pt<double> p = pt_from_double<double>(pts + 3 * i);
+        double b[3], J[3][3];
+        scott_eval<double>(f, p, b, J);
+        memcpy(out + 9 * i, J, sizeof(J));
+    }
+}
+
+void hc_trace_plain(const double* bx, const double* by, const double* bz, int nx, int ny, int nz, int precision,
+                    int method, double s0, double tol_coef, const float* ncross, long n, const float* seeds,
+                    double* start, double* end, int* fs, int* fe, double* len, long* evals) {
+    PlainParams P;
+    P.method = method; P.s0 = s0; P.tol_coef = tol_coef;
+    P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];
+    const long nn = (long)nx * ny * nz;
+    std::vector<float> fx(bx, bx + nn), fy(by, by + nn), fz(bz, bz + nn);
+    std::vector<float4> cells = build_cells(fx.data(), fy.data(), fz.data(), nx, ny, nz);
+    FieldSoA<double> fd{bx, by, bz, {nx, ny, nz}};
+    FieldCell32 fc{cells.data(), {nx, ny, nz}};
+    for (long i = 0; i < n; ++i) {
+        if (precision == 1) {
+            PlainLine<double> L;
+            plain_init<double>(L, fd, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+            while (!L.done) plain_attempt<double>(L, fd, P);
+            for (int a = 0; a < 3; ++a) { start[3 * i + a] = L.xs[a]; end[3 * i + a] = L.xe[a]; }
+            fs[i] = L.flag_s; fe[i] = L.flag_e; len[i] = L.len_total; evals[i] = L.evals;
+        } else {
+            PlainLine<float> L;
+            plain_init<float>(L, fc, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
+            while (!L.done) plain_attempt<float>(L, fc, P);
+            for (int a = 0; a < 3; ++a) { start[3 * i + a] = L.xs[a]; end[3 * i + a] = L.xe[a]; }
+            fs[i] = L.flag_s; fe[i] = L.flag_e; len[i] = L.len_total; evals[i] = L.evals;
+        }
+    }
+}
+}
+
+extern "C" int hc_scott_debug(const double* bx, const double* by, const double* bz, int nx, int ny, int nz,
+                              int method, double s0, double tol, const float* seed, int max_rec, double* rec) {
+    // per attempt: s used, position after, evals so far
+    ScottParams P{method, s0, tol};
+    FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
+    ScottLine<double> L;
+    scott_init<double>(L, f, P, seed[0], seed[1], seed[2]);
+    int k = 0;
+    while (!L.done && k < max_rec) {
+        const double s_used = L.s * L.dir;
+        scott_attempt<double>(L, f, P);
+        rec[5 * k + 0] = s_used;
+        rec[5 * k + 1] = L.x.get(0);
+        rec[5 * k + 2] = L.x.get(1);
+        rec[5 * k + 3] = L.x.get(2);
+        rec[5 * k + 4] = L.s;
+        ++k;
+    }
+    return k;
+}
+
+extern "C" double hc_scott_step(const double* bx, const double* by, const double* bz, int nx, int ny, int nz,
+                                int method, const double* y, double s, double* out) {
+    // one accepted-or-not attempt from state y with step s (sign = direction)
+    ScottParams P{method, fabs(s), 1e300};
+    FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
+    ScottLine<double> L;
+    L.x.set(y);
+    for (int a = 0; a < 3; ++a) { L.u[a] = y[3 + a]; L.v[a] = y[6 + a]; }
+    L.s = fabs(s); L.dir = s < 0 ? -1 : 1; L.len = 0; L.len_lim = 1e300; L.steps = 0; L.attempts = 0;
+    L.evals = 0; L.done = false; L.aborted = false; L.flag_s = L.flag_e = 0;
+    L.p0[0] = (float)y[0]; L.p0[1] = (float)y[1]; L.p0[2] = (float)y[2];
+    scott_attempt<double>(L, f, P);
+    for (int a = 0; a < 3; ++a) { out[a] = L.x.get(a); out[3 + a] = L.u[a]; out[6 + a] = L.v[a]; }
+    return L.s;
+}

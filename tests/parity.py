@@ -1,0 +1,42 @@
+"""Parity metrics shared by the GPU tests (GPU result vs FP64 oracle)."""
+import numpy as np
+
+
+def log_q_diff(q_gpu, q_ref):
+    """|log10 q_gpu - log10 q_ref| per line; 0 where both are NaN or both 0,
+    inf where only one is finite/positive."""
+    a = np.asarray(q_gpu, np.float64)
+    b = np.asarray(q_ref, np.float64)
+    d = np.full(a.shape, np.inf)
+    ok = np.isfinite(a) & np.isfinite(b) & (a > 0) & (b > 0)
+    d[ok] = np.abs(np.log10(a[ok]) - np.log10(b[ok]))
+    d[np.isnan(a) & np.isnan(b)] = 0.0
+    d[(a == 0) & (b == 0)] = 0.0
+    return d
+
+
+def class_match(fs_a, fe_a, fs_b, fe_b):
+    return (np.asarray(fs_a) == np.asarray(fs_b)) & (np.asarray(fe_a) == np.asarray(fe_b))
+
+
+def conditioned(ref, ref_perturbed, thresh=1e-4):
+    """Lines whose oracle Q and classes are stable under a perturbation far
+    below float32 resolution (seed moved by 1e-6 cells, or tolerance scaled
+    by 1 + 1e-12).  Lines on a separatrix through a null (Q -> infinity) or
+    whose adaptive step sequence forks on round-off fail this."""
+    d = log_q_diff(ref["q0"], ref_perturbed["q0"])
+    c = class_match(ref["flag_start"], ref["flag_end"], ref_perturbed["flag_start"],
+                    ref_perturbed["flag_end"])
+    return c & (d < thresh)
+
+
+def summary(d, cls, mask=None):
+    if mask is not None:
+        d = d[mask]
+        cls = cls[mask]
+    fin = d[np.isfinite(d)]
+    return dict(n=int(d.size), class_match=float(cls.mean()) if cls.size else 1.0,
+                frac_le_1e3=float(np.mean(d <= 1e-3)) if d.size else 1.0,
+                p999=float(np.quantile(fin, 0.999)) if fin.size else 0.0,
+                median=float(np.median(fin)) if fin.size else 0.0,
+                max=float(fin.max()) if fin.size else 0.0)

@@ -1,0 +1,181 @@
+"""GPU parity of the Scott tracer against the FP64 oracle.
+
+Bars (per line, over lines whose oracle result is well conditioned -- stable
+under a perturbation far below float32 resolution, see parity.conditioned):
+
+* RK4 fixed step (the quadrupole workload, 64^3 float64 field, 256^2 seeds):
+  FP64 path -- classes identical, |dlog10 Q| <= 1e-5 (float32 output);
+  FP32-compensated path -- classes match on >= 99.99 %, |dlog10 Q| <= 1e-3
+  on >= 99.9 %.
+* RKF45 (random multipole, 256^3 float32, tol 1e-6): the reference's
+  controller forks on round-off, so the oracle's own result moves when its
+  tolerance is scaled by 1 + 1e-12.  The GPU must stay within that noise
+  floor: classes and the |dlog10 Q| <= 1e-3 fraction no worse than the
+  oracle-vs-perturbed-oracle figures (minus a small sampling allowance).
+"""
+import numpy as np
+import pytest
+
+from fastqsl_amd import synth
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(r, keys=("start_x", "start_y", "start_z", "end_x", "end_y", "end_z", "flag_start",
+                 "flag_end", "q_0", "q_perp", "line_len", "evals", "b_this_x", "b_this_y",
+                 "b_this_z")):
+    import torch
+    torch.cuda.synchronize()
+    return {k: r[k].cpu().numpy() for k in keys if k in r}
+
+
+@pytest.fixture(scope="module")
+def fq():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import fastqsl_amd
+    fastqsl_amd.lib()
+    return fastqsl_amd
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu")
+    g = np.linspace(0, 63, 256, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]).astype(np.float32)
+    ref = oracle.trace_scott(field, seeds, method="rk4", s0=0.25)
+    refp = oracle.trace_scott(field, seeds.astype(np.float64) + np.array([1e-6, -1e-6, 0.0]),
+                              method="rk4", s0=0.25)
+    cond = parity.conditioned(ref, refp)
+    return field, seeds, ref, cond
+
+
+def _trace(fq, f, seeds, **kw):
+    return _np(fq.trace_scott(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], **kw))
+
+
+def test_c1_fp64_matches_oracle(fq, c1):
+    field, seeds, ref, cond = c1
+    print(f"well-conditioned lines: {cond.mean():.5f}")
+    assert cond.mean() > 0.99
+    with fq.Field(*field) as f:
+        for sched in ("per_line", "persistent"):
+            r = _trace(fq, f, seeds, method="rk4", step=0.25, precision="fp64", schedule=sched)
+            cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+            d = parity.log_q_diff(r["q_0"], ref["q0"])
+            dp = parity.log_q_diff(r["q_perp"], ref["qperp"])
+            print(sched, parity.summary(d, cls, cond), "all:", parity.summary(d, cls))
+            assert cls[cond].all()
+            assert d[cond].max() <= 1e-5 and dp[cond].max() <= 1e-5
+            end = np.column_stack([r["end_x"], r["end_y"], r["end_z"]])
+            start = np.column_stack([r["start_x"], r["start_y"], r["start_z"]])
+            assert np.abs(end - ref["end"])[cond].max() < 1e-4
+            assert np.abs(start - ref["start"])[cond].max() < 1e-4
+            np.testing.assert_allclose(r["line_len"][cond], ref["length"][cond], rtol=1e-8)
+            assert np.array_equal(r["evals"][cond].astype(np.int64), ref["evals"][cond])
+            bt = np.column_stack([r["b_this_x"], r["b_this_y"], r["b_this_z"]])
+            np.testing.assert_allclose(bt, ref["b_this"], rtol=1e-5, atol=1e-6)
+
+
+def test_c1_fp32c_parity(fq, c1):
+    field, seeds, ref, cond = c1
+    with fq.Field(*field) as f:
+        r = _trace(fq, f, seeds, method="rk4", step=0.25, precision="fp32c", schedule="persistent")
+    cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    d = parity.log_q_diff(r["q_0"], ref["q0"])
+    s = parity.summary(d, cls, cond)
+    print("fp32c conditioned:", s, "all:", parity.summary(d, cls))
+    assert s["class_match"] >= 0.9999
+    assert s["frac_le_1e3"] >= 0.999
+    end = np.column_stack([r["end_x"], r["end_y"], r["end_z"]])
+    err = np.abs(end - ref["end"]).max(axis=1)[cond & cls]
+    print("endpoint error (cells): p99.9 %.2e max %.2e" % (np.quantile(err, 0.999), err.max()))
+
+
+def test_schedules_bitwise_identical(fq, c1):
+    """A line's result does not depend on which lane or launch traced it."""
+    field, seeds, ref, cond = c1
+    with fq.Field(*field) as f:
+        for prec in ("fp32c", "fp64"):
+            a = _trace(fq, f, seeds, method="rk4", step=0.25, precision=prec, schedule="persistent")
+            b = _trace(fq, f, seeds, method="rk4", step=0.25, precision=prec, schedule="per_line")
+            perm = np.random.default_rng(0).permutation(len(seeds))
+            c = _trace(fq, f, seeds[perm], method="rk4", step=0.25, precision=prec, schedule="persistent")
+            inv = np.argsort(perm)
+            for k in a:
+                assert np.array_equal(a[k], b[k], equal_nan=True), (prec, k)
+                assert np.array_equal(a[k], c[k][inv], equal_nan=True), (prec, k)
+
+
+@pytest.fixture(scope="module")
+def c2(oracle):
+    field = synth.multipole_256(seed=0, device="cuda")
+    n = 1024
+    rng = np.random.default_rng(0)
+    ij = rng.choice(n * n, size=8192, replace=False)
+    g = np.linspace(0, 255, n, dtype=np.float32)
+    seeds = np.column_stack([g[ij % n], g[ij // n], np.zeros(ij.size, np.float32)]).astype(np.float32)
+    ref = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6)
+    refp = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6 * (1 + 1e-12))
+    return field, seeds, ref, refp
+
+
+def test_c2_rkf45_within_oracle_noise_floor(fq, c2):
+    field, seeds, ref, refp = c2
+    dfl = parity.log_q_diff(refp["q0"], ref["q0"])
+    cfl = parity.class_match(refp["flag_start"], refp["flag_end"], ref["flag_start"], ref["flag_end"])
+    floor = parity.summary(dfl, cfl)
+    print("oracle vs perturbed oracle:", floor)
+    with fq.Field(*field) as f:
+        for prec in ("fp64", "fp32c"):
+            r = _trace(fq, f, seeds, method="rkf45", step=0.25, tol=1e-6, precision=prec)
+            cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+            d = parity.log_q_diff(r["q_0"], ref["q0"])
+            s = parity.summary(d, cls)
+            print(prec, s)
+            assert s["class_match"] >= floor["class_match"] - 2e-3
+            assert s["frac_le_1e3"] >= floor["frac_le_1e3"] - 0.05
+            assert s["median"] <= 2.0 * floor["median"] + 1e-5
+            ev = r["evals"].astype(np.float64).sum() / ref["evals"].sum()
+            print(prec, "evals ratio vs oracle %.4f" % ev)
+            assert 0.8 < ev < 1.25
+
+
+def test_hyperbolic_closed_form_gpu(fq):
+    shape = (41, 41, 33)
+    k = 0.03
+    field = synth.hyperbolic(shape, k=k, b0=1.0)
+    rng = np.random.default_rng(4)
+    seeds = np.column_stack([rng.uniform(17, 23, 64), rng.uniform(17, 23, 64), np.zeros(64)]).astype(np.float32)
+    seeds[:16, 2] = rng.uniform(1, 20, 16)
+    Qc = 2 * np.cosh(2 * k * shape[2])
+    with fq.Field(*field) as f:
+        for prec in ("fp64", "fp32c"):
+            for method, step, tol in (("rk4", 0.125, 0.0), ("rkf45", 0.5, 1e-9)):
+                r = _trace(fq, f, seeds, method=method, step=step, tol=max(tol, 1e-12), precision=prec)
+                assert np.all(r["flag_start"] == 5) and np.all(r["flag_end"] == 6)
+                np.testing.assert_allclose(r["q_0"], Qc, rtol=2e-3)
+                np.testing.assert_allclose(r["end_z"], shape[2], atol=1e-5)
+
+
+def test_edge_cases_gpu(fq, oracle):
+    shape = (32, 32, 32)
+    hyp = synth.hyperbolic(shape, k=0.03, b0=1.0)
+    null = synth.hyperbolic(shape, k=0.03, b0=0.0)
+    seeds = np.array([[10.0, 10.0, -0.5],     # below the surface
+                      [0.0, 5.0, 0.0],        # on the x = 0 face
+                      [31.0, 12.0, 3.0],      # on the x max face
+                      [5.0, 0.0, 2.0],        # y = 0 (outside by the strict rule)
+                      [15.5, 15.5, 4.0]], np.float32)   # the null (second field)
+    for field in (hyp, null):
+        ref = oracle.trace_scott(field, seeds, method="rk4", s0=0.25)
+        with fq.Field(*field) as f:
+            for prec in ("fp64", "fp32c"):
+                r = _trace(fq, f, seeds, method="rk4", step=0.25, precision=prec)
+                np.testing.assert_array_equal(r["flag_start"], ref["flag_start"])
+                np.testing.assert_array_equal(r["flag_end"], ref["flag_end"])
+                np.testing.assert_array_equal(np.isnan(r["q_0"]), np.isnan(ref["q0"]))
+                np.testing.assert_array_equal(r["q_0"] == 0, ref["q0"] == 0)
