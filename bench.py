@@ -42,7 +42,9 @@ def parse():
     ap.add_argument("--schedule", default="persistent", choices=["persistent", "per_line"])
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (known bytes) for counter scaling")
     return ap.parse_args()
 
 
@@ -97,6 +99,14 @@ def main():
         out = fq.trace_scott(field, sx, sy, sz, precision=precision, schedule=schedule,
                              out=out, **kw)
         return out
+
+    if args.calibrate:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1073741824 B read + 1073741824 B written")
+        del src, dst
 
     for _ in range(args.warmup):
         run(args.precision, args.schedule)

@@ -72,6 +72,8 @@ PROTOTYPES = {
     "fqsl_TraceAllBlineScott": (i32, [vp] * 30 + [i32, i32, vp]),
     "fqsl_TraceAllBline": (i32, [i32] + [vp] * 35 + [i32, i32, vp]),
     "fqsl_last_kernel_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
+    "fqsl_tune_set": (i32, [i32, i32]),
+    "fqsl_tune_get": (i32, [i32, ctypes.POINTER(i32)]),
 }
 
 
@@ -95,6 +97,19 @@ def check(rc: int) -> None:
     if rc != FQSL_OK:
         msg = lib().fqsl_last_error().decode(errors="replace")
         raise FqslError(f"libfqsl error {rc}: {msg}")
+
+
+TUNE_KNOBS = {"scott_waves": 0, "grid_pct": 1, "queues": 2, "refill_min": 3}
+
+
+def tune_set(knob: str, value: int) -> None:
+    check(lib().fqsl_tune_set(TUNE_KNOBS[knob], int(value)))
+
+
+def tune_get(knob: str) -> int:
+    v = i32()
+    check(lib().fqsl_tune_get(TUNE_KNOBS[knob], ctypes.byref(v)))
+    return int(v.value)
 
 
 def last_kernel_ms() -> float:

@@ -202,11 +202,35 @@ extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) 
     return FQSL_OK;
 }
 
+// ------------------------------------------------------------- tuning knobs
+static int g_tune[FQSL_TUNE_COUNT] = {0, 100, 1, 16};
+
+extern "C" int fqsl_tune_set(int knob, int value) {
+    if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
+    if (knob == FQSL_TUNE_SCOTT_WAVES && value != 0 && value != 2 && value != 3)
+        return set_err(FQSL_ERR_INVALID, "scott waves/SIMD target %d not in {0, 2, 3}", value);
+    if (knob == FQSL_TUNE_QUEUES && value != 1 && value != 8)
+        return set_err(FQSL_ERR_INVALID, "queue count %d not in {1, 8}", value);
+    if (knob == FQSL_TUNE_REFILL_MIN && (value < 1 || value > 64))
+        return set_err(FQSL_ERR_INVALID, "refill group %d not in 1..64", value);
+    if (knob == FQSL_TUNE_GRID_PCT && (value < 1 || value > 1000))
+        return set_err(FQSL_ERR_INVALID, "grid percentage %d not in 1..1000", value);
+    g_tune[knob] = value;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_tune_get(int knob, int* value) {
+    if (knob < 0 || knob >= FQSL_TUNE_COUNT || !value) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
+    *value = g_tune[knob];
+    return FQSL_OK;
+}
+
 // ------------------------------------------------------- device work queue
 // A ring of 64-bit queue counters; each trace call takes the next slot and
 // clears it with a fill kernel on its stream before the persistent launch.
 // Calls that are more than kQueueSlots launches apart never share a slot.
 static constexpr int kQueueSlots = 4096;
+static constexpr int kQueueSlotWidth = 8;  // counters per slot (one per XCD queue)
 struct DeviceQueues {
     unsigned long long* slots = nullptr;
     unsigned next = 0;
@@ -222,10 +246,10 @@ static int queue_slot(unsigned long long** slot) {
     if ((int)g_queues.size() <= dev) g_queues.resize(dev + 1);
     DeviceQueues& d = g_queues[dev];
     if (!d.slots) {
-        HIP_TRY(hipMalloc(&d.slots, kQueueSlots * sizeof(unsigned long long)));
-        HIP_TRY(hipMemset(d.slots, 0, kQueueSlots * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&d.slots, (size_t)kQueueSlots * kQueueSlotWidth * sizeof(unsigned long long)));
+        HIP_TRY(hipMemset(d.slots, 0, (size_t)kQueueSlots * kQueueSlotWidth * sizeof(unsigned long long)));
     }
-    *slot = d.slots + (d.next++ % kQueueSlots);
+    *slot = d.slots + (size_t)kQueueSlotWidth * (d.next++ % kQueueSlots);
     return FQSL_OK;
 }
 
@@ -242,7 +266,9 @@ static int cu_count() {
     return g_cu_count[dev];
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void k_zero_u64(unsigned long long* p, int count) {
+    if ((int)threadIdx.x < count) p[threadIdx.x] = 0ull;
+}
 
 // Scalars the reference passes as one-element device arrays.  When `shape`
 // is non-NULL the kernel reads them on the device.
@@ -280,14 +306,20 @@ struct PlainOut {
 };
 
 template <typename R, class F>
-__device__ __forceinline__ void scott_write(const ScottLine<R>& L, const F& f, unsigned long long i,
-                                            const ScottOut& o, const DevScalars& ds) {
-    double q0, qp;
-    scott_q<R>(L, f, &q0, &qp);
-    if (o.sx) { o.sx[i] = (float)L.ss.x[0]; o.sy[i] = (float)L.ss.x[1]; o.sz[i] = (float)L.ss.x[2]; }
-    if (o.ex) { o.ex[i] = (float)L.se.x[0]; o.ey[i] = (float)L.se.x[1]; o.ez[i] = (float)L.se.x[2]; }
-    if (o.fs) o.fs[i] = L.flag_s;
-    if (o.fe) o.fe[i] = L.flag_e;
+__device__ __attribute__((noinline)) void scott_write(const ScottLine<R>& L, const ScottEnd<R>& e, int status,
+                                                      const F& f, unsigned long long i, const ScottOut& o,
+                                                      const DevScalars& ds) {
+    double q0, qp, nb0 = 0.0;
+    if (status == kFinished) {
+        R u0[3], v0[3], b0z;
+        seed_frame<R>(f, L.p0, u0, v0, &b0z, &nb0);
+        nb0 = (double)(R)nb0;  // |B0| as the persistent path's seed frame stores it
+    }
+    scott_q<R>(L.st, e, status, nb0, f, &q0, &qp);
+    if (o.sx) { o.sx[i] = (float)L.st.x[0]; o.sy[i] = (float)L.st.x[1]; o.sz[i] = (float)L.st.x[2]; }
+    if (o.ex) { o.ex[i] = (float)e.x[0]; o.ey[i] = (float)e.x[1]; o.ez[i] = (float)e.x[2]; }
+    if (o.fs) o.fs[i] = L.st.flag;
+    if (o.fe) o.fe[i] = e.flag;
     if (o.q0) o.q0[i] = (float)q0;
     if (o.qp) o.qp[i] = (float)qp;
     if (o.len) o.len[i] = L.len;
@@ -315,12 +347,85 @@ __device__ __forceinline__ unsigned long long bcast64(unsigned long long v, int 
     return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
 
-// Persistent Scott tracer: every lane owns one line; lanes whose line is done
-// take the next queue entries (one atomic per wavefront refill).  A wave exits
-// once the queue is drained and all its lanes are idle.
+// Finished line as handed from the stepping kernel to the Q-assembly kernel.
+template <typename R>
+struct ScottRec {
+    ScottEnd<R> st, en;
+    double len;
+    int32_t status;
+    uint32_t evals;
+};
+
+// Work queues of the persistent kernels: kQueues counters, queue q owning
+// seeds [q*n/kQueues, (q+1)*n/kQueues).  A workgroup starts on queue
+// blockIdx % kQueues -- consecutive workgroups land on different XCDs, so each
+// XCD first drains its own contiguous stretch of seeds (its L2 then holds that
+// region of the field) -- and moves on to the other queues when its own is
+// empty.  With one queue all workgroups share a single counter.
+constexpr int kQueues = 8;
+
+struct QueueCursor {
+    int cur, tried;
+};
+
+// Claim `cnt` consecutive seed indices for a wavefront; returns false once
+// every queue is empty.  base/avail describe the claimed range.
+__device__ __forceinline__ bool claim(unsigned long long* queues, int nq, unsigned long long n, QueueCursor& qc,
+                                      int cnt, int leader, unsigned long long& base, int& avail) {
+    const int lane = __lane_id();
+    while (qc.tried < nq) {
+        const unsigned long long lo = n * (unsigned long long)qc.cur / nq;
+        const unsigned long long hi = n * (unsigned long long)(qc.cur + 1) / nq;
+        unsigned long long got = 0;
+        if (lane == leader) got = atomicAdd(queues + qc.cur, (unsigned long long)cnt);
+        got = bcast64(got, leader);
+        if (lo + got < hi) {
+            base = lo + got;
+            const unsigned long long left = hi - base;
+            avail = left < (unsigned long long)cnt ? (int)left : cnt;
+            if (left <= (unsigned long long)cnt) { qc.cur = (qc.cur + 1) % nq; ++qc.tried; }
+            return true;
+        }
+        qc.cur = (qc.cur + 1) % nq;
+        ++qc.tried;
+    }
+    return false;
+}
+
+// Seed preparation, one thread per seed: frame (u0, v0, b0z, |B0|) for the
+// stepping kernel and B at the seed (Interp3d rule) straight to the output.
 template <typename R, class F>
-__global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Seeds s, unsigned long long n,
-                                                          ScottOut out, DevScalars ds, unsigned long long* queue) {
+__global__ void __launch_bounds__(256) k_scott_prep(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                    R* __restrict__ frames, ScottOut o, DevScalars ds) {
+    apply_scalars(f, P, n, ds);
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float p0[3] = {s.x[i], s.y[i], s.z[i]};
+        R u0[3], v0[3], b0z;
+        double nb0;
+        seed_frame<R>(f, p0, u0, v0, &b0z, &nb0);
+        R* fr = frames + (unsigned long long)kFrameWords * i;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { fr[a] = u0[a]; fr[3 + a] = v0[a]; }
+        fr[6] = b0z;
+        fr[7] = (R)nb0;
+        if (o.btx) {
+            float bt[3];
+            b_clamped<R>(f, p0, bt);
+            o.btx[i] = bt[0]; o.bty[i] = bt[1]; o.btz[i] = bt[2];
+        }
+    }
+}
+
+// Persistent stepping: every lane owns one line; idle lanes take the next
+// queue entries once at least `refill_min` lanes of the wavefront are idle (or
+// all are), so the setup of refilled lines runs once for the group.  A wave
+// exits once every queue is empty and all its lanes are idle.
+template <typename R, class F>
+__device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                      const R* __restrict__ frames, ScottRec<R>* __restrict__ recs,
+                                                      DevScalars ds, unsigned long long* queues, int nq,
+                                                      int refill_min) {
     apply_scalars(f, P, n, ds);
     ScottLine<R> L;
     bool active = false;
@@ -328,22 +433,26 @@ __global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Se
     unsigned long long idx = 0;
     const int lane = __lane_id();
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    QueueCursor qc{(int)(blockIdx.x % (unsigned)nq), 0};
     for (;;) {
         if (!drained) {
             const unsigned long long need = __ballot(!active);
-            if (need) {
+            const int cnt = __popcll(need);
+            if (cnt >= refill_min || need == ~0ull) {
                 const int leader = __ffsll((long long)need) - 1;
-                const int cnt = __popcll(need);
                 unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(queue, (unsigned long long)cnt);
-                base = bcast64(base, leader);
-                if (base + (unsigned long long)cnt >= n) drained = true;
-                if (!active) {
-                    idx = base + (unsigned long long)__popcll(need & below);
-                    if (idx < n) {
-                        scott_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx]);
-                        if (L.done) scott_write<R>(L, f, idx, out, ds);
-                        else active = true;
+                int avail = 0;
+                if (!claim(queues, nq, n, qc, cnt, leader, base, avail)) drained = true;
+                const int rank = __popcll(need & below);
+                if (!active && rank < avail) {
+                    idx = base + (unsigned long long)rank;
+                    ScottEnd<R> e;
+                    const int st = scott_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx], e,
+                                                 frames + (unsigned long long)kFrameWords * idx);
+                    if (st != kTracing) {
+                        recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                    } else {
+                        active = true;
                     }
                 }
             }
@@ -353,12 +462,57 @@ __global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Se
             continue;
         }
         if (active) {
-            scott_attempt<R>(L, f, P);
-            if (L.done) {
-                scott_write<R>(L, f, idx, out, ds);
+            ScottEnd<R> e;
+            const int st = scott_attempt<R>(L, f, P, e);
+            if (st != kTracing) {
+                recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
                 active = false;
             }
         }
+    }
+}
+
+// W = waves per SIMD the register allocation targets.
+#define FQSL_PERSISTENT_KERNEL(W)                                                                     \
+    template <typename R, class F>                                                                    \
+    __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                \
+    k_scott_persistent_w##W(F f, ScottParams P, Seeds s, unsigned long long n, const R* frames,       \
+                            ScottRec<R>* recs, DevScalars ds, unsigned long long* queues, int nq,     \
+                            int refill_min) {                                                         \
+        scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min);            \
+    }
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                          const R* frames, ScottRec<R>* recs, DevScalars ds,
+                                                          unsigned long long* queues, int nq, int refill_min) {
+    scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min);
+}
+FQSL_PERSISTENT_KERNEL(2)
+FQSL_PERSISTENT_KERNEL(3)
+
+// Q assembly and output, one thread per line (TraceBlineScott.cu:475-508,
+// outputs as TraceAllBlineScott :565-582).
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_scott_finish(F f, ScottParams P, unsigned long long n,
+                                                      const R* __restrict__ frames,
+                                                      const ScottRec<R>* __restrict__ recs, ScottOut o,
+                                                      DevScalars ds) {
+    apply_scalars(f, P, n, ds);
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const ScottRec<R> r = recs[i];
+        double q0, qp;
+        scott_q<R>(r.st, r.en, r.status, (double)frames[(unsigned long long)kFrameWords * i + 7], f, &q0, &qp);
+        if (o.sx) { o.sx[i] = (float)r.st.x[0]; o.sy[i] = (float)r.st.x[1]; o.sz[i] = (float)r.st.x[2]; }
+        if (o.ex) { o.ex[i] = (float)r.en.x[0]; o.ey[i] = (float)r.en.x[1]; o.ez[i] = (float)r.en.x[2]; }
+        if (o.fs) o.fs[i] = r.st.flag;
+        if (o.fe) o.fe[i] = r.en.flag;
+        if (o.q0) o.q0[i] = (float)q0;
+        if (o.qp) o.qp[i] = (float)qp;
+        if (o.len) o.len[i] = r.len;
+        if (o.evals) o.evals[i] = r.evals;
+        if (ds.twist_flag && ds.twist && ds.twist_flag[0]) ds.twist[i] = NAN;
     }
 }
 
@@ -371,15 +525,16 @@ __global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seed
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
         ScottLine<R> L;
-        scott_init<R>(L, f, P, s.x[i], s.y[i], s.z[i]);
-        while (!L.done) scott_attempt<R>(L, f, P);
-        scott_write<R>(L, f, i, out, ds);
+        ScottEnd<R> e;
+        int st = scott_init<R>(L, f, P, s.x[i], s.y[i], s.z[i], e);
+        while (st == kTracing) st = scott_attempt<R>(L, f, P, e);
+        scott_write<R>(L, e, st, f, i, out, ds);
     }
 }
 
 // ------------------------------------------------------------ plain kernels
 template <typename R, class F>
-__device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, const PlainParams& P,
+__device__ __attribute__((noinline)) void plain_write(const PlainLine<R>& L, const F& f, const PlainParams& P,
                                             unsigned long long i, const PlainOut& o, const DevScalars& ds) {
     const float xs[3] = {(float)L.xs[0], (float)L.xs[1], (float)L.xs[2]};
     const float xe[3] = {(float)L.xe[0], (float)L.xe[1], (float)L.xe[2]};
@@ -507,19 +662,43 @@ template <typename R, class F>
 static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsigned long long n, const ScottOut& o,
                         const DevScalars& ds, int schedule, hipStream_t st, unsigned long long n_hint) {
     if (schedule == FQSL_SCHED_PERSISTENT) {
+        // n_hint: number of lines, known on the host (sizes the workspace)
+        if (n_hint == 0) return FQSL_OK;
         unsigned long long* q = nullptr;
         int rc = queue_slot(&q);
         if (rc) return rc;
+        const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES] : 0;
+        auto kern = waves == 2 ? k_scott_persistent_w2<R, F>
+                  : waves == 3 ? k_scott_persistent_w3<R, F>
+                               : k_scott_persistent<R, F>;
         int grid = 0;
-        rc = persistent_grid(k_scott_persistent<R, F>, &grid);
+        rc = persistent_grid(kern, &grid);
         if (rc) return rc;
-        k_zero_u64<<<1, 1, 0, st>>>(q);
-        HIP_TRY(hipGetLastError());
-        rc = record_start(st);
+        grid = (int)((long)grid * g_tune[FQSL_TUNE_GRID_PCT] / 100);
+        if (grid < 1) grid = 1;
+        const int nq = g_tune[FQSL_TUNE_QUEUES];
+        const int refill_min = g_tune[FQSL_TUNE_REFILL_MIN];
+        const size_t frame_bytes = (size_t)kFrameWords * sizeof(R) * n_hint;
+        const size_t rec_off = (frame_bytes + 255) & ~(size_t)255;
+        void* ws = nullptr;
+        HIP_TRY(hipMallocAsync(&ws, rec_off + sizeof(ScottRec<R>) * n_hint, st));
+        R* frames = (R*)ws;
+        ScottRec<R>* recs = (ScottRec<R>*)((char*)ws + rec_off);
+        k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) rc = record_start(st);
+        if (e == hipSuccess && !rc) {
+            k_scott_prep<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, frames, o, ds);
+            kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min);
+            k_scott_finish<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, n, frames, recs, o, ds);
+            e = hipGetLastError();
+            if (e == hipSuccess) rc = record_end(st);
+        }
+        const hipError_t ef = hipFreeAsync(ws, st);
+        if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "Scott trace launch: %s", hipGetErrorString(e));
         if (rc) return rc;
-        k_scott_persistent<R, F><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, q);
-        HIP_TRY(hipGetLastError());
-        return record_end(st);
+        if (ef != hipSuccess) return set_err(FQSL_ERR_HIP, "hipFreeAsync: %s", hipGetErrorString(ef));
+        return FQSL_OK;
     }
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
@@ -542,7 +721,7 @@ static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsign
         int grid = 0;
         rc = persistent_grid(k_plain_persistent<R, F>, &grid);
         if (rc) return rc;
-        k_zero_u64<<<1, 1, 0, st>>>(q);
+        k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
         HIP_TRY(hipGetLastError());
         rc = record_start(st);
         if (rc) return rc;
@@ -666,8 +845,13 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
                q_0, q_perp, LineLen, nullptr};
     DevScalars ds{BshapeN, s_len, N, tol_coef, curB_flag, twist};
     FieldSoA<float> f{Bx, By, Bz, {2, 2, 2}};
-    return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream,
-                               (unsigned long long)kPerLineMaxBlocks * 256);
+    unsigned long long n_hint = (unsigned long long)kPerLineMaxBlocks * 256;
+    if (schedule == FQSL_SCHED_PERSISTENT) {
+        // the persistent path sizes its workspace from N: read it (synchronises the stream)
+        HIP_TRY(hipMemcpyAsync(&n_hint, N, sizeof(n_hint), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream, n_hint);
 }
 
 extern "C" int fqsl_TraceAllBline(int method, float* Bx, float* By, float* Bz, int* BshapeN, float* curB_x,

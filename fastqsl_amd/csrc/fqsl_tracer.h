@@ -45,6 +45,7 @@
 #include <stdint.h>
 
 #define FQSL_HD __host__ __device__ __forceinline__
+#define FQSL_COLD __host__ __device__ __attribute__((noinline))
 
 namespace fqsl {
 
@@ -81,14 +82,24 @@ struct Tab {
 };
 
 // ------------------------------------------------------------ scalar helpers
+// Hardware reciprocal square root / reciprocal (v_rsq_f32, v_rcp_f32: about
+// 1 ulp) on the device.
 FQSL_HD float rsq(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __frsqrt_rn(x);
+    return __builtin_amdgcn_rsqf(x);
 #else
     return 1.0f / sqrtf(x);
 #endif
 }
 FQSL_HD double rsq(double x) { return 1.0 / sqrt(x); }
+FQSL_HD float rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+FQSL_HD double rcp(double x) { return 1.0 / x; }
 FQSL_HD float flr(float x) { return floorf(x); }
 FQSL_HD double flr(double x) { return floor(x); }
 
@@ -234,7 +245,7 @@ FQSL_HD void fd_quotient(const R B[3], const R ap[3], const R am[3], R out[3]) {
     const R np = np2 * ip, nm = nm2 * im;
     const R inv2h = (R)(0.5 / kFdH);
     const R t1 = (R)0.5 * (ip + im) * inv2h;
-    const R t2 = (R)0.5 * dot3(S, D) * ip * im / (np + nm) * inv2h;
+    const R t2 = (R)0.5 * dot3(S, D) * ip * im * rcp(np + nm) * inv2h;
 #pragma unroll
     for (int q = 0; q < 3; ++q) out[q] = D[q] * t1 - S[q] * t2;
 }
@@ -260,43 +271,71 @@ FQSL_HD Pt<R> pt_from_double(const double* x) {
 
 // Scott right-hand side at p: b = B/|B| and J[j][q] = (difference quotient of
 // b_q along axis j), i.e. grad_unit_vec_B (TraceBlineScott.cu:204-226).
+//
+// kExact (the FP64 path) evaluates the reference's quotient exactly on every
+// axis.  Otherwise an axis whose +-h samples both stay inside the current cell
+// uses the closed-form derivative (d - b (b.d)) / |B| of the same interpolant:
+// the central quotient differs from it by (h^2/6) times the third derivative
+// of b, at most ~2e-7 relative for a field whose scale length is >= 1 cell,
+// below float resolution.  Axes whose samples cross a grid plane or the box
+// face keep the exact quotient, which there blends two cells or halves the
+// one-sided slope.
 template <typename R, class F>
 FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
-    Axis<R> ax[3];
+    constexpr bool kExact = sizeof(R) == 8;
+    const R h = (R)kFdH;
+    int c[3];
+    R w[3];
+    bool simple[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) ax[a] = axis_setup<R>(p.i[a], p.r[a], f.n[a]);
-    R c[3][8];
-    f.template load<R>(ax[0].c, ax[1].c, ax[2].c, c);
+    for (int a = 0; a < 3; ++a) {
+        const int i = p.i[a], n = f.n[a];
+        const R r = p.r[a];
+        const bool low = i < 0 || (i == 0 && r == (R)0);  // x <= 0
+        const bool high = i >= n - 1;                      // x >= N-1
+        c[a] = low ? 0 : (high ? n - 2 : i);
+        w[a] = (r != r) ? r : (low ? (R)0 : (high ? (R)1 : r));
+        simple[a] = !kExact && !low && !high && r >= h && r + h <= (R)1;
+    }
+    R cc[3][8];
+    f.template load<R>(c[0], c[1], c[2], cc);
     R B[3], d[3][3];
-    cell_eval<R>(c, ax[0].w, ax[1].w, ax[2].w, B, d);
+    cell_eval<R>(cc, w[0], w[1], w[2], B, d);
+    const R inb = rsq(dot3(B, B));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
+        if (simple[j]) {
+            const R bd = dot3(b, d[j]);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) J[j][q] = (d[j][q] - b[q] * bd) * inb;
+            continue;
+        }
+        const Axis<R> ax = axis_setup<R>(p.i[j], p.r[j], f.n[j]);
         R ap[3], am[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            ap[q] = ax[j].ap * d[j][q];
-            am[q] = ax[j].am * d[j][q];
+            ap[q] = ax.ap * d[j][q];
+            am[q] = ax.am * d[j][q];
         }
-        if (ax[j].apn != (R)0 || ax[j].amp != (R)0) {  // the +-h sample crosses a grid plane
-            const int dj = ax[j].apn != (R)0 ? 1 : -1;
-            const R coef = ax[j].apn != (R)0 ? ax[j].apn : ax[j].amp;
-            int ci[3] = {ax[0].c, ax[1].c, ax[2].c};
+        if (ax.apn != (R)0 || ax.amp != (R)0) {  // the +-h sample crosses a grid plane
+            // (crossing on both sides at once would need 2h > 1 cell)
+            const int dj = ax.apn != (R)0 ? 1 : -1;
+            const R coef = ax.apn != (R)0 ? ax.apn : ax.amp;
+            int ci[3] = {c[0], c[1], c[2]};
             ci[j] += dj;
             R cn[3][8], Bn[3], dn[3][3];
             f.template load<R>(ci[0], ci[1], ci[2], cn);
-            cell_eval<R>(cn, ax[0].w, ax[1].w, ax[2].w, Bn, dn);
+            cell_eval<R>(cn, w[0], w[1], w[2], Bn, dn);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 if (dj > 0) ap[q] += coef * dn[j][q];
                 else am[q] += coef * dn[j][q];
             }
-            // a sample may cross on both sides only if 2h > 1 cell: impossible for h = 1e-3
         }
         fd_quotient<R>(B, ap, am, J[j]);
     }
-    const R inb = rsq(dot3(B, B));
-#pragma unroll
-    for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
 }
 
 // Unit vector b only (plain tracer RHS: Interp3dxyzn(..., true)).
@@ -443,112 +482,131 @@ struct ScottParams {
     double tol;     // RKF45 absolute per-step tolerance, cells
 };
 
+// Status of a line after init / attempt.
+enum : int { kTracing = 0, kFinished = 1, kAborted = 2 };
+
+// Footpoint of one side: position, deviation vectors u, v, class.
 template <typename R>
-struct ScottSide {
-    double x[3];
-    R u[3], v[3];
+struct ScottEnd {
+    R x[3], u[3], v[3];
+    int flag;
 };
 
-// State of one Scott line (TraceBlineScott, TraceBlineScott.cu:334-519).
+// State of one Scott line (TraceBlineScott, TraceBlineScott.cu:334-519).  Only
+// what a line carries from one attempt to the next lives here; the seed frame
+// (b0, u0, v0) is recomputed when the second direction starts and the end
+// footpoint is handed back to the caller when the line finishes.
 template <typename R>
 struct ScottLine {
-    // seed
     float p0[3];
-    R u0[3], v0[3];
-    double nb0;
-    float b0z;
-    // current trace
     Pos<R> x;
     R u[3], v[3];
-    double s, len, len_lim;
+    double s, len;
     int steps, attempts, dir;  // dir: -1 backward (first), +1 forward
-    int flag_s, flag_e;
-    bool done, aborted;
     uint32_t evals;
-    ScottSide<R> ss, se;  // start (backward) and end (forward) footpoint states
+    ScottEnd<R> st;            // start (backward) footpoint once known
+    const R* fr;               // precomputed seed frame (u0, v0, b0z, |B0|) or nullptr
 };
 
-// Record the current state as the footpoint of the active side.
-template <typename R>
-FQSL_HD void scott_record(ScottLine<R>& L, const double* x, const R* u, const R* v, int flag) {
-    ScottSide<R>& S = L.dir < 0 ? L.ss : L.se;
+// Seed frame record written by a preparation pass: u0[3], v0[3], b0z, |B0|.
+constexpr int kFrameWords = 8;
+
+// Unit field b0 at the seed, |B0| and the initial deviation vectors
+// v0 = normalize((b.y, -b.x, 0) - ((b.y,-b.x,0).b) b), u0 = normalize(b x v0)
+// (TraceBlineScott.cu:368-386).
+template <typename R, class F>
+FQSL_HD void seed_frame(const F& f, const float* p0, R u0[3], R v0[3], R* b0z, double* nb0) {
+    const double p0d[3] = {p0[0], p0[1], p0[2]};
+    double B0[3];
+    raw_b<R>(f, p0d, B0);
+    if (nb0) *nb0 = sqrt(B0[0] * B0[0] + B0[1] * B0[1] + B0[2] * B0[2]);
+    R Bv[3] = {(R)B0[0], (R)B0[1], (R)B0[2]};
+    const R inb = rsq(dot3(Bv, Bv));
+    const R b0[3] = {Bv[0] * inb, Bv[1] * inb, Bv[2] * inb};
+    *b0z = b0[2];
+    R v[3] = {b0[1], -b0[0], (R)0};
+    const R dv = dot3(v, b0);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { S.x[a] = x[a]; S.u[a] = u[a]; S.v[a] = v[a]; }
-    if (L.dir < 0) L.flag_s = flag; else L.flag_e = flag;
+    for (int a = 0; a < 3; ++a) v[a] -= dv * b0[a];
+    const R iv = rsq(dot3(v, v));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] *= iv;
+    R u[3] = {b0[1] * v[2] - b0[2] * v[1], b0[2] * v[0] - b0[0] * v[2], b0[0] * v[1] - b0[1] * v[0]};
+    const R iu = rsq(dot3(u, u));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { u0[a] = u[a] * iu; v0[a] = v[a]; }
+}
+
+// Record a footpoint for the active side (start when dir < 0, else `e`).
+template <typename R>
+FQSL_HD void scott_record(ScottLine<R>& L, ScottEnd<R>& e, const double* x, const R* u, const R* v, int flag) {
+    ScottEnd<R>& S = L.dir < 0 ? L.st : e;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { S.x[a] = (R)x[a]; S.u[a] = u[a]; S.v[a] = v[a]; }
+    S.flag = flag;
 }
 
 template <typename R>
-FQSL_HD bool scott_can_step(const ScottLine<R>& L) {
-    return L.len < L.len_lim && L.steps < kScottStepLim && L.attempts < kAttemptCap;
+FQSL_HD bool scott_can_step(const ScottLine<R>& L, int len_lim) {
+    return L.len < (double)len_lim && L.steps < kScottStepLim && L.attempts < kAttemptCap;
 }
 
-// Start direction L.dir from the seed; handles the z0 shortcut
-// (TraceBlineScott.cu:396-401) and directions that cannot take a step
-// because the shared length/step budget is spent.  Advances to the next
-// direction or marks the line done as needed.
-template <typename R>
-FQSL_HD void scott_begin_dir(ScottLine<R>& L, const ScottParams& P) {
+template <class F>
+FQSL_HD int scott_len_lim(const F& f) {
+    return (int)kScottLenRatio * (f.n[0] + f.n[1] + f.n[2]);
+}
+
+// Start direction L.dir from the seed: the z0 shortcut
+// (TraceBlineScott.cu:396-401) and directions that cannot take a step because
+// the shared length/step budget is spent record their footpoint at once.
+// Returns kTracing, or kFinished once the forward side is recorded.
+template <typename R, class F>
+FQSL_COLD int scott_begin_dir(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
     for (;;) {
+        R u0[3], v0[3], b0z;
+        if (L.fr) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { u0[a] = L.fr[a]; v0[a] = L.fr[3 + a]; }
+            b0z = L.fr[6];
+        } else {
+            seed_frame<R>(f, L.p0, u0, v0, &b0z, nullptr);
+        }
         const bool z0 = L.p0[2] < 1e-6f;
         L.x.set_f(L.p0);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { L.u[a] = L.u0[a]; L.v[a] = L.v0[a]; }
+        for (int a = 0; a < 3; ++a) { L.u[a] = u0[a]; L.v[a] = v0[a]; }
         const double px[3] = {L.p0[0], L.p0[1], L.p0[2]};
-        if (z0 && (double)L.b0z * L.dir < 0) {
-            scott_record(L, px, L.u0, L.v0, 5);
+        if (z0 && (double)b0z * L.dir < 0) {
+            scott_record(L, e, px, u0, v0, 5);
         } else {
             L.s = fabs(L.s);
-            if (scott_can_step(L)) return;  // trace this direction
-            scott_record(L, px, L.u0, L.v0, kClassLimit);
+            if (scott_can_step(L, scott_len_lim(f))) return kTracing;
+            scott_record(L, e, px, u0, v0, kClassLimit);
         }
         if (L.dir < 0) { L.dir = 1; continue; }
-        L.done = true;
-        return;
+        return kFinished;
     }
 }
 
 template <typename R, class F>
-FQSL_HD void scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float px, float py, float pz) {
+FQSL_COLD int scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float px, float py, float pz,
+                         ScottEnd<R>& e, const R* frame = nullptr) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
+    L.fr = frame;
     L.len = 0.0;
-    L.len_lim = kScottLenRatio * (f.n[0] + f.n[1] + f.n[2]);
     L.steps = 0; L.attempts = 0; L.evals = 0;
     L.s = P.s0;
-    L.flag_s = L.flag_e = kClassLimit;
-    L.done = false; L.aborted = false;
     L.dir = -1;
-    const double p0d[3] = {px, py, pz};
     if (pz < -1e-8f) {  // TraceBlineScott.cu:366
-        L.flag_s = L.flag_e = kClassBelow;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { L.ss.x[a] = p0d[a]; L.se.x[a] = p0d[a]; }
-        L.done = true;
-        return;
+        for (int a = 0; a < 3; ++a) { L.st.x[a] = (R)L.p0[a]; e.x[a] = (R)L.p0[a]; }
+        L.st.flag = e.flag = kClassBelow;
+        return kFinished;
     }
-    double B0[3];
-    raw_b<R>(f, p0d, B0);
-    R Bv[3] = {(R)B0[0], (R)B0[1], (R)B0[2]};
-    L.nb0 = sqrt(B0[0] * B0[0] + B0[1] * B0[1] + B0[2] * B0[2]);
-    const R inb = rsq(dot3(Bv, Bv));
-    R b0[3] = {Bv[0] * inb, Bv[1] * inb, Bv[2] * inb};
-    L.b0z = (float)b0[2];
-    // v0 = normalize((b.y, -b.x, 0) - ((b.y,-b.x,0).b) b); u0 = normalize(b x v0)
-    // (TraceBlineScott.cu:380-386)
-    R v0[3] = {b0[1], -b0[0], (R)0};
-    const R dv = dot3(v0, b0);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v0[a] -= dv * b0[a];
-    const R iv = rsq(dot3(v0, v0));
-#pragma unroll
-    for (int a = 0; a < 3; ++a) v0[a] *= iv;
-    R u0[3] = {b0[1] * v0[2] - b0[2] * v0[1], b0[2] * v0[0] - b0[0] * v0[2], b0[0] * v0[1] - b0[1] * v0[0]};
-    const R iu = rsq(dot3(u0, u0));
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { L.u0[a] = u0[a] * iu; L.v0[a] = v0[a]; }
-    scott_begin_dir(L, P);
+    return scott_begin_dir<R>(L, f, e);
 }
 
-// Right-hand side for a stage: k = s * f(y + delta), returns the 9-vector.
+// Right-hand side for a stage: k = s * f(y + delta) for the 9-vector.
 template <typename R, class F>
 FQSL_HD void scott_stage(const F& f, const ScottLine<R>& L, const R* dx, const R* du, const R* dv,
                          R sc, R k[9]) {
@@ -566,12 +624,23 @@ FQSL_HD void scott_stage(const F& f, const ScottLine<R>& L, const R* dx, const R
     }
 }
 
-// One integrator attempt (one pass of the while loop at TraceBlineScott.cu:407-470).
+// Close the active side; returns the line status after it.
 template <typename R, class F>
-FQSL_HD void scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P) {
+FQSL_HD int scott_side_done(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
+    if (L.dir < 0) {
+        L.dir = 1;
+        return scott_begin_dir<R>(L, f, e);
+    }
+    return kFinished;
+}
+
+// One integrator attempt (one pass of the while loop at
+// TraceBlineScott.cu:407-470).  Returns kTracing, kFinished (both footpoints
+// known: start in L.st, end in e) or kAborted (class 42: Q = 0).
+template <typename R, class F>
+FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, ScottEnd<R>& e) {
     const R sc = (R)(L.s * L.dir);
     R inc[9];
-    double err = 0.0;
     L.attempts += 1;
     if (P.method == kRKF45) {
         R k1[9], k2[9], k3[9], k4[9], k5[9], k6[9], t[9];
@@ -602,22 +671,21 @@ FQSL_HD void scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P) {
         double e2 = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double e = Tab::e1 * (double)k1[c] + Tab::e3 * (double)k3[c] + Tab::e4 * (double)k4[c] +
-                             Tab::e5 * (double)k5[c] + Tab::e6 * (double)k6[c];
-            e2 += e * e;
+            const double ec = Tab::e1 * (double)k1[c] + Tab::e3 * (double)k3[c] + Tab::e4 * (double)k4[c] +
+                              Tab::e5 * (double)k5[c] + Tab::e6 * (double)k6[c];
+            e2 += ec * ec;
         }
-        err = sqrt(e2);
         L.evals += 6;
         // step controller (TraceBlineScott.cu:413-419), FP64
-        const double scale = pow(P.tol / err / 11.09, 0.2);
+        const double scale = pow(P.tol / sqrt(e2) / 11.09, 0.2);
         if (scale < 0.618) {
             L.s = L.s * 0.618;
-            if (!scott_can_step(L)) {  // attempt budget exhausted: limit exit
-                double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
-                scott_record(L, xa, L.u, L.v, kClassLimit);
-                if (L.dir < 0) { L.dir = 1; scott_begin_dir(L, P); } else L.done = true;
+            if (!scott_can_step(L, scott_len_lim(f))) {  // attempt budget spent: limit exit
+                const double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
+                scott_record(L, e, xa, L.u, L.v, kClassLimit);
+                return scott_side_done<R>(L, f, e);
             }
-            return;
+            return kTracing;
         }
         L.s = L.s * scale;
         if (L.s > 100.) L.s = 100.;
@@ -647,9 +715,6 @@ FQSL_HD void scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P) {
     Pos<R> xb = L.x;
     xb.add(inc);
     const double xbv[3] = {xb.get(0), xb.get(1), xb.get(2)};
-    R ub[3], vb[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { ub[a] = L.u[a] + inc[3 + a]; vb[a] = L.v[a] + inc[6 + a]; }
     const double dL = sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
     L.len += dL;
     const int flag = check_flag(f.n, xbv);
@@ -668,58 +733,61 @@ FQSL_HD void scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     xl[a] = (xa[a] * (p2 - pm) + xbv[a] * (pm - p1)) / (p2 - p1);
-                    ul[a] = (R)((double)L.u[a] * wa + (double)ub[a] * wb);
-                    vl[a] = (R)((double)L.v[a] * wa + (double)vb[a] * wb);
+                    ul[a] = (R)((double)L.u[a] * wa + (double)(L.u[a] + inc[3 + a]) * wb);
+                    vl[a] = (R)((double)L.v[a] * wa + (double)(L.v[a] + inc[6 + a]) * wb);
                 }
-                scott_record(L, xl, ul, vl, flag);
                 // arc length to the face with b at the last in-box state
                 R ba[3];
                 const R zero3[3] = {0, 0, 0};
                 unit_b<R>(f, L.x.stage(zero3), ba);
                 L.len += fabs(pm - p1) / (1e-4 + fabs((double)ba[dim]));
+                scott_record(L, e, xl, ul, vl, flag);
             } else {
-                scott_record(L, xa, L.u, L.v, flag);
+                scott_record(L, e, xa, L.u, L.v, flag);
             }
             L.steps += 1;
-            if (L.dir < 0) { L.dir = 1; scott_begin_dir(L, P); } else L.done = true;
-        } else {
-            // class 42 (TraceBlineScott.cu:460-465): q = 0, the line stops here
-            scott_record(L, xa, L.u, L.v, kClassOther);
-            if (L.dir < 0) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) L.se.x[a] = L.p0[a];
-                L.flag_e = kClassOther;
-            }
-            L.aborted = true;
-            L.done = true;
+            return scott_side_done<R>(L, f, e);
         }
-        return;
+        // class 42 (TraceBlineScott.cu:460-465): q = 0, the line stops here;
+        // a forward side never traced keeps the seed
+        scott_record(L, e, xa, L.u, L.v, kClassOther);
+        if (L.dir < 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { e.x[a] = (R)L.p0[a]; e.u[a] = 0; e.v[a] = 0; }
+            e.flag = kClassOther;
+        }
+        return kAborted;
     }
     L.x = xb;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { L.u[a] = ub[a]; L.v[a] = vb[a]; }
+    for (int a = 0; a < 3; ++a) { L.u[a] += inc[3 + a]; L.v[a] += inc[6 + a]; }
     L.steps += 1;
-    if (!scott_can_step(L)) {  // length / step limit: class 0, last state
-        scott_record(L, xbv, ub, vb, kClassLimit);
-        if (L.dir < 0) { L.dir = 1; scott_begin_dir(L, P); } else L.done = true;
+    if (!scott_can_step(L, scott_len_lim(f))) {  // length / step limit: class 0, last state
+        scott_record(L, e, xbv, L.u, L.v, kClassLimit);
+        return scott_side_done<R>(L, f, e);
     }
+    return kTracing;
 }
 
-// Q assembly (TraceBlineScott.cu:475-508) in FP64.
+// Q assembly (TraceBlineScott.cu:475-508) in FP64 from the start footpoint
+// st, the end footpoint e and |B0| at the seed.
 template <typename R, class F>
-FQSL_HD void scott_q(const ScottLine<R>& L, const F& f, double* q0, double* qperp) {
-    if (L.aborted) { *q0 = 0.0; *qperp = 0.0; return; }
-    if (L.flag_s < 1 || L.flag_s > 6 || L.flag_e < 1 || L.flag_e > 6) {
+FQSL_COLD void scott_q(const ScottEnd<R>& st, const ScottEnd<R>& e, int status, double nb0, const F& f,
+                       double* q0, double* qperp) {
+    if (status == kAborted) { *q0 = 0.0; *qperp = 0.0; return; }
+    if (st.flag < 1 || st.flag > 6 || e.flag < 1 || e.flag > 6) {
         *q0 = NAN; *qperp = NAN; return;
     }
     double Bs[3], Be[3];
-    raw_b<R>(f, L.ss.x, Bs);
-    raw_b<R>(f, L.se.x, Be);
-    const int ds = (L.flag_s - 1) / 2, de = (L.flag_e - 1) / 2;
+    const double xs[3] = {(double)st.x[0], (double)st.x[1], (double)st.x[2]};
+    const double xe[3] = {(double)e.x[0], (double)e.x[1], (double)e.x[2]};
+    raw_b<R>(f, xs, Bs);
+    raw_b<R>(f, xe, Be);
+    const int ds = (st.flag - 1) / 2, de = (e.flag - 1) / 2;
     double us[3], vs[3], ue[3], ve[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        us[a] = L.ss.u[a]; vs[a] = L.ss.v[a]; ue[a] = L.se.u[a]; ve[a] = L.se.v[a];
+        us[a] = st.u[a]; vs[a] = st.v[a]; ue[a] = e.u[a]; ve[a] = e.v[a];
     }
     double us1[3], vs1[3], ue1[3], ve1[3];
     const double BNs = Bs[ds], BNe = Be[de];
@@ -730,7 +798,7 @@ FQSL_HD void scott_q(const ScottLine<R>& L, const F& f, double* q0, double* qper
         ue1[a] = ue[a] - ue[de] / BNe * Be[a];
         ve1[a] = ve[a] - ve[de] / BNe * Be[a];
     }
-    const double nb2 = L.nb0 * L.nb0;
+    const double nb2 = nb0 * nb0;
     double num = dot3(ue1, ue1) * dot3(vs1, vs1) + dot3(us1, us1) * dot3(ve1, ve1) -
                  2.0 * dot3(ue1, ve1) * dot3(us1, vs1);
     *q0 = fabs(num) / fabs(nb2 / (BNs * BNe));
@@ -784,7 +852,7 @@ FQSL_HD void plain_record(PlainLine<R>& L, const double* x, int flag) {
 
 // Begin the direction L.dir from the seed with a fresh step and budget.
 template <typename R, class F>
-FQSL_HD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P) {
+FQSL_COLD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P) {
     for (;;) {
         L.x.set_f(L.p0);
         L.s = P.s0;
@@ -801,7 +869,7 @@ FQSL_HD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P) 
 }
 
 template <typename R, class F>
-FQSL_HD void plain_init(PlainLine<R>& L, const F& f, const PlainParams& P, float px, float py, float pz) {
+FQSL_COLD void plain_init(PlainLine<R>& L, const F& f, const PlainParams& P, float px, float py, float pz) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
     const bool rkf = P.method == kRKF45;
     L.len_lim = (rkf ? kPlainRkf45LenRatio : kPlainRk4LenRatio) * (f.n[0] + f.n[1] + f.n[2]);
@@ -831,7 +899,7 @@ FQSL_HD void plain_finish_dir(PlainLine<R>& L, const F& f, const PlainParams& P)
 
 // RK4_boundary (TraceBline.cu:253-274): integrate dx/dx_dim = b / b_dim over s.
 template <typename R, class F>
-FQSL_HD void rk4_boundary(const F& f, const double* p, double s, int dim, double out[3]) {
+FQSL_COLD void rk4_boundary(const F& f, const double* p, double s, int dim, double out[3]) {
     double k[4][3], t[3];
     const double cf[4] = {0.0, 0.5, 0.5, 1.0};
 #pragma unroll

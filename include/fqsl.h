@@ -170,7 +170,9 @@ int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, co
  * A set curB_flag (twist) is not implemented: twist[] is filled with NaN. */
 
 /* TraceAllBlineScott (TraceBlineScott.cu:521-528).  tol_coef scales the
- * per-step tolerance 1e-3 * tol_coef.  precision / schedule select the path. */
+ * per-step tolerance 1e-3 * tol_coef.  precision / schedule select the path;
+ * the persistent schedule reads N back to the host (one stream
+ * synchronisation) to size its workspace. */
 int fqsl_TraceAllBlineScott(float *Bx, float *By, float *Bz, int *BshapeN, float *curB_x,
                             float *curB_y, float *curB_z, double *twist, bool *curB_flag,
                             float *inp_x, float *inp_y, float *inp_z, float *inp_cross_dir,
@@ -191,6 +193,18 @@ int fqsl_TraceAllBline(int method, float *Bx, float *By, float *Bz, int *BshapeN
                        float *B_start_z, float *B_end_x, float *B_end_y, float *B_end_z,
                        float *s_len, unsigned long long *N, double *LineLen, float *tol_coef,
                        int precision, int schedule, void *stream);
+
+/* ---------------------------------------------------------- tuning knobs */
+/* Process-wide launch parameters picked by the autotuner (fastqsl_amd.tune).
+ * They change speed only, never results. */
+#define FQSL_TUNE_SCOTT_WAVES 0 /* waves/SIMD the FP32 persistent Scott kernel is compiled for:
+                                 0 = compiler's choice (default), 2 or 3 */
+#define FQSL_TUNE_GRID_PCT 1    /* persistent grid size in % of one full occupancy wave (default 100) */
+#define FQSL_TUNE_QUEUES 2      /* seed queues of the persistent Scott kernel: 1 (default) or 8 (one per XCD) */
+#define FQSL_TUNE_REFILL_MIN 3  /* idle lanes a wavefront gathers before it refills them (1..64, default 16) */
+#define FQSL_TUNE_COUNT 4
+int fqsl_tune_set(int knob, int value);
+int fqsl_tune_get(int knob, int *value);
 
 /* -------------------------------------------------------------- timing */
 /* Duration in milliseconds of the most recent trace kernel launched by

@@ -36,11 +36,18 @@ void run_scott(const F& f, const ScottParams& P, long n, const float* seeds, dou
                int* fs, int* fe, double* q0, double* qp, double* len, long* evals) {
     for (long i = 0; i < n; ++i) {
         ScottLine<R> L;
-        scott_init<R>(L, f, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2]);
-        while (!L.done) scott_attempt<R>(L, f, P);
-        scott_q<R>(L, f, &q0[i], &qp[i]);
-        for (int a = 0; a < 3; ++a) { out_start[3 * i + a] = L.ss.x[a]; out_end[3 * i + a] = L.se.x[a]; }
-        fs[i] = L.flag_s; fe[i] = L.flag_e; len[i] = L.len; evals[i] = L.evals;
+        ScottEnd<R> e;
+        int st = scott_init<R>(L, f, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2], e);
+        while (st == kTracing) st = scott_attempt<R>(L, f, P, e);
+        double nb0 = 0.0;
+        {
+            R u0[3], v0[3], b0z;
+            seed_frame<R>(f, L.p0, u0, v0, &b0z, &nb0);
+            nb0 = (double)(R)nb0;
+        }
+        scott_q<R>(L.st, e, st, nb0, f, &q0[i], &qp[i]);
+        for (int a = 0; a < 3; ++a) { out_start[3 * i + a] = L.st.x[a]; out_end[3 * i + a] = e.x[a]; }
+        fs[i] = L.st.flag; fe[i] = e.flag; len[i] = L.len; evals[i] = L.evals;
     }
 }
 }  // namespace
@@ -105,39 +112,3 @@ void hc_trace_plain(const double* bx, const double* by, const double* bz, int nx
 }
 }
 
-extern "C" int hc_scott_debug(const double* bx, const double* by, const double* bz, int nx, int ny, int nz,
-                              int method, double s0, double tol, const float* seed, int max_rec, double* rec) {
-    // per attempt: s used, position after, evals so far
-    ScottParams P{method, s0, tol};
-    FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
-    ScottLine<double> L;
-    scott_init<double>(L, f, P, seed[0], seed[1], seed[2]);
-    int k = 0;
-    while (!L.done && k < max_rec) {
-        const double s_used = L.s * L.dir;
-        scott_attempt<double>(L, f, P);
-        rec[5 * k + 0] = s_used;
-        rec[5 * k + 1] = L.x.get(0);
-        rec[5 * k + 2] = L.x.get(1);
-        rec[5 * k + 3] = L.x.get(2);
-        rec[5 * k + 4] = L.s;
-        ++k;
-    }
-    return k;
-}
-
-extern "C" double hc_scott_step(const double* bx, const double* by, const double* bz, int nx, int ny, int nz,
-                                int method, const double* y, double s, double* out) {
-    // one accepted-or-not attempt from state y with step s (sign = direction)
-    ScottParams P{method, fabs(s), 1e300};
-    FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
-    ScottLine<double> L;
-    L.x.set(y);
-    for (int a = 0; a < 3; ++a) { L.u[a] = y[3 + a]; L.v[a] = y[6 + a]; }
-    L.s = fabs(s); L.dir = s < 0 ? -1 : 1; L.len = 0; L.len_lim = 1e300; L.steps = 0; L.attempts = 0;
-    L.evals = 0; L.done = false; L.aborted = false; L.flag_s = L.flag_e = 0;
-    L.p0[0] = (float)y[0]; L.p0[1] = (float)y[1]; L.p0[2] = (float)y[2];
-    scott_attempt<double>(L, f, P);
-    for (int a = 0; a < 3; ++a) { out[a] = L.x.get(a); out[3 + a] = L.u[a]; out[6 + a] = L.v[a]; }
-    return L.s;
-}

@@ -8,10 +8,11 @@ under a perturbation far below float32 resolution, see parity.conditioned):
   FP32-compensated path -- classes match on >= 99.99 %, |dlog10 Q| <= 1e-3
   on >= 99.9 %.
 * RKF45 (random multipole, 256^3 float32, tol 1e-6): the reference's
-  controller forks on round-off, so the oracle's own result moves when its
-  tolerance is scaled by 1 + 1e-12.  The GPU must stay within that noise
-  floor: classes and the |dlog10 Q| <= 1e-3 fraction no worse than the
-  oracle-vs-perturbed-oracle figures (minus a small sampling allowance).
+  controller forks on round-off (the oracle's own Q moves by > 1e-3 on about
+  a third of the lines when its tolerance is scaled by 1 + 1e-12), so a
+  line-by-line 1e-3 bar cannot hold for any other implementation.  The GPU
+  must be as accurate as the oracle against the converged answer (tol 1e-10)
+  and match its classes within the oracle's own round-off sensitivity.
 """
 import numpy as np
 import pytest
@@ -120,28 +121,43 @@ def c2(oracle):
     seeds = np.column_stack([g[ij % n], g[ij // n], np.zeros(ij.size, np.float32)]).astype(np.float32)
     ref = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6)
     refp = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6 * (1 + 1e-12))
-    return field, seeds, ref, refp
+    truth = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-10)
+    return field, seeds, ref, refp, truth
 
 
-def test_c2_rkf45_within_oracle_noise_floor(fq, c2):
-    field, seeds, ref, refp = c2
+def test_c2_rkf45_as_accurate_as_the_oracle(fq, c2):
+    """RKF45 at tol 1e-6: the GPU's Q is as close to the converged Q (tol
+    1e-10) as the FP64 oracle's own tol-1e-6 result is, and its classes match
+    the oracle's within the oracle's own round-off sensitivity."""
+    field, seeds, ref, refp, truth = c2
     dfl = parity.log_q_diff(refp["q0"], ref["q0"])
     cfl = parity.class_match(refp["flag_start"], refp["flag_end"], ref["flag_start"], ref["flag_end"])
     floor = parity.summary(dfl, cfl)
+    same = parity.class_match(ref["flag_start"], ref["flag_end"], truth["flag_start"], truth["flag_end"])
+    err_orc = parity.log_q_diff(ref["q0"], truth["q0"])
     print("oracle vs perturbed oracle:", floor)
+    print("oracle(1e-6) vs converged:", parity.summary(err_orc, same))
     with fq.Field(*field) as f:
         for prec in ("fp64", "fp32c"):
             r = _trace(fq, f, seeds, method="rkf45", step=0.25, tol=1e-6, precision=prec)
             cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
             d = parity.log_q_diff(r["q_0"], ref["q0"])
-            s = parity.summary(d, cls)
-            print(prec, s)
-            assert s["class_match"] >= floor["class_match"] - 2e-3
-            assert s["frac_le_1e3"] >= floor["frac_le_1e3"] - 0.05
-            assert s["median"] <= 2.0 * floor["median"] + 1e-5
+            print(prec, "vs oracle:", parity.summary(d, cls))
+            cls_t = parity.class_match(r["flag_start"], r["flag_end"], truth["flag_start"], truth["flag_end"])
+            err_gpu = parity.log_q_diff(r["q_0"], truth["q0"])
+            m = same & cls_t & np.isfinite(err_orc) & np.isfinite(err_gpu)
+            print(prec, "vs converged:", parity.summary(err_gpu, cls_t),
+                  "median ratio %.3f" % (np.median(err_gpu[m]) / np.median(err_orc[m])))
+            assert cls.mean() >= floor["class_match"] - 2e-3
+            assert np.median(err_gpu[m]) <= 1.5 * np.median(err_orc[m]) + 1e-6
+            assert np.quantile(err_gpu[m], 0.99) <= 1.5 * np.quantile(err_orc[m], 0.99) + 1e-6
+            # The FP32 error estimate has a round-off floor (~1e-8 cells), so
+            # when the true error is far below the tolerance the controller
+            # grows the step less and wastes fewer rejected attempts: fewer
+            # evaluations for the same accuracy against the converged answer.
             ev = r["evals"].astype(np.float64).sum() / ref["evals"].sum()
             print(prec, "evals ratio vs oracle %.4f" % ev)
-            assert 0.8 < ev < 1.25
+            assert (0.95 < ev < 1.05) if prec == "fp64" else (0.6 < ev < 1.1)
 
 
 def test_hyperbolic_closed_form_gpu(fq):
