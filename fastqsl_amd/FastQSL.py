@@ -1,0 +1,188 @@
+"""Drop-in for ``fastqslpy.FastQSL`` (fastqslpy/FastQSL.py) plus the Qcut
+assembly the demo notebooks do inline (demo/Q_RKF45CuPy.ipynb cells [10]-[11]).
+
+``QCalcPlane``, ``CookPseudoLine`` and ``ResReshape`` keep the reference's
+positional signatures and return values.  Inputs may be torch CUDA tensors or
+NumPy arrays; results come back in the same kind (CUDA tensors stay on the
+device).  The arithmetic runs in the HIP kernels of libfqsl.so
+(fqsl_qcalc_plane, fqsl_pseudo_seeds, fqsl_qcut_scatter) and follows the
+reference's float32 expression trees operation by operation.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class _Dev:
+    """Device views of a mix of NumPy arrays and CUDA tensors."""
+
+    def __init__(self):
+        self.any_numpy = False
+        self.keep = []
+
+    def __call__(self, a, dtype):
+        torch = _torch()
+        tdt = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+        if isinstance(a, torch.Tensor):
+            t = a.to(device="cuda", dtype=tdt).contiguous()
+        else:
+            self.any_numpy = True
+            t = torch.as_tensor(np.ascontiguousarray(np.asarray(a, dtype=dtype)), device="cuda")
+        self.keep.append(t)
+        return t
+
+    def out(self, t):
+        return t.cpu().numpy() if self.any_numpy else t
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def ResReshape(shape, *args):
+    """Reshape flat per-line outputs to the seed-grid shape (FastQSL.py:59-63)."""
+    return [a.reshape(shape) for a in args]
+
+
+def QCalcPlane(x_end_arr, y_end_arr, z_end_arr, flag_end_arr,
+               x_start_arr, y_start_arr, z_start_arr, flag_start_arr,
+               Bx_in_arr, By_in_arr, Bz_in_arr,
+               Bx_out_arr, By_out_arr, Bz_out_arr,
+               Bx_0_arr, By_0_arr, Bz_0_arr,
+               B_flag_arr, stride_step, CrossBC_dir=(0.0, 0.0, 1.0)):
+    """Neighbour-difference Q on a (ny, nx) seed grid (FastQSL.py:65-117);
+    returns (ny-2, nx-2) float32 with Q < 1 clamped to 1 and NaN where a
+    neighbour has another start or end class."""
+    torch = _torch()
+    d = _Dev()
+    ny, nx = np.shape(flag_end_arr)
+    f32, i32 = np.float32, np.int32
+    xe, ye, ze = d(x_end_arr, f32), d(y_end_arr, f32), d(z_end_arr, f32)
+    fe = d(flag_end_arr, i32)
+    xs, ys, zs = d(x_start_arr, f32), d(y_start_arr, f32), d(z_start_arr, f32)
+    fs = d(flag_start_arr, i32)
+    bzin, bzout = d(Bz_in_arr, f32), d(Bz_out_arr, f32)
+    b0x, b0y, b0z = d(Bx_0_arr, f32), d(By_0_arr, f32), d(Bz_0_arr, f32)
+    cd = CrossBC_dir.detach().cpu().numpy() if isinstance(CrossBC_dir, torch.Tensor) else CrossBC_dir
+    cdh = (ctypes.c_float * 3)(*[float(v) for v in np.asarray(cd, np.float32).reshape(3)])
+    q = torch.empty((ny - 2, nx - 2), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.lib().fqsl_qcalc_plane(
+        int(ny), int(nx), xe.data_ptr(), ye.data_ptr(), ze.data_ptr(), fe.data_ptr(),
+        xs.data_ptr(), ys.data_ptr(), zs.data_ptr(), fs.data_ptr(), bzin.data_ptr(),
+        bzout.data_ptr(), b0x.data_ptr(), b0y.data_ptr(), b0z.data_ptr(), float(stride_step),
+        cdh, q.data_ptr(), _stream()))
+    return d.out(q)
+
+
+def cook_pseudo_lines(x_start, y_start, z_start, flag_start, bz_start, bz_end, b_flag, stride):
+    """Device-side CookPseudoLine core: four seeds per b_flag == 1 pixel (in
+    pixel order) around its start footpoint.  Returns a dict of CUDA tensors:
+    seed_x/y/z (4m), bz0_start, bz0_end (m), pixel (m, flat pixel index)."""
+    torch = _torch()
+    d = _Dev()
+    f32, i32 = np.float32, np.int32
+    bf = d(b_flag, i32).reshape(-1)
+    xs, ys, zs = (d(a, f32).reshape(-1) for a in (x_start, y_start, z_start))
+    fs = d(flag_start, i32).reshape(-1)
+    bzs, bze = d(bz_start, f32).reshape(-1), d(bz_end, f32).reshape(-1)
+    n = bf.numel()
+    L = _lib.lib()
+    cnt = ctypes.c_uint64()
+    null = ctypes.c_void_p()
+    _lib.check(L.fqsl_pseudo_seeds(n, bf.data_ptr(), xs.data_ptr(), ys.data_ptr(), zs.data_ptr(),
+                                   fs.data_ptr(), bzs.data_ptr(), bze.data_ptr(),
+                                   ctypes.c_float(float(stride)), 0, null, null, null, null, null,
+                                   null, ctypes.byref(cnt), _stream()))
+    m = int(cnt.value)
+    out = dict(seed_x=torch.empty(4 * m, dtype=torch.float32, device="cuda"),
+               seed_y=torch.empty(4 * m, dtype=torch.float32, device="cuda"),
+               seed_z=torch.empty(4 * m, dtype=torch.float32, device="cuda"),
+               bz0_start=torch.empty(m, dtype=torch.float32, device="cuda"),
+               bz0_end=torch.empty(m, dtype=torch.float32, device="cuda"),
+               pixel=torch.empty(m, dtype=torch.int64, device="cuda"))
+    if m:
+        _lib.check(L.fqsl_pseudo_seeds(n, bf.data_ptr(), xs.data_ptr(), ys.data_ptr(), zs.data_ptr(),
+                                       fs.data_ptr(), bzs.data_ptr(), bze.data_ptr(),
+                                       ctypes.c_float(float(stride)), m, out["seed_x"].data_ptr(),
+                                       out["seed_y"].data_ptr(), out["seed_z"].data_ptr(),
+                                       out["bz0_start"].data_ptr(), out["bz0_end"].data_ptr(),
+                                       out["pixel"].data_ptr(), ctypes.byref(cnt), _stream()))
+    return out
+
+
+def CookPseudoLine(x_end_arr, y_end_arr, z_end_arr, flag_end_arr,
+                   x_start_arr, y_start_arr, z_start_arr, flag_start_arr,
+                   Bx_in_arr, By_in_arr, Bz_in_arr,
+                   Bx_out_arr, By_out_arr, Bz_out_arr,
+                   Bz_0_arr, B_flag_arr, stride_this):
+    """FastQSL.py:4-57: seeds for the pseudo lines and zeroed output arrays
+    for the second TraceAllBline launch, in the reference's 25-tuple order."""
+    torch = _torch()
+    c = cook_pseudo_lines(x_start_arr, y_start_arr, z_start_arr, flag_start_arr, Bz_in_arr,
+                          Bz_out_arr, B_flag_arr, stride_this)
+    n4 = c["seed_x"].numel()
+    z = lambda dt=torch.float32: torch.zeros(n4, dtype=dt, device="cuda")
+    (cut_start_x, cut_start_y, cut_start_z, cut_end_x, cut_end_y, cut_end_z, Bx_start_cut,
+     By_start_cut, Bz_start_cut, Bx_end_cut, By_end_cut, Bz_end_cut, Bx_inp_cut, By_inp_cut,
+     Bz_inp_cut) = [z() for _ in range(15)]
+    line_len_cut = z(torch.float64)
+    N_cut = torch.tensor([n4], dtype=torch.int64, device="cuda")  # unsigned long long on the device
+    B_flag_cut, flag_cut_start, flag_cut_end = z(torch.int32), z(torch.int32), z(torch.int32)
+    res = (c["seed_x"], c["seed_y"], c["seed_z"],
+           cut_start_x, cut_start_y, cut_start_z, flag_cut_start,
+           cut_end_x, cut_end_y, cut_end_z, flag_cut_end,
+           Bx_inp_cut, By_inp_cut, Bz_inp_cut, B_flag_cut,
+           Bx_start_cut, By_start_cut, Bz_start_cut,
+           Bx_end_cut, By_end_cut, Bz_end_cut, N_cut, line_len_cut,
+           c["bz0_start"], c["bz0_end"])
+    if not isinstance(B_flag_arr, torch.Tensor):
+        res = tuple(t.cpu().numpy() for t in res)
+        res = res[:21] + (res[21].astype(np.uint64),) + res[22:]
+    return res
+
+
+def Qcut(cut_start_x, cut_start_y, cut_start_z, flag_cut_start,
+         cut_end_x, cut_end_y, cut_end_z, flag_cut_end, Bz0_start, Bz0_end, stride_this,
+         Q_all=None, B_flag_arr=None, pixel=None):
+    """Q of the pseudo-line pixels from their four traces (demo/Q_RKF45CuPy.ipynb
+    cell [10]).  With ``Q_all`` and the pixel indices (or ``B_flag_arr``) the
+    values are scattered into ``Q_all`` in place (cell [11],
+    ``Q_all[B_flag_arr==1] = Qcut``); otherwise a length-m array is returned."""
+    torch = _torch()
+    d = _Dev()
+    f32, i32 = np.float32, np.int32
+    sx, sy, sz = (d(a, f32).reshape(-1) for a in (cut_start_x, cut_start_y, cut_start_z))
+    ex, ey, ez = (d(a, f32).reshape(-1) for a in (cut_end_x, cut_end_y, cut_end_z))
+    fs, fe = d(flag_cut_start, i32).reshape(-1), d(flag_cut_end, i32).reshape(-1)
+    bzs, bze = d(Bz0_start, f32).reshape(-1), d(Bz0_end, f32).reshape(-1)
+    m = bzs.numel()
+    if Q_all is None:
+        target = torch.empty(m, dtype=torch.float32, device="cuda")
+        pix = torch.arange(m, dtype=torch.int64, device="cuda")
+    else:
+        if not (isinstance(Q_all, torch.Tensor) and Q_all.is_cuda and Q_all.is_contiguous()):
+            raise ValueError("Q_all must be a contiguous CUDA float32 tensor")
+        target = Q_all
+        if pixel is None:
+            bf = d(B_flag_arr, i32).reshape(-1)
+            pix = torch.nonzero(bf == 1).reshape(-1)
+        else:
+            pix = pixel.to(device="cuda", dtype=torch.int64).contiguous()
+        if pix.numel() != m:
+            raise ValueError(f"{pix.numel()} flagged pixels but {m} pseudo-line groups")
+    _lib.check(_lib.lib().fqsl_qcut_scatter(
+        m, sx.data_ptr(), sy.data_ptr(), sz.data_ptr(), fs.data_ptr(), ex.data_ptr(), ey.data_ptr(),
+        ez.data_ptr(), fe.data_ptr(), bzs.data_ptr(), bze.data_ptr(), ctypes.c_float(float(stride_this)),
+        pix.data_ptr(), target.data_ptr(), _stream()))
+    if Q_all is None:
+        return d.out(target)
+    return Q_all

@@ -65,7 +65,7 @@ PROTOTYPES = {
                                ctypes.POINTER(fqsl_scott_out), vp]),
     "fqsl_trace_plain": (i32, [vp, ctypes.POINTER(fqsl_plain_params), u64, vp, vp, vp,
                                ctypes.POINTER(fqsl_plain_out), vp]),
-    "fqsl_qcalc_plane": (i32, [i32, i32] + [vp] * 15 + [ctypes.c_double, vp, vp, vp]),
+    "fqsl_qcalc_plane": (i32, [i32, i32] + [vp] * 13 + [ctypes.c_double, vp, vp, vp]),
     "fqsl_pseudo_seeds": (i32, [u64, vp, vp, vp, vp, vp, vp, vp, ctypes.c_float, u64,
                                 vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64), vp]),
     "fqsl_qcut_scatter": (i32, [u64] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),

@@ -329,7 +329,6 @@ __device__ __attribute__((noinline)) void scott_write(const ScottLine<R>& L, con
         b_clamped<R>(f, L.p0, bt);
         o.btx[i] = bt[0]; o.bty[i] = bt[1]; o.btz[i] = bt[2];
     }
-    if (ds.twist_flag && ds.twist && ds.twist_flag[0]) ds.twist[i] = NAN;
 }
 
 template <class F>
@@ -512,8 +511,7 @@ __global__ void __launch_bounds__(256) k_scott_finish(F f, ScottParams P, unsign
         if (o.qp) o.qp[i] = (float)qp;
         if (o.len) o.len[i] = r.len;
         if (o.evals) o.evals[i] = r.evals;
-        if (ds.twist_flag && ds.twist && ds.twist_flag[0]) ds.twist[i] = NAN;
-    }
+        }
 }
 
 // One thread per line (grid-stride), the reference's mapping
@@ -554,7 +552,6 @@ __device__ __attribute__((noinline)) void plain_write(const PlainLine<R>& L, con
         const float bn = fabsf(b[0] * P.ncross[0] + b[1] * P.ncross[1] + b[2] * P.ncross[2]) * 100.f;
         o.bflag[i] = bn < sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) ? 1 : 0;
     }
-    if (ds.twist_flag && ds.twist && ds.twist_flag[0]) ds.twist[i] = NAN;
 }
 
 template <class F>
@@ -1095,6 +1092,7 @@ extern "C" int fqsl_pseudo_seeds(uint64_t n, const int32_t* b_flag, const float*
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "pseudo-line compaction: %s", hipGetErrorString(e));
     *count_host = cnt;
+    if (capacity == 0) return FQSL_OK;  // count only
     if (cnt > capacity)
         return set_err(FQSL_ERR_INVALID, "%llu pseudo pixels exceed capacity %llu (outputs hold the first %llu)",
                        (unsigned long long)cnt, (unsigned long long)capacity, (unsigned long long)capacity);

@@ -1,0 +1,80 @@
+"""Q maps on a plane of seeds: the two pipelines the reference's notebooks run.
+
+* ``scott_qmap``     -- Scott method, one trace per pixel, q_0 / q_perp
+  (demo/Q_RKF45CuPyScott.ipynb cells [5]-[7]).
+* ``neighbour_qmap`` -- position-only traces, QCalcPlane, pseudo lines for
+  pixels where B is nearly tangent to the plane, Qcut and merge
+  (demo/Q_RKF45CuPy.ipynb cells [6]-[11]).
+
+Seed grids use the notebooks' conventions (``linspace`` x ``meshgrid``, rows
+= the second in-plane axis), see ``synth.plane_seeds_z`` / ``plane_seeds_abc``.
+Everything stays on the GPU; results are torch CUDA tensors.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import FastQSL
+from .field import Field
+from .trace import trace_plain, trace_scott
+
+
+def _grid(seeds, shape):
+    import torch
+    s = seeds if isinstance(seeds, torch.Tensor) else torch.as_tensor(np.asarray(seeds, np.float32))
+    s = s.to(device="cuda", dtype=torch.float32).reshape(-1, 3)
+    if s.shape[0] != int(np.prod(shape)):
+        raise ValueError(f"{s.shape[0]} seeds do not fill a {shape} grid")
+    return s[:, 0].contiguous(), s[:, 1].contiguous(), s[:, 2].contiguous()
+
+
+def scott_qmap(field: Field, seeds, shape, method="rkf45", step=0.25, tol=1e-6,
+               precision="fp32c", schedule="persistent"):
+    """Scott Q on a seed grid of `shape` (rows, cols); returns a dict of
+    (rows, cols) tensors: q_0, q_perp, flag_start, flag_end, b_this_z, ..."""
+    x, y, z = _grid(seeds, shape)
+    r = trace_scott(field, x, y, z, method=method, step=step, tol=tol, precision=precision,
+                    schedule=schedule)
+    return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
+
+
+def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), method="rkf45",
+                   step=0.125, tol_coef=1.0, pseudo_stride=None, pseudo_tol_scale=0.1,
+                   pseudo_normal=None, precision="fp32c", schedule="persistent"):
+    """Neighbour-difference Q (QCalcPlane) with the pseudo-line fix-up for
+    pixels where B is nearly tangent to the plane (B_flag == 1).
+
+    ``stride`` is the pixel spacing in cells (``stride_step`` of the
+    notebooks); ``pseudo_stride`` the offset of the four pseudo seeds (default
+    ``stride``); the pseudo lines are traced with ``tol_coef *
+    pseudo_tol_scale`` and ``pseudo_normal`` (default ``normal``).  Returns a
+    dict with ``q`` = Q_final of shape (rows-2, cols-2), ``q_all`` (rows,
+    cols), the per-pixel traces and the number of pseudo pixels."""
+    import torch
+    x, y, z = _grid(seeds, shape)
+    r = trace_plain(field, x, y, z, ncross=normal, method=method, step=step, tol_coef=tol_coef,
+                    precision=precision, schedule=schedule)
+    g = {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
+    q_inner = FastQSL.QCalcPlane(g["end_x"], g["end_y"], g["end_z"], g["flag_end"],
+                                 g["start_x"], g["start_y"], g["start_z"], g["flag_start"],
+                                 g["b_start_x"], g["b_start_y"], g["b_start_z"],
+                                 g["b_end_x"], g["b_end_y"], g["b_end_z"],
+                                 g["b_this_x"], g["b_this_y"], g["b_this_z"],
+                                 g["b_flag"], stride, normal)
+    ps = stride if pseudo_stride is None else pseudo_stride
+    c = FastQSL.cook_pseudo_lines(g["start_x"], g["start_y"], g["start_z"], g["flag_start"],
+                                  g["b_start_z"], g["b_end_z"], g["b_flag"], ps)
+    q_all = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    q_all[1:-1, 1:-1] = q_inner
+    m = int(c["pixel"].numel())
+    if m:
+        rc = trace_plain(field, c["seed_x"], c["seed_y"], c["seed_z"],
+                         ncross=normal if pseudo_normal is None else pseudo_normal, method=method,
+                         step=step, tol_coef=tol_coef * pseudo_tol_scale, precision=precision,
+                         schedule=schedule,
+                         outputs=("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
+                                  "flag_start", "flag_end"))
+        FastQSL.Qcut(rc["start_x"], rc["start_y"], rc["start_z"], rc["flag_start"],
+                     rc["end_x"], rc["end_y"], rc["end_z"], rc["flag_end"],
+                     c["bz0_start"], c["bz0_end"], ps, Q_all=q_all, pixel=c["pixel"])
+    return dict(q=q_all[1:-1, 1:-1], q_all=q_all, q_inner=q_inner, traces=g, n_pseudo=m)

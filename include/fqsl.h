@@ -148,7 +148,8 @@ int fqsl_qcalc_plane(int ny, int nx, const float *x_end, const float *y_end, con
  * pixel order) write four seeds at its start footpoint +/- stride along the
  * two in-face axes, plus Bz at both footpoints and the pixel index.  The
  * output arrays hold up to `capacity` pixels (4*capacity seeds); the number of
- * selected pixels is written to *count_host after a stream synchronisation. */
+ * selected pixels is written to *count_host after a stream synchronisation.
+ * capacity == 0 only counts (output pointers may then be NULL). */
 int fqsl_pseudo_seeds(uint64_t n, const int32_t *b_flag, const float *x_start, const float *y_start,
                       const float *z_start, const int32_t *flag_start, const float *bz_start,
                       const float *bz_end, float stride, uint64_t capacity, float *seed_x,
@@ -167,7 +168,8 @@ int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, co
 /* Same parameters, order and meaning as the CUDA kernels the fastqslpy host
  * code launches (all device pointers; scalars as one-element device arrays),
  * plus the stream.  The field arrays are read in place (SoA float layout).
- * A set curB_flag (twist) is not implemented: twist[] is filled with NaN. */
+ * Twist (curB_flag set) is not implemented: callers must pass curB_flag false;
+ * twist[] and the curB arrays are never accessed. */
 
 /* TraceAllBlineScott (TraceBlineScott.cu:521-528).  tol_coef scales the
  * per-step tolerance 1e-3 * tol_coef.  precision / schedule select the path;

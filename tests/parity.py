@@ -19,15 +19,22 @@ def class_match(fs_a, fe_a, fs_b, fe_b):
     return (np.asarray(fs_a) == np.asarray(fs_b)) & (np.asarray(fe_a) == np.asarray(fe_b))
 
 
-def conditioned(ref, ref_perturbed, thresh=1e-4):
-    """Lines whose oracle Q and classes are stable under a perturbation far
-    below float32 resolution (seed moved by 1e-6 cells, or tolerance scaled
-    by 1 + 1e-12).  Lines on a separatrix through a null (Q -> infinity) or
-    whose adaptive step sequence forks on round-off fail this."""
-    d = log_q_diff(ref["q0"], ref_perturbed["q0"])
-    c = class_match(ref["flag_start"], ref["flag_end"], ref_perturbed["flag_start"],
-                    ref_perturbed["flag_end"])
-    return c & (d < thresh)
+def conditioned(ref, *perturbed, thresh=1e-4, key="q0"):
+    """Lines whose oracle result is stable under perturbations far below
+    float32 resolution (seed moved by +-1e-6 cells, or tolerance scaled by
+    1 + 1e-12): same classes and Q (or, with key="end", footpoints within
+    1e-3 cells) in every perturbed run.  Lines on a separatrix through a null
+    (Q -> infinity) or whose adaptive step sequence forks on round-off fail
+    this."""
+    ok = np.ones(np.asarray(ref["flag_start"]).shape, bool)
+    for p in perturbed:
+        ok &= class_match(ref["flag_start"], ref["flag_end"], p["flag_start"], p["flag_end"])
+        if key == "q0":
+            ok &= log_q_diff(ref["q0"], p["q0"]) < thresh
+        else:
+            ok &= np.abs(np.asarray(ref["end"]) - p["end"]).max(axis=1) < 1e-3
+            ok &= np.abs(np.asarray(ref["start"]) - p["start"]).max(axis=1) < 1e-3
+    return ok
 
 
 def summary(d, cls, mask=None):

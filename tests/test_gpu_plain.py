@@ -1,0 +1,277 @@
+"""GPU parity of the neighbour-difference path and of the reference-shaped
+drop-in entry points.
+
+* Position-only tracer (TraceAllBline) against the FP64 oracle: RK4 lines are
+  deterministic, so classes and footpoints must agree on well-conditioned
+  lines; RKF45 is held to the oracle's own round-off sensitivity.
+* QCalcPlane, CookPseudoLine and Qcut on the GPU against the NumPy float32
+  restatement (oracle/qplane.py) on identical inputs: bit for bit.
+* The full pipeline on the linear X-point field against the closed form.
+* ``fastqsl_amd.kernels`` called with the reference's argument tuples
+  (torch and NumPy arrays) against the direct API.
+"""
+import numpy as np
+import pytest
+
+from fastqsl_amd import synth
+from oracle import qplane
+from tests import parity
+
+pytestmark = pytest.mark.gpu
+
+TRACE_KEYS = ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z", "flag_start",
+              "flag_end", "b_this_x", "b_this_y", "b_this_z", "b_flag", "b_start_x", "b_start_y",
+              "b_start_z", "b_end_x", "b_end_y", "b_end_z", "line_len", "evals")
+
+
+@pytest.fixture(scope="module")
+def fq():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import fastqsl_amd
+    fastqsl_amd.lib()
+    return fastqsl_amd
+
+
+def _np(r):
+    import torch
+    torch.cuda.synchronize()
+    return {k: r[k].cpu().numpy() for k in TRACE_KEYS if k in r}
+
+
+def _xyz(r, pre):
+    return np.column_stack([r[pre + "_x"], r[pre + "_y"], r[pre + "_z"]])
+
+
+@pytest.fixture(scope="module")
+def quad():
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu")
+    seeds, shape = synth.plane_seeds_z(96, 96, (2, 61), (2, 61), z=1e-4)
+    return field, seeds, shape
+
+
+def test_plain_rk4_matches_oracle(fq, oracle, quad):
+    field, seeds, shape = quad
+    ref = oracle.trace_plain(field, seeds, method="rk4", s0=0.25)
+    refp = [oracle.trace_plain(field, seeds.astype(np.float64) + sgn * np.array([1e-6, -1e-6, 0]),
+                               method="rk4", s0=0.25) for sgn in (1.0, -1.0)]
+    cond = parity.conditioned(ref, *refp, key="end")
+    print("well-conditioned:", cond.mean())
+    assert cond.mean() > 0.99
+    with fq.Field(*field) as f:
+        for prec, tol in (("fp64", 1e-4), ("fp32c", 2e-3)):
+            for sched in ("persistent", "per_line"):
+                r = _np(fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rk4", step=0.25,
+                                       precision=prec, schedule=sched))
+                cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+                print(prec, sched, "class match", cls.mean(), "conditioned", cls[cond].mean())
+                assert cls[cond].mean() >= (1.0 if prec == "fp64" else 0.9999)
+                m = cond & cls
+                de = np.abs(_xyz(r, "end") - ref["end"]).max(axis=1)[m]
+                ds = np.abs(_xyz(r, "start") - ref["start"]).max(axis=1)[m]
+                print("  endpoint error max %.2e / %.2e" % (de.max(), ds.max()))
+                assert de.max() < tol and ds.max() < tol
+                np.testing.assert_allclose(_xyz(r, "b_this"), ref["b_this"], rtol=2e-5, atol=1e-6)
+                np.testing.assert_allclose(_xyz(r, "b_end")[m], ref["b_end"][m], rtol=1e-2, atol=1e-3)
+                assert np.mean(r["b_flag"] == ref["b_flag"]) > 0.9995
+                if prec == "fp64":
+                    assert np.array_equal(r["evals"][m].astype(np.int64), ref["evals"][m])
+
+
+def test_plain_rkf45_as_accurate_as_the_oracle(fq, oracle):
+    """RKF45 with the reference's tolerance rule (1e-3 |b.n|^1.5 cells, up to
+    1e-3): footpoints and classes are only as good as that tolerance, so the
+    GPU is held to the oracle's own agreement with a converged trace
+    (tol_coef 1e-4)."""
+    field = synth.multipole_256(seed=0, device="cuda")
+    rng = np.random.default_rng(3)
+    seeds = np.column_stack([rng.uniform(0, 255, 4096), rng.uniform(0, 255, 4096),
+                             np.full(4096, 1e-4)]).astype(np.float32)
+    ref = oracle.trace_plain(field, seeds, method="rkf45", s0=0.125, tol_coef=1.0)
+    truth = oracle.trace_plain(field, seeds, method="rkf45", s0=0.125, tol_coef=1e-4)
+    c_orc = parity.class_match(ref["flag_start"], ref["flag_end"], truth["flag_start"], truth["flag_end"])
+    e_orc = np.abs(ref["end"] - truth["end"]).max(axis=1)
+    print("oracle vs converged: class %.5f end err median %.2e p99 %.2e"
+          % (c_orc.mean(), np.median(e_orc[c_orc]), np.quantile(e_orc[c_orc], 0.99)))
+    with fq.Field(*field) as f:
+        for prec in ("fp64", "fp32c"):
+            r = _np(fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rkf45", step=0.125,
+                                   tol_coef=1.0, precision=prec))
+            c = parity.class_match(r["flag_start"], r["flag_end"], truth["flag_start"], truth["flag_end"])
+            e = np.abs(_xyz(r, "end") - truth["end"]).max(axis=1)
+            m = c & c_orc
+            print(prec, "vs converged: class %.5f end err median %.2e p99 %.2e"
+                  % (c.mean(), np.median(e[m]), np.quantile(e[m], 0.99)))
+            assert c.mean() >= c_orc.mean() - 2e-3
+            assert np.median(e[m]) <= 1.5 * np.median(e_orc[m]) + 1e-5
+            assert np.quantile(e[m], 0.99) <= 1.5 * np.quantile(e_orc[m], 0.99) + 1e-5
+
+
+@pytest.fixture(scope="module")
+def plane_traces(fq):
+    """FP32 traces on a 160 x 128 grid over the random multipole's z = 0 plane."""
+    field = synth.multipole((96, 96, 96), seed=5, device="cuda")
+    seeds, shape = synth.plane_seeds_z(160, 128, (10, 85), (10, 73), z=1e-4)
+    with fq.Field(*field) as f:
+        r = _np(fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rkf45", step=0.125))
+    return field, seeds, shape, {k: v.reshape(shape) for k, v in r.items()}
+
+
+def test_qcalc_plane_bit_exact(fq, plane_traces):
+    import torch
+    field, seeds, shape, g = plane_traces
+    stride = 75.0 / 159
+    for cross in ((0.0, 0.0, 1.0), (0.3, -0.2, 0.93)):
+        args = (g["end_x"], g["end_y"], g["end_z"], g["flag_end"], g["start_x"], g["start_y"],
+                g["start_z"], g["flag_start"], g["b_start_x"], g["b_start_y"], g["b_start_z"],
+                g["b_end_x"], g["b_end_y"], g["b_end_z"], g["b_this_x"], g["b_this_y"], g["b_this_z"],
+                g["b_flag"], stride)
+        ref = qplane.qcalc_plane(*args, cross_dir=cross)
+        got = fq.FastQSL.QCalcPlane(*args, CrossBC_dir=np.asarray(cross, np.float32))
+        assert got.shape == ref.shape == (shape[0] - 2, shape[1] - 2)
+        np.testing.assert_array_equal(got, ref)
+        assert np.isnan(ref).any() and np.isfinite(ref).mean() > 0.5
+        # CUDA-tensor inputs give a CUDA tensor back
+        gt = fq.FastQSL.QCalcPlane(*[torch.as_tensor(a).cuda() if isinstance(a, np.ndarray) else a
+                                     for a in args], CrossBC_dir=cross)
+        assert gt.is_cuda
+        np.testing.assert_array_equal(gt.cpu().numpy(), ref)
+
+
+def test_pseudo_lines_and_qcut_bit_exact(fq, plane_traces):
+    field, seeds, shape, g = plane_traces
+    stride = np.float32(0.25)
+    flagged = int((g["b_flag"] == 1).sum())
+    print("pseudo pixels:", flagged)
+    assert flagged > 0
+    ref_seeds, ref_bzs, ref_bze = qplane.pseudo_seeds(g["start_x"], g["start_y"], g["start_z"],
+                                                      g["flag_start"], g["b_start_z"], g["b_end_z"],
+                                                      g["b_flag"], stride)
+    tup = fq.FastQSL.CookPseudoLine(g["end_x"], g["end_y"], g["end_z"], g["flag_end"],
+                                    g["start_x"], g["start_y"], g["start_z"], g["flag_start"],
+                                    g["b_start_x"], g["b_start_y"], g["b_start_z"],
+                                    g["b_end_x"], g["b_end_y"], g["b_end_z"],
+                                    g["b_this_z"], g["b_flag"], stride)
+    assert len(tup) == 25
+    np.testing.assert_array_equal(np.column_stack(tup[0:3]), ref_seeds)
+    np.testing.assert_array_equal(tup[23], ref_bzs)
+    np.testing.assert_array_equal(tup[24], ref_bze)
+    assert int(tup[21][0]) == 4 * flagged
+    # trace the pseudo seeds, then Qcut + merge on both sides
+    with fq.Field(*field) as f:
+        r = _np(fq.trace_plain(f, tup[0], tup[1], tup[2], method="rkf45", step=0.125, tol_coef=0.1))
+    st, en = _xyz(r, "start"), _xyz(r, "end")
+    ref_q = qplane.qcut(st, r["flag_start"], en, r["flag_end"], ref_bzs, ref_bze, stride)
+    got_q = fq.FastQSL.Qcut(r["start_x"], r["start_y"], r["start_z"], r["flag_start"], r["end_x"],
+                            r["end_y"], r["end_z"], r["flag_end"], tup[23], tup[24], stride)
+    np.testing.assert_array_equal(got_q, ref_q)
+    import torch
+    q_inner = qplane.qcalc_plane(g["end_x"], g["end_y"], g["end_z"], g["flag_end"], g["start_x"],
+                                 g["start_y"], g["start_z"], g["flag_start"], g["b_start_x"],
+                                 g["b_start_y"], g["b_start_z"], g["b_end_x"], g["b_end_y"],
+                                 g["b_end_z"], g["b_this_x"], g["b_this_y"], g["b_this_z"],
+                                 g["b_flag"], 0.25)
+    ref_final = qplane.merge(q_inner, ref_q, g["b_flag"])
+    q_all = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    q_all[1:-1, 1:-1] = torch.as_tensor(q_inner).cuda()
+    fq.FastQSL.Qcut(r["start_x"], r["start_y"], r["start_z"], r["flag_start"], r["end_x"], r["end_y"],
+                    r["end_z"], r["flag_end"], tup[23], tup[24], stride, Q_all=q_all, B_flag_arr=g["b_flag"])
+    np.testing.assert_array_equal(q_all[1:-1, 1:-1].cpu().numpy(), ref_final)
+
+
+def test_neighbour_pipeline_hyperbolic(fq):
+    shape = (41, 41, 33)
+    k = 0.03
+    field = synth.hyperbolic(shape, k=k, b0=1.0)
+    seeds, sh = synth.plane_seeds_z(24, 24, (17, 23), (17, 23), z=1e-4)
+    with fq.Field(*field) as f:
+        for method in ("rk4", "rkf45"):
+            out = fq.qmap.neighbour_qmap(f, seeds, sh, stride=6.0 / 23, method=method, step=0.125,
+                                         tol_coef=1e-4)
+            q = out["q"].cpu().numpy()
+            assert out["n_pseudo"] == 0
+            np.testing.assert_allclose(q, 2 * np.cosh(2 * k * shape[2]), rtol=3e-3)
+
+
+def _scott_args(field32, seeds, torch_arrays):
+    import torch
+    bx, by, bz = (np.asfortranarray(b.astype(np.float32)) for b in field32)
+    n = seeds.shape[0]
+    f32 = lambda: np.zeros(n, np.float32)
+    a = dict(Bx=bx, By=by, Bz=bz, BshapeN=np.array(bx.shape, np.int32), d1=np.zeros((1, 1), np.float32),
+             d2=np.zeros((1, 1), np.float32), d3=np.zeros((1, 1), np.float32),
+             d4=np.zeros((1, 1), np.float32), flag_twist=np.array([False]),
+             x=np.ascontiguousarray(seeds[:, 0]), y=np.ascontiguousarray(seeds[:, 1]),
+             z=np.ascontiguousarray(seeds[:, 2]), norm=np.array([0, 0, 1], np.float32),
+             sx=f32(), sy=f32(), sz=f32(), fs=np.zeros(n, np.int32), ex=f32(), ey=f32(), ez=f32(),
+             fe=np.zeros(n, np.int32), btx=f32(), bty=f32(), btz=f32(), q0=f32(), qp=f32(),
+             s_len=np.array([0.25], np.float32), N=np.array([n], np.uint64),
+             line_len=np.zeros(n, np.float64), tol_coef=np.array([1e-3], np.float32))
+    if torch_arrays:
+        a = {k: (torch.as_tensor(v.ravel(order="K").view(np.uint8) if v.dtype == np.bool_ else
+                                 v.ravel(order="K").astype(np.int64) if v.dtype == np.uint64 else
+                                 v.ravel(order="K")).cuda()) for k, v in a.items()}
+        a["flag_twist"] = a["flag_twist"].to(torch.bool)
+    return a
+
+
+def test_dropin_scott_kernel_matches_direct_api(fq):
+    import torch
+    field = synth.multipole((64, 48, 40), seed=2, device="cpu")
+    rng = np.random.default_rng(1)
+    seeds = np.column_stack([rng.uniform(0, 63, 3000), rng.uniform(0, 47, 3000),
+                             np.r_[np.zeros(2000), rng.uniform(0, 39, 1000)]]).astype(np.float32)
+    with fq.Field(*field) as f:
+        # the drop-in's tolerance is 1e-3 * tol_coef with tol_coef a float32
+        tol = 1e-3 * float(np.float32(1e-3))
+        ref = fq.trace_scott(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rkf45", step=0.25, tol=tol)
+        torch.cuda.synchronize()
+        ref = {k: v.cpu().numpy() for k, v in ref.items() if k != "_seeds"}
+    kern = fq.kernels.compileTraceBlineScott()
+    for use_torch in (False, True):
+        for sched in ("persistent", "per_line"):
+            kern.schedule = sched
+            a = _scott_args(field, seeds, use_torch)
+            kern((16, 1), (256, 1, 1), tuple(a.values()))
+            torch.cuda.synchronize()
+            got = {k: (v.cpu().numpy() if isinstance(v, torch.Tensor) else v) for k, v in a.items()}
+            for mine, theirs in (("sx", "start_x"), ("ey", "end_y"), ("fs", "flag_start"), ("fe", "flag_end"),
+                                 ("q0", "q_0"), ("qp", "q_perp"), ("btz", "b_this_z"), ("line_len", "line_len")):
+                assert np.array_equal(got[mine], ref[theirs], equal_nan=True), (use_torch, sched, mine)
+    a = _scott_args(field, seeds[:10], False)
+    a["flag_twist"][0] = True
+    with pytest.raises(NotImplementedError):
+        kern((1, 1), (64, 1, 1), tuple(a.values()))
+
+
+def test_dropin_plain_kernels_match_direct_api(fq):
+    import torch
+    field = synth.multipole((64, 48, 40), seed=3, device="cpu")
+    seeds, shape = synth.plane_seeds_z(50, 40, (2, 60), (2, 45), z=1e-4)
+    n = seeds.shape[0]
+    bx, by, bz = (np.asfortranarray(b.astype(np.float32)) for b in field)
+    for which, ctor in (("rkf45", fq.kernels.compileTraceBlineAdaptive), ("rk4", fq.kernels.compileTraceBlineRK4)):
+        with fq.Field(*field) as f:
+            ref = fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method=which, step=0.125,
+                                 tol_coef=0.5, precision="fp32c")
+            torch.cuda.synchronize()
+            ref = {k: v.cpu().numpy() for k, v in ref.items() if k != "_seeds"}
+        f32 = lambda: np.zeros(n, np.float32)
+        outs = dict(sx=f32(), sy=f32(), sz=f32(), fs=np.zeros(n, np.int32), ex=f32(), ey=f32(), ez=f32(),
+                    fe=np.zeros(n, np.int32), btx=f32(), bty=f32(), btz=f32(), bf=np.zeros(n, np.int32),
+                    bsx=f32(), bsy=f32(), bsz=f32(), bex=f32(), bey=f32(), bez=f32())
+        dummy = np.zeros((1, 1), np.float32)
+        args = [bx, by, bz, np.array(bx.shape, np.int32), dummy, dummy, dummy, dummy, np.array([False]),
+                np.ascontiguousarray(seeds[:, 0]), np.ascontiguousarray(seeds[:, 1]),
+                np.ascontiguousarray(seeds[:, 2]), np.array([0, 0, 1], np.float32)]
+        args += list(outs.values())
+        line_len = np.zeros(n, np.float64)
+        args += [np.array([0.125], np.float32), np.array([n], np.uint64), line_len]
+        if which == "rkf45":
+            args.append(np.array([0.5], np.float32))
+        ctor()((28, 1), (128, 1, 1), tuple(args))
+        for mine, theirs in (("sx", "start_x"), ("ez", "end_z"), ("fs", "flag_start"), ("fe", "flag_end"),
+                             ("bf", "b_flag"), ("bsz", "b_start_z"), ("bez", "b_end_z"), ("btx", "b_this_x")):
+            assert np.array_equal(outs[mine], ref[theirs], equal_nan=True), (which, mine)
+        assert np.array_equal(line_len, ref["line_len"]), which

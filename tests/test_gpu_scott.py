@@ -48,9 +48,9 @@ def c1(oracle):
     X, Y = np.meshgrid(g, g)
     seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]).astype(np.float32)
     ref = oracle.trace_scott(field, seeds, method="rk4", s0=0.25)
-    refp = oracle.trace_scott(field, seeds.astype(np.float64) + np.array([1e-6, -1e-6, 0.0]),
-                              method="rk4", s0=0.25)
-    cond = parity.conditioned(ref, refp)
+    refp = [oracle.trace_scott(field, seeds.astype(np.float64) + sgn * np.array([1e-6, -1e-6, 0.0]),
+                               method="rk4", s0=0.25) for sgn in (1.0, -1.0)]
+    cond = parity.conditioned(ref, *refp)
     return field, seeds, ref, cond
 
 
