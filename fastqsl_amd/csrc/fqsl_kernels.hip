@@ -203,7 +203,7 @@ extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) 
 }
 
 // ------------------------------------------------------------- tuning knobs
-static int g_tune[FQSL_TUNE_COUNT] = {0, 100, 1, 16};
+static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
@@ -306,16 +306,17 @@ struct PlainOut {
 };
 
 template <typename R, class F>
-__device__ __attribute__((noinline)) void scott_write(const ScottLine<R>& L, const ScottEnd<R>& e, int status,
-                                                      const F& f, unsigned long long i, const ScottOut& o,
-                                                      const DevScalars& ds) {
+__device__ __forceinline__ void scott_write(const ScottLine<R>& L, const ScottEnd<R>& e, int status,
+                                            const F& f, unsigned long long i, const ScottOut& o,
+                                            const DevScalars& ds) {
     double q0, qp, nb0 = 0.0;
     if (status == kFinished) {
         R u0[3], v0[3], b0z;
         seed_frame<R>(f, L.p0, u0, v0, &b0z, &nb0);
         nb0 = (double)(R)nb0;  // |B0| as the persistent path's seed frame stores it
     }
-    scott_q<R>(L.st, e, status, nb0, f, &q0, &qp);
+    const ScottEnd<R> st = L.st, en = e;  // copies: scott_q is out of line
+    scott_q<R>(st, en, status, nb0, f, &q0, &qp);
     if (o.sx) { o.sx[i] = (float)L.st.x[0]; o.sy[i] = (float)L.st.x[1]; o.sz[i] = (float)L.st.x[2]; }
     if (o.ex) { o.ex[i] = (float)e.x[0]; o.ey[i] = (float)e.x[1]; o.ez[i] = (float)e.x[2]; }
     if (o.fs) o.fs[i] = L.st.flag;
@@ -517,8 +518,8 @@ __global__ void __launch_bounds__(256) k_scott_finish(F f, ScottParams P, unsign
 // One thread per line (grid-stride), the reference's mapping
 // (TraceBlineScott.cu:530-553).
 template <typename R, class F>
-__global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seeds s, unsigned long long n,
-                                                        ScottOut out, DevScalars ds) {
+__device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                    ScottOut out, DevScalars ds) {
     apply_scalars(f, P, n, ds);
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -530,9 +531,22 @@ __global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seed
     }
 }
 
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seeds s, unsigned long long n,
+                                                        ScottOut out, DevScalars ds) {
+    scott_per_line_body<R, F>(f, P, s, n, out, ds);
+}
+
+// FP32 variant compiled for two waves per SIMD (like the persistent kernel).
+template <typename R, class F>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_scott_per_line_w2(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds) {
+    scott_per_line_body<R, F>(f, P, s, n, out, ds);
+}
+
 // ------------------------------------------------------------ plain kernels
 template <typename R, class F>
-__device__ __attribute__((noinline)) void plain_write(const PlainLine<R>& L, const F& f, const PlainParams& P,
+__device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, const PlainParams& P,
                                             unsigned long long i, const PlainOut& o, const DevScalars& ds) {
     const float xs[3] = {(float)L.xs[0], (float)L.xs[1], (float)L.xs[2]};
     const float xe[3] = {(float)L.xe[0], (float)L.xe[1], (float)L.xe[2]};
@@ -700,7 +714,9 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
         if (rc) return rc;
-        k_scott_per_line<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds);
+        auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F>
+                                                                        : k_scott_per_line<R, F>;
+        kern<<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }

@@ -538,12 +538,21 @@ FQSL_HD void seed_frame(const F& f, const float* p0, R u0[3], R v0[3], R* b0z, d
 }
 
 // Record a footpoint for the active side (start when dir < 0, else `e`).
+// (Two explicit branches: selecting a reference to one of the two records
+// would force both, and the line state, out of registers into scratch.  For
+// the same reason no reference into a line's state, or to the field view,
+// is ever passed to an out-of-line (FQSL_COLD) function.)
 template <typename R>
 FQSL_HD void scott_record(ScottLine<R>& L, ScottEnd<R>& e, const double* x, const R* u, const R* v, int flag) {
-    ScottEnd<R>& S = L.dir < 0 ? L.st : e;
+    if (L.dir < 0) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { S.x[a] = (R)x[a]; S.u[a] = u[a]; S.v[a] = v[a]; }
-    S.flag = flag;
+        for (int a = 0; a < 3; ++a) { L.st.x[a] = (R)x[a]; L.st.u[a] = u[a]; L.st.v[a] = v[a]; }
+        L.st.flag = flag;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { e.x[a] = (R)x[a]; e.u[a] = u[a]; e.v[a] = v[a]; }
+        e.flag = flag;
+    }
 }
 
 template <typename R>
@@ -561,7 +570,7 @@ FQSL_HD int scott_len_lim(const F& f) {
 // the shared length/step budget is spent record their footpoint at once.
 // Returns kTracing, or kFinished once the forward side is recorded.
 template <typename R, class F>
-FQSL_COLD int scott_begin_dir(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
+FQSL_HD int scott_begin_dir(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
     for (;;) {
         R u0[3], v0[3], b0z;
         if (L.fr) {
@@ -589,7 +598,7 @@ FQSL_COLD int scott_begin_dir(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
 }
 
 template <typename R, class F>
-FQSL_COLD int scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float px, float py, float pz,
+FQSL_HD int scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float px, float py, float pz,
                          ScottEnd<R>& e, const R* frame = nullptr) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
     L.fr = frame;
@@ -772,7 +781,7 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
 // Q assembly (TraceBlineScott.cu:475-508) in FP64 from the start footpoint
 // st, the end footpoint e and |B0| at the seed.
 template <typename R, class F>
-FQSL_COLD void scott_q(const ScottEnd<R>& st, const ScottEnd<R>& e, int status, double nb0, const F& f,
+FQSL_COLD void scott_q(const ScottEnd<R>& st, const ScottEnd<R>& e, int status, double nb0, const F f,
                        double* q0, double* qperp) {
     if (status == kAborted) { *q0 = 0.0; *qperp = 0.0; return; }
     if (st.flag < 1 || st.flag > 6 || e.flag < 1 || e.flag > 6) {
@@ -845,14 +854,18 @@ FQSL_HD bool plain_can_step(const PlainLine<R>& L) {
 
 template <typename R>
 FQSL_HD void plain_record(PlainLine<R>& L, const double* x, int flag) {
-    double* d = L.dir > 0 ? L.xe : L.xs;
-    d[0] = x[0]; d[1] = x[1]; d[2] = x[2];
-    if (L.dir > 0) L.flag_e = flag; else L.flag_s = flag;
+    if (L.dir > 0) {
+        L.xe[0] = x[0]; L.xe[1] = x[1]; L.xe[2] = x[2];
+        L.flag_e = flag;
+    } else {
+        L.xs[0] = x[0]; L.xs[1] = x[1]; L.xs[2] = x[2];
+        L.flag_s = flag;
+    }
 }
 
 // Begin the direction L.dir from the seed with a fresh step and budget.
 template <typename R, class F>
-FQSL_COLD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P) {
+FQSL_HD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P) {
     for (;;) {
         L.x.set_f(L.p0);
         L.s = P.s0;
@@ -869,7 +882,7 @@ FQSL_COLD void plain_begin_dir(PlainLine<R>& L, const F& f, const PlainParams& P
 }
 
 template <typename R, class F>
-FQSL_COLD void plain_init(PlainLine<R>& L, const F& f, const PlainParams& P, float px, float py, float pz) {
+FQSL_HD void plain_init(PlainLine<R>& L, const F& f, const PlainParams& P, float px, float py, float pz) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
     const bool rkf = P.method == kRKF45;
     L.len_lim = (rkf ? kPlainRkf45LenRatio : kPlainRk4LenRatio) * (f.n[0] + f.n[1] + f.n[2]);
@@ -899,7 +912,7 @@ FQSL_HD void plain_finish_dir(PlainLine<R>& L, const F& f, const PlainParams& P)
 
 // RK4_boundary (TraceBline.cu:253-274): integrate dx/dx_dim = b / b_dim over s.
 template <typename R, class F>
-FQSL_COLD void rk4_boundary(const F& f, const double* p, double s, int dim, double out[3]) {
+FQSL_COLD void rk4_boundary(const F f, const double* p, double s, int dim, double out[3]) {
     double k[4][3], t[3];
     const double cf[4] = {0.0, 0.5, 0.5, 1.0};
 #pragma unroll

@@ -199,11 +199,11 @@ int fqsl_TraceAllBline(int method, float *Bx, float *By, float *Bz, int *BshapeN
 /* ---------------------------------------------------------- tuning knobs */
 /* Process-wide launch parameters picked by the autotuner (fastqsl_amd.tune).
  * They change speed only, never results. */
-#define FQSL_TUNE_SCOTT_WAVES 0 /* waves/SIMD the FP32 persistent Scott kernel is compiled for:
-                                 0 = compiler's choice (default), 2 or 3 */
+#define FQSL_TUNE_SCOTT_WAVES 0 /* waves/SIMD the FP32 Scott kernels are compiled for:
+                                 0 = compiler's choice, 2 (default) or 3 */
 #define FQSL_TUNE_GRID_PCT 1    /* persistent grid size in % of one full occupancy wave (default 100) */
 #define FQSL_TUNE_QUEUES 2      /* seed queues of the persistent Scott kernel: 1 (default) or 8 (one per XCD) */
-#define FQSL_TUNE_REFILL_MIN 3  /* idle lanes a wavefront gathers before it refills them (1..64, default 16) */
+#define FQSL_TUNE_REFILL_MIN 3  /* idle lanes a wavefront gathers before it refills them (1..64, default 8) */
 #define FQSL_TUNE_COUNT 4
 int fqsl_tune_set(int knob, int value);
 int fqsl_tune_get(int knob, int *value);
