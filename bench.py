@@ -48,6 +48,23 @@ def parse():
     return ap.parse_args()
 
 
+PROFILE = os.path.join(ROOT, "profiles", "scott_rkf45_multipole256_1024sq.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the stepping kernel as measured with rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and recorded in the profile
+    summary for this workload; None if absent."""
+    try:
+        with open(PROFILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(field_np, seeds, sample, log):
     from oracle import oracle as O
     rng = np.random.default_rng(1)
@@ -165,7 +182,10 @@ def main():
                    "lines_per_step": nlines, "precision": args.precision,
                    "schedule": args.schedule, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": (measured_traffic() if args.precision == "fp32c" and n == 1024 else None),
+                     "traffic_source": "profiles/scott_rkf45_multipole256_1024sq.md (rocprofv3 "
+                                       "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
                      "kernel_ms": kms, "evals_per_launch": evals, "bytes_per_eval": bpe},
         "cpu_baseline": cpu,
         "classes": {str(int(c)): int(k) for c, k in zip(classes, counts)},
