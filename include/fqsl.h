@@ -24,6 +24,9 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -197,7 +200,7 @@ int fqsl_TraceAllBline(int method, float *Bx, float *By, float *Bz, int *BshapeN
                        int precision, int schedule, void *stream);
 
 /* ---------------------------------------------------------- tuning knobs */
-/* Process-wide launch parameters picked by the autotuner (fastqsl_amd.tune).
+/* Process-wide launch parameters (fastqsl_amd._lib.tune_set / tune_get).
  * They change speed only, never results. */
 #define FQSL_TUNE_SCOTT_WAVES 0 /* waves/SIMD the FP32 Scott kernels are compiled for:
                                  0 = compiler's choice, 2 (default) or 3 */

@@ -1,0 +1,136 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py).
+
+CPU: the oracle, rebuilt from source, still produces them (drift pin).
+GPU: both device precisions against them, over the lines the fixture marks
+stable (same classes and Q / footpoints under +-1e-6-cell seed moves).
+"""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import parity
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TRACE_CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz"))
+                     if not os.path.basename(p).startswith("qcalc"))
+
+
+def load(name):
+    with np.load(os.path.join(HERE, name + ".npz")) as z:
+        d = {k: z[k] for k in z.files}
+    d["meta"] = json.loads(str(d["meta"]))
+    return d
+
+
+def field_of(meta):
+    from tests.golden import make_golden
+    return make_golden.field_of(meta["generator"], meta["generator_kwargs"])
+
+
+def test_fixture_set_is_complete():
+    assert {"scott_quadrupole64_rk4", "scott_multipole64_rkf45", "scott_multipole64_converged",
+            "scott_rope_rk4_volume", "plain_quadrupole64_rk4"} <= set(TRACE_CASES)
+    for name in TRACE_CASES:
+        d = load(name)
+        assert d["seeds"].dtype == np.float32 and d["stable"].mean() > 0.3, name
+
+
+@pytest.mark.parametrize("name", TRACE_CASES)
+def test_oracle_reproduces_fixture(oracle, name):
+    d = load(name)
+    m = d["meta"]
+    f = field_of(m)
+    trace = oracle.trace_scott if m["tracer"] == "scott" else oracle.trace_plain
+    r = trace(f, d["seeds"], **m["tracer_kwargs"])
+    st = d["stable"]
+    cls = parity.class_match(r["flag_start"], r["flag_end"], d["flag_start"], d["flag_end"])
+    if m["tracer_kwargs"]["method"] == "rk4":
+        assert cls[st].all()
+        assert np.abs(r["end"] - d["end"])[st].max() < 1e-9
+        if m["tracer"] == "scott":
+            assert parity.log_q_diff(r["q0"], d["q0"])[st].max() < 1e-9
+    else:
+        # adaptive steps fork on round-off: classes and Q to the oracle's own spread
+        assert cls[st].mean() > 0.99
+        assert np.median(parity.log_q_diff(r["q0"], d["q0"])[st]) < 1e-6
+
+
+def test_qcalc_fixture(oracle):
+    from oracle import qplane
+    with np.load(os.path.join(HERE, "qcalc_plane_hyperbolic.npz")) as z:
+        d = {k: z[k] for k in z.files}
+    names = ("x_end", "y_end", "z_end", "flag_end", "x_start", "y_start", "z_start", "flag_start",
+             "bx_in", "by_in", "bz_in", "bx_out", "by_out", "bz_out", "bx_0", "by_0", "bz_0", "b_flag")
+    q = qplane.qcalc_plane(*[d[n] for n in names], float(d["stride"]))
+    np.testing.assert_array_equal(q, d["q"])
+    np.testing.assert_allclose(q, 2 * np.cosh(2 * 0.03 * 33), rtol=3e-3)
+
+
+# ------------------------------------------------------------------ GPU
+
+@pytest.fixture(scope="module")
+def fq():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import fastqsl_amd
+    fastqsl_amd.lib()
+    return fastqsl_amd
+
+
+def _np(r):
+    import torch
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy() for k, v in r.items() if k != "_seeds"}
+    for a in ("start", "end"):
+        out[a] = np.column_stack([out[a + "_x"], out[a + "_y"], out[a + "_z"]])
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
+@pytest.mark.parametrize("name", TRACE_CASES)
+def test_gpu_matches_fixture(fq, name, precision):
+    d = load(name)
+    m = d["meta"]
+    kw = dict(m["tracer_kwargs"])
+    kw["step"] = kw.pop("s0")
+    s = d["seeds"]
+    print(f"fixture {name} precision {precision}", flush=True)
+    with fq.Field(*field_of(m)) as f:
+        if m["tracer"] == "scott":
+            r = _np(fq.trace_scott(f, s[:, 0], s[:, 1], s[:, 2], precision=precision, **kw))
+        else:
+            r = _np(fq.trace_plain(f, s[:, 0], s[:, 1], s[:, 2], precision=precision, **kw))
+    st = d["stable"]
+    cls = parity.class_match(r["flag_start"], r["flag_end"], d["flag_start"], d["flag_end"])
+    rk4 = kw["method"] == "rk4"
+    if m["tracer"] == "scott":
+        dq = parity.log_q_diff(r["q_0"], d["q0"])
+        print(parity.summary(dq, cls, st))
+        if rk4 and precision == "fp64":
+            assert cls[st].all() and dq[st].max() <= 1e-5
+        elif rk4:
+            assert cls[st].mean() >= 0.995 and np.mean(dq[st] <= 1e-3) >= 0.99
+        else:
+            # RKF45: classes within the adaptive controller's round-off forks
+            assert cls[st].mean() >= 0.95 and np.median(dq[st & cls]) <= 1e-3
+    else:
+        err = np.abs(r["end"] - d["end"]).max(axis=1)
+        assert cls[st].all()
+        assert err[st].max() < (1e-4 if precision == "fp64" else 2e-3)
+
+
+@pytest.mark.gpu
+def test_gpu_qcalc_fixture_bit_exact(fq):
+    from fastqsl_amd import FastQSL
+    with np.load(os.path.join(HERE, "qcalc_plane_hyperbolic.npz")) as z:
+        d = {k: z[k] for k in z.files}
+    names = ("x_end", "y_end", "z_end", "flag_end", "x_start", "y_start", "z_start", "flag_start",
+             "bx_in", "by_in", "bz_in", "bx_out", "by_out", "bz_out", "bx_0", "by_0", "bz_0", "b_flag")
+    q = FastQSL.QCalcPlane(*[d[n] for n in names], float(d["stride"]))
+    assert isinstance(q, np.ndarray)
+    np.testing.assert_array_equal(q, d["q"])

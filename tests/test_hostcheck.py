@@ -1,0 +1,98 @@
+"""The product's per-line tracer code (fastqsl_amd/csrc/fqsl_tracer.h), built
+for the host, against the FP64 oracle -- no GPU needed.
+
+The HIP kernels run exactly these functions per lane, so this pins their
+logic (the one-stencil difference quotient, the cell-stencil layout, the
+compensated FP32 position, the line driver and the Q assembly) on the CPU;
+the GPU tests then only have to show that the device build computes the same.
+"""
+import numpy as np
+import pytest
+
+from fastqsl_amd import synth
+from tests import parity
+from tests.hostcheck import hc
+
+
+@pytest.fixture(scope="module")
+def quad(oracle):
+    f = synth.quadrupole_64(dtype=np.float64, device="cpu")
+    rng = np.random.default_rng(11)
+    seeds = np.column_stack([rng.uniform(0, 63, 400), rng.uniform(0, 63, 400),
+                             np.zeros(400)]).astype(np.float32)
+    ref = oracle.trace_scott(f, seeds, method="rk4", s0=0.25)
+    refp = [oracle.trace_scott(f, seeds.astype(np.float64) + sgn * np.array([1e-6, -1e-6, 0.0]),
+                               method="rk4", s0=0.25) for sgn in (1.0, -1.0)]
+    return f, seeds, ref, parity.conditioned(ref, *refp)
+
+
+def test_one_stencil_quotient_equals_the_seven_stencil_quotient(oracle):
+    """grad b from one 8-corner stencil (closed-form +-h coverage, neighbour
+    cell across a plane, face clamps) vs the oracle's literal central
+    difference of seven clamped interpolations (TraceBlineScott.cu:204-226)."""
+    rng = np.random.default_rng(12)
+    shape = (9, 8, 7)
+    f = tuple(np.asfortranarray(rng.normal(size=shape) + 2.0) for _ in range(3))
+    hi = np.array(shape, float) - 1
+    pts = [rng.uniform(0, hi, size=(300, 3))]
+    # within h = 1e-3 of an interior plane or a face, and exactly on them
+    q = rng.uniform(0, hi, size=(300, 3))
+    ax = rng.integers(0, 3, 300)
+    q[np.arange(300), ax] = np.round(q[np.arange(300), ax]) + rng.choice([-6e-4, 0.0, 4e-4, 1e-3], 300)
+    pts.append(np.clip(q, 0, hi))
+    pts = np.concatenate(pts)
+    got = hc.grad_b(f, pts)
+    want = oracle.grad_b(f, pts, mode=0)
+    scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(got - want) / scale) < 1e-10
+
+
+def test_fp64_tracer_matches_the_oracle(quad):
+    f, seeds, ref, cond = quad
+    r = hc.trace_scott(f, seeds, precision=1, method=1, s0=0.25)
+    cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    assert cls[cond].all()
+    d = parity.log_q_diff(r["q0"], ref["q0"])
+    assert np.max(d[cond]) < 1e-9
+    np.testing.assert_array_equal(r["evals"], ref["evals"])
+
+
+def test_fp32c_tracer_matches_the_oracle(quad):
+    f, seeds, ref, cond = quad
+    r = hc.trace_scott(f, seeds, precision=0, method=1, s0=0.25)
+    cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    assert cls[cond].mean() >= 0.9975
+    d = parity.log_q_diff(r["q0"], ref["q0"])
+    assert np.mean(d[cond] <= 1e-3) >= 0.995
+    assert np.median(d[cond]) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_hyperbolic_closed_form(precision):
+    shape = (41, 41, 33)
+    k = 0.03
+    f = synth.hyperbolic(shape, k=k, b0=1.0)
+    rng = np.random.default_rng(13)
+    seeds = np.column_stack([rng.uniform(17, 23, 30), rng.uniform(17, 23, 30),
+                             np.zeros(30)]).astype(np.float32)
+    for method, s0, tol in ((1, 0.125, 0.0), (0, 0.5, 1e-9)):
+        r = hc.trace_scott(f, seeds, precision=precision, method=method, s0=s0, tol=tol)
+        assert np.all(r["flag_start"] == 5) and np.all(r["flag_end"] == 6)
+        np.testing.assert_allclose(r["q0"], 2 * np.cosh(2 * k * shape[2]), rtol=2e-3)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_plain_tracer_matches_the_oracle(oracle, quad, precision):
+    f, seeds, _, _ = quad
+    s = seeds[:120].astype(np.float32)
+    s[:, 2] = 0.5                                   # above the surface: traced both ways
+    ref = oracle.trace_plain(f, s, method="rk4", s0=0.25)
+    refp = [oracle.trace_plain(f, s.astype(np.float64) + sgn * np.array([1e-6, -1e-6, 0.0]),
+                               method="rk4", s0=0.25) for sgn in (1.0, -1.0)]
+    cond = parity.conditioned(ref, *refp, key="end")
+    r = hc.trace_plain(f, s, precision=precision, method=1, s0=0.25)
+    cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    assert cls[cond].all()
+    tol = 1e-8 if precision == 1 else 2e-3
+    assert np.abs(r["end"] - ref["end"])[cond].max() < tol
+    assert np.abs(r["start"] - ref["start"])[cond].max() < tol

@@ -192,6 +192,36 @@ def test_scott_uniform_field_q_is_two(oracle):
     assert np.all(r["length"] > 0)
 
 
+@pytest.mark.parametrize("method,s0,tol", [("rk4", 0.125, 0.0), ("rkf45", 0.5, 1e-9)])
+def test_scott_monopole_conformal_map(oracle, method, s0, tol):
+    """A single sub-surface monopole maps z = 0 to the top face by a radial
+    projection, which is conformal, so Q = 2.  The reference lands max-face
+    exits on the plane z = N, one cell beyond the last grid plane, where the
+    clamped field is B(x, y, N-1): there Bx/Bz = x/D (D = distance from the
+    source to plane N-1, in cells), so the slab scales the map by 1 + 1/D
+    while Bn at the landing point stays B(N-1)'s.  The Q formula then gives
+    2 (1 + 1/D)^2 -- a 4 % offset that a landing on N-1, or an inverted Bn
+    ratio, would miss (residual: trilinear error, (h/r)^2 ~ 4e-4)."""
+    n, depth = 33, 4.0                               # box [-1,1]^2 x [0,2], h = 1/16
+    f = synth.point_sources((n, n, n), [1.0], [(0.0, 0.0, -depth)], dtype=np.float64,
+                            device="cpu")
+    c, w = (n - 1) / 2, (n - 1) / 2 * 0.35
+    rng = np.random.default_rng(5)
+    seeds = np.column_stack([rng.uniform(c - w, c + w, 40), rng.uniform(c - w, c + w, 40),
+                             np.zeros(40)])
+    r = oracle.trace_scott(f, seeds, method=method, s0=s0, tol=tol)
+    assert np.all(r["flag_start"] == 5) and np.all(r["flag_end"] == 6)
+    D = (depth + 2.0) * (n - 1) / 2
+    Qc = 2 * (1 + 1 / D) ** 2
+    np.testing.assert_allclose(r["q0"], Qc, rtol=1.5e-3)
+    np.testing.assert_allclose(r["qperp"], Qc, rtol=1.5e-3)
+    # radial projection: the exit point lies on the ray from the source
+    src = np.array([c, c, -depth * (n - 1) / 2])
+    ray = (r["end"] - src) / np.linalg.norm(r["end"] - src, axis=1, keepdims=True)
+    ray0 = (r["start"] - src) / np.linalg.norm(r["start"] - src, axis=1, keepdims=True)
+    assert np.max(np.linalg.norm(ray - ray0, axis=1)) < 2e-3
+
+
 def test_scott_uniform_rkf45_grows_step_to_edge_exit(oracle):
     """With a zero error estimate the controller grows s to its 100-cell cap and
     the next step leaves through an edge: class 42, Q = 0 (reference rule)."""
