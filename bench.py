@@ -35,9 +35,14 @@ BYTES_PER_EVAL_F64 = 192
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (plane 20, volume 2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (plane 2, volume 1)")
+    ap.add_argument("--workload", default="plane", choices=["plane", "volume"],
+                    help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
+                         "of the 512x256x256 rope-arcade field")
     ap.add_argument("--pixels", type=int, default=1024)
+    ap.add_argument("--per-line-fp32", action="store_true",
+                    help="also time the FP32 one-thread-per-line kernel (always on for volume)")
     ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
     ap.add_argument("--schedule", default="persistent", choices=["persistent", "per_line"])
     ap.add_argument("--no-fp64-compare", action="store_true")
@@ -65,18 +70,66 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(field_np, seeds, sample, log):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(field_np, seeds_at, nlines, sample, log):
+    """The FP64 C oracle on a uniform sample of the same seeds, on all the
+    threads the process may use, plus a single-thread figure on a smaller
+    sample.  Returns (record, sample indices, oracle results)."""
     from oracle import oracle as O
     rng = np.random.default_rng(1)
-    idx = rng.choice(len(seeds), size=min(sample, len(seeds)), replace=False)
+    idx = np.sort(rng.choice(nlines, size=min(sample, nlines), replace=False))
+    seeds = seeds_at(idx)
     nt = O.num_threads()
     t0 = time.perf_counter()
-    O.trace_scott(field_np, seeds[idx], method="rkf45", s0=0.25, tol=1e-6, nthreads=nt)
+    ref = O.trace_scott(field_np, seeds, method="rkf45", s0=0.25, tol=1e-6, nthreads=nt)
     dt = time.perf_counter() - t0
-    log(f"cpu baseline: {len(idx)} lines in {dt:.2f} s on {nt} threads")
-    return {"value": len(idx) / dt, "unit": "lines/s", "cores": nt, "kind": "port",
-            "sample": f"{len(idx)} seeds drawn uniformly (default_rng(1)) from the same "
-                      f"{len(seeds)}-seed plane, FP64 C oracle, OpenMP dynamic schedule"}
+    n1 = min(len(idx), 4096 if nlines < 1 << 24 else 1024)
+    t1 = time.perf_counter()
+    O.trace_scott(field_np, seeds[:n1], method="rkf45", s0=0.25, tol=1e-6, nthreads=1)
+    dt1 = time.perf_counter() - t1
+    log(f"cpu baseline: {len(idx)} lines in {dt:.2f} s on {nt} threads; {n1} lines in "
+        f"{dt1:.2f} s on 1 thread")
+    rec = {"value": len(idx) / dt, "unit": "lines/s", "cores": nt, "kind": "port",
+           "sample": f"{len(idx)} seeds drawn uniformly (default_rng(1)) from the same "
+                     f"{nlines} seeds, FP64 C oracle, OpenMP dynamic schedule",
+           "cpu_model": cpu_model(), "single_thread_lines_per_s": n1 / dt1,
+           "oracle_evals_per_line": float(ref["evals"].mean())}
+    return rec, idx, ref
+
+
+def error_vs_oracle(out, idx, ref):
+    """Per-line error of the GPU map against the FP64 oracle on the sampled
+    lines: class match, |dlog10 Q| distribution, endpoint error in cells."""
+    g = {k: out[k][idx].cpu().numpy() for k in ("flag_start", "flag_end", "q_0", "end_x", "end_y",
+                                                 "end_z")}
+    cls = (g["flag_start"] == ref["flag_start"]) & (g["flag_end"] == ref["flag_end"])
+    a, b = g["q_0"].astype(np.float64), ref["q0"]
+    d = np.full(a.shape, np.inf)
+    ok = np.isfinite(a) & np.isfinite(b) & (a > 0) & (b > 0)
+    d[ok] = np.abs(np.log10(a[ok]) - np.log10(b[ok]))
+    d[(np.isnan(a) & np.isnan(b)) | ((a == 0) & (b == 0))] = 0.0
+    dm = d[cls]
+    fin = dm[np.isfinite(dm)]
+    end = np.column_stack([g["end_x"], g["end_y"], g["end_z"]]).astype(np.float64)
+    e = np.abs(end - ref["end"]).max(axis=1)[cls]
+    e = e[np.isfinite(e)]
+    return {"lines": int(len(idx)), "class_match": float(cls.mean()),
+            "frac_dlog10q_le_1e-3": float(np.mean(dm <= 1e-3)),
+            "dlog10q_median": float(np.median(fin)), "dlog10q_p999": float(np.quantile(fin, 0.999)),
+            "endpoint_err_cells_median": float(np.median(e)),
+            "endpoint_err_cells_p99": float(np.quantile(e, 0.99)),
+            "note": "RKF45 step sequences fork on round-off; the oracle against itself with "
+                    "tol x (1 + 1e-12) gives class match ~0.999 and ~0.67 within 1e-3"}
 
 
 def main():
@@ -94,28 +147,50 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
 
+    volume = args.workload == "volume"
+    if args.steps is None:
+        args.steps = 2 if volume else 20
+    if args.warmup is None:
+        args.warmup = 1 if volume else 2
     t0 = time.perf_counter()
-    bx, by, bz = fq.synth.multipole_256(seed=0, device="cuda")
-    log(f"field generated in {time.perf_counter() - t0:.1f} s")
-    n = args.pixels
-    g = np.linspace(0, 255, n, dtype=np.float32)
-    X, Y = np.meshgrid(g, g)
-    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size, np.float32)])
-    sx, sy, sz = (torch.as_tensor(np.ascontiguousarray(seeds[:, i]), device=dev) for i in range(3))
-    nlines = seeds.shape[0]
+    if volume:
+        bx, by, bz = fq.synth.rope_arcade(device="cuda")
+        from fastqsl_amd import qmap
+        shape = bx.shape
+        sx, sy, sz, _ = qmap.cube_seeds(*(np.arange(m, dtype=np.float32) for m in shape))
+        nx_, ny_, nz_ = shape
+
+        def seeds_at(idx):   # cube_seeds order: x slowest, z fastest
+            return np.column_stack([idx // (ny_ * nz_), (idx // nz_) % ny_, idx % nz_]).astype(np.float32)
+        nlines = int(sx.numel())
+        workload = ("Gold-Hoyle rope (1.5 turns, r 0.2, axis z 0.4) + 2 bipoles, 512x256x256 "
+                    "float32, 3-D Scott Q on every grid node, RKF45 tol 1e-6")
+    else:
+        bx, by, bz = fq.synth.multipole_256(seed=0, device="cuda")
+        n = args.pixels
+        g = np.linspace(0, 255, n, dtype=np.float32)
+        X, Y = np.meshgrid(g, g)
+        seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size, np.float32)])
+        sx, sy, sz = (torch.as_tensor(np.ascontiguousarray(seeds[:, i]), device=dev) for i in range(3))
+        seeds_at = lambda idx: seeds[idx]
+        nlines = seeds.shape[0]
+        workload = ("random multipole (40 sources, default_rng(0)) 256^3 float32, "
+                    "Scott Q, RKF45 tol 1e-6, z=0 plane %dx%d" % (n, n))
+    log(f"field generated in {time.perf_counter() - t0:.1f} s; {nlines} lines per step")
 
     layouts = ("soa64", "cell32")
     field = fq.Field(bx, by, bz, layouts=layouts)
     log(f"field resident: {field.nbytes / 1e9:.2f} GB ({', '.join(layouts)})")
 
-    out = None
+    outs = {}
     kw = dict(method="rkf45", step=0.25, tol=1e-6)
 
     def run(precision, schedule):
-        nonlocal out
-        out = fq.trace_scott(field, sx, sy, sz, precision=precision, schedule=schedule,
-                             out=out, **kw)
-        return out
+        # output buffers are reused across steps, one set per path
+        key = (precision, schedule)
+        outs[key] = fq.trace_scott(field, sx, sy, sz, precision=precision, schedule=schedule,
+                                   out=outs.get(key), **kw)
+        return outs[key]
 
     if args.calibrate:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
@@ -129,12 +204,16 @@ def main():
         run(args.precision, args.schedule)
     torch.cuda.synchronize()
     kernel_ms = []
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         run(args.precision, args.schedule)
-        kernel_ms.append(fq.last_kernel_ms())
+        kernel_ms.append(fq.last_kernel_ms())        # waits for the map's end event
+        step_ms.append((time.perf_counter() - ts) * 1e3)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    out = outs[(args.precision, args.schedule)]
     evals = int(out["evals"].to(torch.int64).sum().item())
     flags = torch.stack([out["flag_start"], out["flag_end"]]).cpu().numpy()
     ms_step = wall / args.steps * 1e3
@@ -146,6 +225,17 @@ def main():
         f"kernel {kms:.2f} ms, {evals / nlines:.1f} evals/line")
 
     extra = {}
+    if (volume or args.per_line_fp32) and (args.precision, args.schedule) == ("fp32c", "persistent"):
+        run("fp32c", "per_line")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        run("fp32c", "per_line")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        extra["fp32c_per_line"] = {"lines_per_s": nlines / dt, "ms_per_step": dt * 1e3,
+                                   "kernel_ms": fq.last_kernel_ms()}
+        extra["persistent_vs_per_line_fp32c"] = value / (nlines / dt)
+        log(f"fp32c/per_line: {nlines / dt:.4g} lines/s ({dt * 1e3:.1f} ms)")
     if not args.no_fp64_compare:
         run("fp64", "per_line")
         torch.cuda.synchronize()
@@ -161,7 +251,18 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         field_np = tuple(np.asarray(b, dtype=np.float64) for b in (bx, by, bz))
-        cpu = cpu_baseline(field_np, seeds, args.cpu_sample, log)
+        cpu, idx, ref = cpu_baseline(field_np, seeds_at, nlines,
+                                     args.cpu_sample if not volume else min(args.cpu_sample, 1 << 15),
+                                     log)
+        extra["error_vs_oracle"] = error_vs_oracle(out, idx, ref)
+        if ("fp64", "per_line") in outs:
+            extra["fp64_per_line"]["error_vs_oracle"] = error_vs_oracle(outs[("fp64", "per_line")],
+                                                                        idx, ref)
+        # the same map's bytes counted with the oracle's evaluations per line
+        # (the reference algorithm's work; the FP32 controller spends fewer)
+        ev_orc = cpu["oracle_evals_per_line"] * nlines
+        extra["roofline_at_oracle_evals"] = {"achieved": ev_orc * bpe / (kms * 1e-3) / 1e9,
+                                             "frac": ev_orc * bpe / (kms * 1e-3) / HBM_PEAK}
 
     classes, counts = np.unique(flags, return_counts=True)
     rec = {
@@ -172,18 +273,21 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                       "p99": float(np.percentile(step_ms, 99)),
+                       "what": "one full Q map, launch to results resident (host-synchronised)"},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32c" else "f64",
         "data": "synthetic",
-        "config": {"workload": "random multipole (40 sources, default_rng(0)) 256^3 float32, "
-                               "Scott Q, RKF45 tol 1e-6, z=0 plane %dx%d" % (n, n),
+        "config": {"workload": workload,
                    "lines_per_step": nlines, "precision": args.precision,
                    "schedule": args.schedule, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": (measured_traffic() if args.precision == "fp32c" and n == 1024 else None),
+                     "traffic": (measured_traffic() if args.precision == "fp32c" and not volume
+                                 and args.pixels == 1024 else None),
                      "traffic_source": "profiles/scott_rkf45_multipole256_1024sq.md (rocprofv3 "
                                        "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
                      "kernel_ms": kms, "evals_per_launch": evals, "bytes_per_eval": bpe},

@@ -38,6 +38,34 @@ def scott_qmap(field: Field, seeds, shape, method="rkf45", step=0.25, tol=1e-6,
     return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
 
 
+CUBE_OUTPUTS = ("q_0", "q_perp", "flag_start", "flag_end", "evals")
+
+
+def cube_seeds(x_axis, y_axis, z_axis):
+    """Seeds of a 3-D sampling box: every (x, y, z) of the three 1-D axes,
+    x slowest and z fastest, so that results reshape to the (nx, ny, nz) cube
+    the reference fills slice by slice (``Qube[:, :, iz]`` from
+    ``meshgrid(x_i, y_i, indexing='ij')`` at each z,
+    demo/Q_RKF45CuPy3DScott.ipynb cells [4]-[7])."""
+    import torch
+    ax = [a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a, np.float32))
+          for a in (x_axis, y_axis, z_axis)]
+    ax = [a.to(device="cuda", dtype=torch.float32).reshape(-1) for a in ax]
+    X, Y, Z = torch.meshgrid(*ax, indexing="ij")
+    return X.reshape(-1), Y.reshape(-1), Z.reshape(-1), tuple(int(a.numel()) for a in ax)
+
+
+def scott_qcube(field: Field, x_axis, y_axis, z_axis, method="rkf45", step=0.25, tol=1e-6,
+                precision="fp32c", schedule="persistent", outputs=CUBE_OUTPUTS):
+    """3-D Scott Q: one line per sample of the box x_axis x y_axis x z_axis
+    (grid-index coordinates), all voxels in one launch.  Returns a dict of
+    (nx, ny, nz) tensors (``Qube`` of the reference's 3-D notebooks)."""
+    x, y, z, shape = cube_seeds(x_axis, y_axis, z_axis)
+    r = trace_scott(field, x, y, z, method=method, step=step, tol=tol, precision=precision,
+                    schedule=schedule, outputs=outputs)
+    return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
+
+
 def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), method="rkf45",
                    step=0.125, tol_coef=1.0, pseudo_stride=None, pseudo_tol_scale=0.1,
                    pseudo_normal=None, precision="fp32c", schedule="persistent"):

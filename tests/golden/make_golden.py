@@ -63,7 +63,9 @@ def perturbed(trace, f, seeds, kw):
 def make(name):
     gen, gkw, tracer, tkw, kind = CASES[name]
     f = field_of(gen, gkw)
-    rng = np.random.default_rng(sum(map(ord, name)))
+    # seeds depend on the field and the seed kind only, so the RKF45 and the
+    # converged fixtures of one field trace the same lines
+    rng = np.random.default_rng(sum(map(ord, gen + json.dumps(gkw, sort_keys=True) + kind)))
     seeds = seeds_of(kind, f[0].shape, rng)
     trace = oracle.trace_scott if tracer == "scott" else oracle.trace_plain
     r = trace(f, seeds, **tkw)

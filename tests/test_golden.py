@@ -99,6 +99,8 @@ def test_gpu_matches_fixture(fq, name, precision):
     kw = dict(m["tracer_kwargs"])
     kw["step"] = kw.pop("s0")
     s = d["seeds"]
+    if name.endswith("_converged") and precision == "fp32c":
+        pytest.skip("tol 1e-10 is below the FP32 error estimate's round-off floor")
     print(f"fixture {name} precision {precision}", flush=True)
     with fq.Field(*field_of(m)) as f:
         if m["tracer"] == "scott":
@@ -115,9 +117,27 @@ def test_gpu_matches_fixture(fq, name, precision):
             assert cls[st].all() and dq[st].max() <= 1e-5
         elif rk4:
             assert cls[st].mean() >= 0.995 and np.mean(dq[st] <= 1e-3) >= 0.99
+        elif name.endswith("_converged"):
+            assert cls[st].mean() >= 0.99 and np.median(dq[st & cls]) <= 1e-5
         else:
-            # RKF45: classes within the adaptive controller's round-off forks
-            assert cls[st].mean() >= 0.95 and np.median(dq[st & cls]) <= 1e-3
+            # RKF45 at a working tolerance forks on round-off line by line, so
+            # the GPU is held to the converged fixture of the same seeds: no
+            # less accurate than the oracle at the same tolerance
+            conv = load(name.replace("_rkf45", "_converged"))
+            assert np.array_equal(conv["seeds"], s)
+            same = parity.class_match(d["flag_start"], d["flag_end"], conv["flag_start"],
+                                      conv["flag_end"])
+            cls_t = parity.class_match(r["flag_start"], r["flag_end"], conv["flag_start"],
+                                       conv["flag_end"])
+            err_orc = parity.log_q_diff(d["q0"], conv["q0"])
+            err_gpu = parity.log_q_diff(r["q_0"], conv["q0"])
+            mk = conv["stable"] & same & cls_t & np.isfinite(err_orc) & np.isfinite(err_gpu)
+            print("vs converged: median gpu %.3g oracle %.3g" % (np.median(err_gpu[mk]),
+                                                                np.median(err_orc[mk])))
+            assert cls[st].mean() >= 0.95
+            assert cls_t[conv["stable"]].mean() >= same[conv["stable"]].mean() - 0.02
+            assert np.median(err_gpu[mk]) <= 1.5 * np.median(err_orc[mk]) + 1e-6
+            assert np.quantile(err_gpu[mk], 0.99) <= 1.5 * np.quantile(err_orc[mk], 0.99) + 1e-6
     else:
         err = np.abs(r["end"] - d["end"]).max(axis=1)
         assert cls[st].all()
