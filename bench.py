@@ -41,6 +41,8 @@ def parse():
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
                          "of the 512x256x256 rope-arcade field")
     ap.add_argument("--pixels", type=int, default=1024)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch knobs instead of autotuning them")
     ap.add_argument("--per-line-fp32", action="store_true",
                     help="also time the FP32 one-thread-per-line kernel (always on for volume)")
     ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
@@ -192,6 +194,15 @@ def main():
                                    out=outs.get(key), **kw)
         return outs[key]
 
+    tuning = None
+    if not args.no_autotune and args.schedule == "persistent":
+        from fastqsl_amd import tune
+        t1 = time.perf_counter()
+        cfg = tune.autotune(field, sx, sy, sz, precision=args.precision, use_cache=False, **kw)
+        tuning = {"cfg": cfg, "seconds": time.perf_counter() - t1,
+                  "how": "coordinate descent on a 2^19-line sample of this workload"}
+        log(f"autotuned in {tuning['seconds']:.1f} s: {cfg}")
+
     if args.calibrate:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -292,6 +303,7 @@ def main():
                                        "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
                      "kernel_ms": kms, "evals_per_launch": evals, "bytes_per_eval": bpe},
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "classes": {str(int(c)): int(k) for c, k in zip(classes, counts)},
         **extra,
     }

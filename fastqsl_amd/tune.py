@@ -1,0 +1,155 @@
+"""Autotuner for the persistent tracers' launch parameters.
+
+The knobs (``_lib.TUNE_KNOBS``) change speed only -- a line's result does not
+depend on the lane, wave, queue or launch that traces it -- so they can be
+picked per workload by measurement:
+
+* ``scott_waves``  waves per SIMD the FP32 Scott stepping kernel is built for
+                   (2 = 256 VGPRs per lane; 0 = the compiler's choice);
+* ``refill_min``   idle lanes a wavefront gathers before it claims new lines
+                   (the queue chunk: one atomic per chunk);
+* ``queues``       one seed queue, or one per XCD (8), the launch's blocks
+                   spread over them;
+* ``grid_pct``     persistent grid size in % of one full occupancy wave.
+
+``autotune`` runs coordinate descent over these on a sample of the actual
+request (the persistent kernel needs >= ~4 lines per lane of one occupancy
+wave to show its steady state, hence the default sample of 2^19 lines), keeps
+the fastest by the HIP-event time of the whole trace (prep, stepping, Q
+assembly), applies it and caches it per (device, field shape, method,
+precision, tolerance decade) in a JSON file.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+from . import _lib
+
+SPACE = {
+    "scott_waves": (2, 0),
+    "refill_min": (8, 4, 16, 32, 1),
+    "queues": (1, 8),
+    "grid_pct": (100, 75, 50),
+}
+DEFAULTS = {"scott_waves": 2, "refill_min": 8, "queues": 1, "grid_pct": 100}
+
+
+def cache_path() -> str:
+    return os.environ.get("FQSL_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "fastqsl_amd", "tune.json"))
+
+
+def cache_key(device: str, shape, method: str, precision: str, tol: float) -> str:
+    dec = "none" if not tol or tol <= 0 else "1e" + str(int(math.floor(math.log10(tol) + 1e-9)))
+    return f"{device}|{'x'.join(str(int(s)) for s in shape)}|{method}|{precision}|tol{dec}"
+
+
+def load_cache(path=None) -> dict:
+    try:
+        with open(path or cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(data: dict, path=None) -> None:
+    p = path or cache_path()
+    os.makedirs(os.path.dirname(p), exist_ok=True)
+    tmp = p + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    os.replace(tmp, p)
+
+
+def current() -> dict:
+    return {k: _lib.tune_get(k) for k in _lib.TUNE_KNOBS}
+
+
+def apply(cfg: dict) -> None:
+    for k, v in cfg.items():
+        _lib.tune_set(k, int(v))
+
+
+def coordinate_descent(measure, space=SPACE, start=DEFAULTS, rounds=2, log=None):
+    """Minimise measure(cfg) -> ms over `space`, one knob at a time, starting
+    from `start`; stops after a round without improvement.  Returns
+    (best cfg, best ms, {cfg as tuple: ms} of every point measured)."""
+    seen = {}
+
+    def m(cfg):
+        key = tuple(sorted(cfg.items()))
+        if key not in seen:
+            seen[key] = float(measure(dict(cfg)))
+            if log:
+                log(f"tune {dict(cfg)}: {seen[key]:.3f} ms")
+        return seen[key]
+
+    best = dict(start)
+    best_ms = m(best)
+    for _ in range(rounds):
+        improved = False
+        for knob, values in space.items():
+            for v in values:
+                if v == best[knob]:
+                    continue
+                cand = dict(best, **{knob: v})
+                ms = m(cand)
+                if ms < best_ms * 0.99:          # keep a change only for a clear gain
+                    best, best_ms, improved = cand, ms, True
+        if not improved:
+            break
+    return best, best_ms, seen
+
+
+def autotune(field, x, y, z, method="rkf45", step=0.25, tol=1e-6, precision="fp32c",
+             sample=1 << 19, repeats=2, use_cache=True, log=None):
+    """Pick the knobs for tracing seeds (x, y, z) on `field` with the persistent
+    Scott tracer; applies them and returns the chosen configuration."""
+    import torch
+    from .trace import trace_scott
+    dev = torch.cuda.get_device_name(torch.cuda.current_device())
+    key = cache_key(dev, field.shape, method, precision, tol)
+    if use_cache:
+        hit = load_cache().get(key)
+        if hit:
+            apply(hit["cfg"])
+            return dict(hit["cfg"])
+    xs, ys, zs = (torch.as_tensor(np.asarray(a, np.float32)) if not isinstance(a, torch.Tensor) else a
+                  for a in (x, y, z))
+    n = int(xs.numel())
+    if n > sample:
+        g = torch.Generator(device="cpu").manual_seed(0)
+        idx = torch.randperm(n, generator=g)[:sample].to(xs.device)
+        xs, ys, zs = xs.reshape(-1)[idx], ys.reshape(-1)[idx], zs.reshape(-1)[idx]
+    before = current()
+    out = {}
+
+    def measure(cfg):
+        apply(cfg)
+        ms = []
+        for r in range(repeats + 1):
+            res = trace_scott(field, xs, ys, zs, method=method, step=step, tol=tol,
+                              precision=precision, schedule="persistent",
+                              outputs=("q_0", "flag_start", "flag_end"), out=out.get("o"))
+            out["o"] = res
+            t = _lib.last_kernel_ms()
+            if r:
+                ms.append(t)
+        return float(np.median(ms))
+
+    try:
+        best, best_ms, seen = coordinate_descent(measure, log=log)
+    except Exception:
+        apply(before)
+        raise
+    apply(best)
+    if use_cache:
+        data = load_cache()
+        data[key] = {"cfg": best, "ms": best_ms, "lines": int(xs.numel()),
+                     "points": len(seen)}
+        save_cache(data)
+    return best
