@@ -53,6 +53,12 @@ class fqsl_plain_out(ctypes.Structure):
                                   "b_end_y", "b_end_z", "line_len", "evals")]
 
 
+class fqsl_scott_request(ctypes.Structure):
+    _fields_ = [("field", vp), ("method", i32), ("reserved", i32), ("step", ctypes.c_double),
+                ("tol", ctypes.c_double), ("n", u64), ("x", vp), ("y", vp), ("z", vp),
+                ("out", fqsl_scott_out)]
+
+
 # name -> (restype, argtypes)
 PROTOTYPES = {
     "fqsl_last_error": (ctypes.c_char_p, []),
@@ -71,6 +77,12 @@ PROTOTYPES = {
     "fqsl_qcut_scatter": (i32, [u64] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),
     "fqsl_TraceAllBlineScott": (i32, [vp] * 30 + [i32, i32, vp]),
     "fqsl_TraceAllBline": (i32, [i32] + [vp] * 35 + [i32, i32, vp]),
+    "fqsl_scott_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(fqsl_scott_request), i32, i32]),
+    "fqsl_scott_batch_create": (i32, [ctypes.POINTER(fqsl_scott_request), i32, i32, vp, ctypes.c_size_t, vp,
+                                      ctypes.POINTER(vp)]),
+    "fqsl_scott_batch_run": (i32, [vp, vp]),
+    "fqsl_scott_batch_times": (i32, [vp, ctypes.POINTER(ctypes.c_double), vp]),
+    "fqsl_scott_batch_destroy": (i32, [vp]),
     "fqsl_last_kernel_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
     "fqsl_tune_set": (i32, [i32, i32]),
     "fqsl_tune_get": (i32, [i32, ctypes.POINTER(i32)]),

@@ -16,6 +16,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -793,6 +794,414 @@ extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params
         return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
     }
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
+// ----------------------------------------------------------------- batches
+// A batch of Scott requests, each with its own field, method, step,
+// tolerance, seeds and outputs.  Requests are grouped by (field, method); the
+// lines of a group form one contiguous range [base, base + n) of the batch's
+// line index and are drained by one persistent launch.  A line finds its
+// request by binary search over the requests' first-line offsets and takes
+// that request's step and tolerance, so one launch mixes requests of any
+// size.  All state lives in a workspace laid out at plan time, and a run only
+// launches kernels (the counters are cleared by a kernel), so a run can be
+// captured in a graph and replayed.
+struct BatchReq {
+    Seeds s;
+    ScottOut o;
+    double step, tol;
+    unsigned long long n;
+};
+
+struct BatchCtl {
+    unsigned long long* queues;     // one counter per group
+    unsigned long long* remaining;  // lines of each request not yet finished
+    unsigned long long* stamps;     // [r]: device wall clock when request r's last line
+                                    // finished stepping; [nreq]: start of the run
+};
+
+// Largest r in [r0, r1) with offs[r] <= i (requests without lines share the
+// offset of the next one and are never returned for a line).
+__device__ __forceinline__ int batch_find(const unsigned long long* __restrict__ offs, int r0, int r1,
+                                          unsigned long long i) {
+    int lo = r0, hi = r1 - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void batch_line_done(const BatchCtl& c, int r) {
+    if (atomicSub(c.remaining + r, 1ull) == 1ull) c.stamps[r] = wall_clock64();
+}
+
+__global__ void k_batch_reset(BatchCtl c, const BatchReq* __restrict__ reqs, int nreq, int ngroups) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ngroups) c.queues[t] = 0ull;
+    if (t < nreq) {
+        c.remaining[t] = reqs[t].n;
+        c.stamps[t] = 0ull;
+    }
+    if (t == 0) c.stamps[nreq] = wall_clock64();
+}
+
+struct BatchGroupArgs {
+    const BatchReq* reqs;
+    const unsigned long long* offs;
+    int r0, r1;
+    unsigned long long base, n;
+};
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_batch_prep(F f, BatchGroupArgs g, R* __restrict__ frames) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < g.n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long gi = g.base + i;
+        const int r = batch_find(g.offs, g.r0, g.r1, gi);
+        const unsigned long long li = gi - g.offs[r];
+        const Seeds s = g.reqs[r].s;
+        const float p0[3] = {s.x[li], s.y[li], s.z[li]};
+        R u0[3], v0[3], b0z;
+        double nb0;
+        seed_frame<R>(f, p0, u0, v0, &b0z, &nb0);
+        R* fr = frames + (unsigned long long)kFrameWords * gi;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { fr[a] = u0[a]; fr[3 + a] = v0[a]; }
+        fr[6] = b0z;
+        fr[7] = (R)nb0;
+        const ScottOut o = g.reqs[r].o;
+        if (o.btx) {
+            float bt[3];
+            b_clamped<R>(f, p0, bt);
+            o.btx[li] = bt[0]; o.bty[li] = bt[1]; o.btz[li] = bt[2];
+        }
+    }
+}
+
+// Persistent stepping over one group: as scott_persistent_body, with the
+// step and tolerance of each lane's request.
+template <typename R, class F>
+__device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs g, const R* __restrict__ frames,
+                                                 ScottRec<R>* __restrict__ recs, BatchCtl c, int qi,
+                                                 int refill_min) {
+    ScottLine<R> L;
+    ScottParams P{method, 0.0, 0.0};
+    bool active = false;
+    bool drained = false;
+    unsigned long long idx = 0;
+    const int lane = __lane_id();
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    QueueCursor qc{0, 0};
+    for (;;) {
+        if (!drained) {
+            const unsigned long long need = __ballot(!active);
+            const int cnt = __popcll(need);
+            if (cnt >= refill_min || need == ~0ull) {
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned long long base = 0;
+                int avail = 0;
+                if (!claim(c.queues + qi, 1, g.n, qc, cnt, leader, base, avail)) drained = true;
+                const int rank = __popcll(need & below);
+                if (!active && rank < avail) {
+                    idx = g.base + base + (unsigned long long)rank;
+                    const int r = batch_find(g.offs, g.r0, g.r1, idx);
+                    const unsigned long long li = idx - g.offs[r];
+                    P.s0 = g.reqs[r].step;
+                    P.tol = g.reqs[r].tol;
+                    const Seeds s = g.reqs[r].s;
+                    ScottEnd<R> e;
+                    const int st = scott_init<R>(L, f, P, s.x[li], s.y[li], s.z[li], e,
+                                                 frames + (unsigned long long)kFrameWords * idx);
+                    if (st != kTracing) {
+                        recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                        batch_line_done(c, r);
+                    } else {
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            ScottEnd<R> e;
+            const int st = scott_attempt<R>(L, f, P, e);
+            if (st != kTracing) {
+                recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                active = false;
+                batch_line_done(c, batch_find(g.offs, g.r0, g.r1, idx));
+            }
+        }
+    }
+}
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_batch_step(F f, int method, BatchGroupArgs g, const R* frames,
+                                                    ScottRec<R>* recs, BatchCtl c, int qi, int refill_min) {
+    scott_batch_body<R, F>(f, method, g, frames, recs, c, qi, refill_min);
+}
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_batch_step_w2(F f, int method, BatchGroupArgs g, const R* frames, ScottRec<R>* recs, BatchCtl c, int qi,
+                int refill_min) {
+    scott_batch_body<R, F>(f, method, g, frames, recs, c, qi, refill_min);
+}
+
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_batch_finish(F f, BatchGroupArgs g, const R* __restrict__ frames,
+                                                      const ScottRec<R>* __restrict__ recs) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < g.n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long gi = g.base + i;
+        const int r = batch_find(g.offs, g.r0, g.r1, gi);
+        const unsigned long long li = gi - g.offs[r];
+        const ScottOut o = g.reqs[r].o;
+        const ScottRec<R> rec = recs[gi];
+        double q0, qp;
+        scott_q<R>(rec.st, rec.en, rec.status, (double)frames[(unsigned long long)kFrameWords * gi + 7], f, &q0,
+                   &qp);
+        if (o.sx) { o.sx[li] = (float)rec.st.x[0]; o.sy[li] = (float)rec.st.x[1]; o.sz[li] = (float)rec.st.x[2]; }
+        if (o.ex) { o.ex[li] = (float)rec.en.x[0]; o.ey[li] = (float)rec.en.x[1]; o.ez[li] = (float)rec.en.x[2]; }
+        if (o.fs) o.fs[li] = rec.st.flag;
+        if (o.fe) o.fe[li] = rec.en.flag;
+        if (o.q0) o.q0[li] = (float)q0;
+        if (o.qp) o.qp[li] = (float)qp;
+        if (o.len) o.len[li] = rec.len;
+        if (o.evals) o.evals[li] = rec.evals;
+    }
+}
+
+struct BatchGroup {
+    const fqsl_field* field;
+    int method;
+    int r0, r1;
+    unsigned long long base, n;
+};
+
+struct fqsl_scott_batch {
+    int device;
+    int precision;
+    int nreq;
+    int refill_min;
+    int waves;
+    std::vector<BatchGroup> groups;
+    std::vector<int> order;  // batch position -> caller's request index
+    char* ws;                // caller's workspace
+    BatchReq* reqs;
+    unsigned long long* offs;
+    BatchCtl ctl;
+    void* frames;
+    void* recs;
+};
+
+namespace {
+// Workspace layout of a batch: [queues | remaining | stamps | reqs | offs | frames | recs]
+struct BatchLayout {
+    size_t queues, remaining, stamps, reqs, offs, frames, recs, total;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+BatchLayout batch_layout(int nreq, int ngroups, unsigned long long lines, int precision) {
+    const size_t rsz = precision == FQSL_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const size_t recsz = precision == FQSL_PREC_FP64 ? sizeof(ScottRec<double>) : sizeof(ScottRec<float>);
+    BatchLayout L;
+    size_t o = 0;
+    L.queues = o; o = align256(o + sizeof(unsigned long long) * (size_t)(ngroups > 0 ? ngroups : 1));
+    L.remaining = o; o = align256(o + sizeof(unsigned long long) * (size_t)(nreq > 0 ? nreq : 1));
+    L.stamps = o; o = align256(o + sizeof(unsigned long long) * (size_t)(nreq + 1));
+    L.reqs = o; o = align256(o + sizeof(BatchReq) * (size_t)(nreq > 0 ? nreq : 1));
+    L.offs = o; o = align256(o + sizeof(unsigned long long) * (size_t)(nreq > 0 ? nreq : 1));
+    L.frames = o; o = align256(o + (size_t)kFrameWords * rsz * lines);
+    L.recs = o; o = align256(o + recsz * lines);
+    L.total = o;
+    return L;
+}
+
+int check_request(const fqsl_scott_request& q, int precision, int k) {
+    if (!q.field) return set_err(FQSL_ERR_INVALID, "request %d: NULL field", k);
+    if (q.method != FQSL_METHOD_RKF45 && q.method != FQSL_METHOD_RK4)
+        return set_err(FQSL_ERR_INVALID, "request %d: method %d", k, q.method);
+    if (!(q.step > 0) || !(q.step <= 100.0)) return set_err(FQSL_ERR_INVALID, "request %d: step %g outside (0, 100]", k, q.step);
+    if (q.method == FQSL_METHOD_RKF45 && !(q.tol > 0)) return set_err(FQSL_ERR_INVALID, "request %d: tol %g must be > 0", k, q.tol);
+    if (q.n && (!q.x || !q.y || !q.z)) return set_err(FQSL_ERR_INVALID, "request %d: NULL seed array", k);
+    const fqsl_scott_out& o = q.out;
+    if ((o.start_x && (!o.start_y || !o.start_z)) || (o.end_x && (!o.end_y || !o.end_z)) ||
+        (o.b_this_x && (!o.b_this_y || !o.b_this_z)))
+        return set_err(FQSL_ERR_INVALID, "request %d: x/y/z output arrays must be all set or all NULL", k);
+    if (precision == FQSL_PREC_FP64 && !q.field->soa64[0])
+        return set_err(FQSL_ERR_INVALID, "request %d: field has no FP64 layout", k);
+    if (precision == FQSL_PREC_FP32C && !q.field->cell32 && !q.field->soa32[0])
+        return set_err(FQSL_ERR_INVALID, "request %d: field has no FP32 layout", k);
+    return FQSL_OK;
+}
+
+// Requests in batch order: grouped by (field, method), stable within a group.
+std::vector<int> batch_order(const fqsl_scott_request* reqs, int nreq) {
+    std::vector<int> ord(nreq);
+    for (int k = 0; k < nreq; ++k) ord[k] = k;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        if (reqs[a].field != reqs[b].field) return reqs[a].field < reqs[b].field;
+        return reqs[a].method < reqs[b].method;
+    });
+    return ord;
+}
+
+template <typename R, class F>
+int batch_launch_group(const fqsl_scott_batch* b, const BatchGroup& G, int gi, const F& f, hipStream_t st) {
+    BatchGroupArgs g{b->reqs, b->offs, G.r0, G.r1, G.base, G.n};
+    R* frames = (R*)b->frames;
+    ScottRec<R>* recs = (ScottRec<R>*)b->recs;
+    auto kern = sizeof(R) == 4 && b->waves == 2 ? k_batch_step_w2<R, F> : k_batch_step<R, F>;
+    int grid = 0;
+    int rc = persistent_grid(kern, &grid);
+    if (rc) return rc;
+    grid = (int)((long)grid * g_tune[FQSL_TUNE_GRID_PCT] / 100);
+    if (grid < 1) grid = 1;
+    k_batch_prep<R, F><<<grid_for((long)G.n, 256), 256, 0, st>>>(f, g, frames);
+    kern<<<grid, 256, 0, st>>>(f, G.method, g, frames, recs, b->ctl, gi, b->refill_min);
+    k_batch_finish<R, F><<<grid_for((long)G.n, 256), 256, 0, st>>>(f, g, frames, recs);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+}  // namespace
+
+extern "C" size_t fqsl_scott_batch_workspace_bytes(const fqsl_scott_request* reqs, int nreq, int precision) {
+    if (!reqs || nreq <= 0) return 0;
+    unsigned long long lines = 0;
+    for (int k = 0; k < nreq; ++k) lines += reqs[k].n;
+    std::vector<int> ord = batch_order(reqs, nreq);
+    int ngroups = 0;
+    for (int k = 0; k < nreq; ++k)
+        if (k == 0 || reqs[ord[k]].field != reqs[ord[k - 1]].field || reqs[ord[k]].method != reqs[ord[k - 1]].method)
+            ++ngroups;
+    return batch_layout(nreq, ngroups, lines, precision).total;
+}
+
+extern "C" int fqsl_scott_batch_create(const fqsl_scott_request* reqs, int nreq, int precision, void* workspace,
+                                       size_t workspace_bytes, void* stream, fqsl_scott_batch** out) {
+    if (!out) return set_err(FQSL_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!reqs || nreq <= 0) return set_err(FQSL_ERR_INVALID, "empty batch");
+    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64)
+        return set_err(FQSL_ERR_INVALID, "precision %d", precision);
+    for (int k = 0; k < nreq; ++k) {
+        int rc = check_request(reqs[k], precision, k);
+        if (rc) return rc;
+    }
+    const size_t need = fqsl_scott_batch_workspace_bytes(reqs, nreq, precision);
+    if (!workspace || workspace_bytes < need)
+        return set_err(FQSL_ERR_INVALID, "workspace of %zu bytes given, %zu needed", workspace_bytes, need);
+    if (((uintptr_t)workspace & 255) != 0) return set_err(FQSL_ERR_INVALID, "workspace must be 256-byte aligned");
+    fqsl_scott_batch* b = new fqsl_scott_batch();
+    HIP_TRY(hipGetDevice(&b->device));
+    b->precision = precision;
+    b->nreq = nreq;
+    b->refill_min = g_tune[FQSL_TUNE_REFILL_MIN];
+    b->waves = g_tune[FQSL_TUNE_SCOTT_WAVES];
+    b->order = batch_order(reqs, nreq);
+    std::vector<BatchReq> table(nreq);
+    std::vector<unsigned long long> offs(nreq);
+    unsigned long long lines = 0;
+    for (int k = 0; k < nreq; ++k) {
+        const fqsl_scott_request& q = reqs[b->order[k]];
+        if (k == 0 || q.field != b->groups.back().field || q.method != b->groups.back().method)
+            b->groups.push_back(BatchGroup{q.field, q.method, k, k, lines, 0});
+        BatchGroup& G = b->groups.back();
+        G.r1 = k + 1;
+        G.n += q.n;
+        offs[k] = lines;
+        const fqsl_scott_out& o = q.out;
+        table[k] = BatchReq{Seeds{q.x, q.y, q.z},
+                            ScottOut{o.start_x, o.start_y, o.start_z, o.end_x, o.end_y, o.end_z, o.flag_start,
+                                     o.flag_end, o.b_this_x, o.b_this_y, o.b_this_z, o.q_0, o.q_perp, o.line_len,
+                                     o.evals},
+                            q.step, q.method == FQSL_METHOD_RKF45 ? q.tol : 0.0, q.n};
+        lines += q.n;
+    }
+    const BatchLayout L = batch_layout(nreq, (int)b->groups.size(), lines, precision);
+    char* ws = (char*)workspace;
+    b->ws = ws;
+    b->reqs = (BatchReq*)(ws + L.reqs);
+    b->offs = (unsigned long long*)(ws + L.offs);
+    b->ctl = BatchCtl{(unsigned long long*)(ws + L.queues), (unsigned long long*)(ws + L.remaining),
+                      (unsigned long long*)(ws + L.stamps)};
+    b->frames = ws + L.frames;
+    b->recs = ws + L.recs;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(b->reqs, table.data(), sizeof(BatchReq) * nreq, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->offs, offs.data(), sizeof(unsigned long long) * nreq, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        delete b;
+        return set_err(FQSL_ERR_HIP, "batch table upload: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_scott_batch_run(const fqsl_scott_batch* b, void* stream) {
+    if (!b) return set_err(FQSL_ERR_INVALID, "batch is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cap));
+    const bool timed = cap == hipStreamCaptureStatusNone;
+    if (timed) {
+        int rc = record_start(st);
+        if (rc) return rc;
+    }
+    const int ng = (int)b->groups.size();
+    k_batch_reset<<<grid_for((long)(b->nreq + 1 > ng ? b->nreq + 1 : ng), 256), 256, 0, st>>>(b->ctl, b->reqs, b->nreq,
+                                                                                             ng);
+    HIP_TRY(hipGetLastError());
+    for (int gi = 0; gi < ng; ++gi) {
+        const BatchGroup& G = b->groups[gi];
+        if (G.n == 0) continue;
+        const fqsl_field* fd = G.field;
+        int rc;
+        if (b->precision == FQSL_PREC_FP64) {
+            FieldSoA<double> f{fd->soa64[0], fd->soa64[1], fd->soa64[2], {fd->n[0], fd->n[1], fd->n[2]}};
+            rc = batch_launch_group<double>(b, G, gi, f, st);
+        } else if (fd->cell32) {
+            FieldCell32 f{fd->cell32, {fd->n[0], fd->n[1], fd->n[2]}};
+            rc = batch_launch_group<float>(b, G, gi, f, st);
+        } else {
+            FieldSoA<float> f{fd->soa32[0], fd->soa32[1], fd->soa32[2], {fd->n[0], fd->n[1], fd->n[2]}};
+            rc = batch_launch_group<float>(b, G, gi, f, st);
+        }
+        if (rc) return rc;
+    }
+    if (timed) return record_end(st);
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_scott_batch_times(const fqsl_scott_batch* b, double* ms, void* stream) {
+    if (!b || !ms) return set_err(FQSL_ERR_INVALID, "NULL batch or ms");
+    std::vector<unsigned long long> stamps(b->nreq + 1);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(stamps.data(), b->ctl.stamps, sizeof(unsigned long long) * (b->nreq + 1),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    if (khz <= 0) return set_err(FQSL_ERR_HIP, "device wall clock rate unknown");
+    const unsigned long long t0 = stamps[b->nreq];
+    for (int k = 0; k < b->nreq; ++k) {
+        const unsigned long long t = stamps[k];
+        ms[b->order[k]] = t ? (double)(long long)(t - t0) / (double)khz : 0.0;
+    }
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_scott_batch_destroy(fqsl_scott_batch* b) {
+    delete b;
+    return FQSL_OK;
 }
 
 extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n, const float* x,

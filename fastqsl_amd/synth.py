@@ -191,3 +191,38 @@ def volume_seeds(shape, dtype=np.float32):
     nx, ny, nz = shape
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     return np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(dtype)
+
+
+def batch_requests(shapes, count=256, seed=0, side=(64, 1024), tol=(1e-7, 1e-5), step=0.25):
+    """Uneven Q-map requests over resident fields of the given shapes: each
+    picks a field uniformly, an arbitrarily oriented plane through three
+    points A, B, C drawn uniformly in that field's box (grid-index
+    coordinates), a side of n x n pixels with n log-uniform in `side`, RK4 or
+    RKF45 with equal odds, and for RKF45 a tolerance log-uniform in `tol`.
+    Draw order per request (default_rng(seed)): field, A, B, C, n, method,
+    tol.  Returns a list of dicts (field index, A, B, C, n, method, tol, step)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        fi = int(rng.integers(len(shapes)))
+        hi = np.asarray(shapes[fi], np.float64) - 1
+        A, B, C = (rng.uniform(0, 1, 3) * hi for _ in range(3))
+        n = int(round(math.exp(rng.uniform(math.log(side[0]), math.log(side[1])))))
+        method = "rk4" if rng.uniform() < 0.5 else "rkf45"
+        t = float(math.exp(rng.uniform(math.log(tol[0]), math.log(tol[1]))))
+        out.append(dict(field=fi, A=A, B=B, C=C, n=n, method=method,
+                        tol=t if method == "rkf45" else 0.0, step=step))
+    return out
+
+
+def plane_seeds_abc_torch(A, B, C, n_u, n_v, device="cuda"):
+    """plane_seeds_abc on the device: x, y, z float32 tensors of n_v * n_u
+    seeds (rows follow C - A), computed in float64 as the host version."""
+    torch = _torch()
+    A, B, C = (torch.as_tensor(np.asarray(v, np.float64), device=device) for v in (A, B, C))
+    u = torch.linspace(0.0, 1.0, n_u, dtype=torch.float64, device=device)
+    v = torch.linspace(0.0, 1.0, n_v, dtype=torch.float64, device=device)
+    V, U = torch.meshgrid(v, u, indexing="ij")
+    P = A + (B - A) * U[..., None] + (C - A) * V[..., None]
+    P = P.reshape(-1, 3).to(torch.float32)
+    return P[:, 0].contiguous(), P[:, 1].contiguous(), P[:, 2].contiguous()

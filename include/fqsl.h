@@ -105,6 +105,49 @@ int fqsl_trace_scott(const fqsl_field *field, const fqsl_scott_params *params, u
                      const float *x, const float *y, const float *z, const fqsl_scott_out *out,
                      void *stream);
 
+/* ----------------------------------------------------------------- batches */
+/* Many Scott requests -- each with its own field, method, step, tolerance,
+ * seeds and outputs -- traced together.  The reference has no batch call:
+ * its notebooks launch TraceAllBlineScott once per map
+ * (demo/Q_RKF45CuPy3DScott.ipynb cell [7] loops over slices); a batch replaces
+ * such a loop of launches.  Requests are grouped by (field, method) and each
+ * group is drained by one persistent launch. */
+typedef struct {
+    const fqsl_field *field;
+    int method;       /* FQSL_METHOD_* */
+    int reserved;
+    double step;      /* initial (RKF45) or fixed (RK4) step, cells */
+    double tol;       /* RKF45 absolute per-step tolerance, cells */
+    uint64_t n;       /* number of seeds (0 allowed) */
+    const float *x, *y, *z;
+    fqsl_scott_out out;
+} fqsl_scott_request;
+
+typedef struct fqsl_scott_batch fqsl_scott_batch;
+
+/* Device workspace (bytes, 256-byte aligned) a batch of these requests needs:
+ * request table, counters and per-line scratch (8 words + one record per line). */
+size_t fqsl_scott_batch_workspace_bytes(const fqsl_scott_request *reqs, int nreq, int precision);
+
+/* Plans a batch in `workspace` (device memory the caller owns and keeps alive,
+ * with the fields, seeds and outputs, for the batch's lifetime): uploads the
+ * request table and synchronises `stream` once. */
+int fqsl_scott_batch_create(const fqsl_scott_request *reqs, int nreq, int precision, void *workspace,
+                            size_t workspace_bytes, void *stream, fqsl_scott_batch **batch);
+
+/* Traces every request of the batch on `stream`.  Only launches kernels (no
+ * allocation, no memset, no synchronisation), so it may be captured in a HIP
+ * graph and replayed; each run overwrites the outputs.  Runs of one batch must
+ * not overlap (they share the workspace). */
+int fqsl_scott_batch_run(const fqsl_scott_batch *batch, void *stream);
+
+/* Per request (caller's order): milliseconds from the start of the last run
+ * to the moment its last line finished stepping, from the device clock.
+ * Synchronises `stream`. */
+int fqsl_scott_batch_times(const fqsl_scott_batch *batch, double *ms, void *stream);
+
+int fqsl_scott_batch_destroy(fqsl_scott_batch *batch);
+
 /* ------------------------------------------------------------ plain tracer */
 typedef struct {
     int method;      /* FQSL_METHOD_RKF45 (TraceBlineAdaptive.cu) or FQSL_METHOD_RK4 (TraceBline.cu) */

@@ -84,7 +84,8 @@ def test_constants_match_the_header():
     assert knobs == _lib.TUNE_KNOBS and mac["FQSL_TUNE_COUNT"] == len(knobs)
 
 
-STRUCTS = ["fqsl_scott_params", "fqsl_scott_out", "fqsl_plain_params", "fqsl_plain_out"]
+STRUCTS = ["fqsl_scott_params", "fqsl_scott_out", "fqsl_plain_params", "fqsl_plain_out",
+           "fqsl_scott_request"]
 
 
 def test_struct_layouts_match_a_c_compiler(tmp_path):
@@ -130,3 +131,29 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIBPATH", str(tmp_path / "libfqsl.so"))
     with pytest.raises(_lib.FqslError, match="no CPU fallback"):
         _lib.lib()
+
+
+def test_batch_workspace_size_is_host_arithmetic(L):
+    """fqsl_scott_batch_workspace_bytes only groups and counts (no device
+    access): grows by the per-line scratch, and by one counter per
+    (field, method) group; create rejects a request without a field."""
+    R = _lib.fqsl_scott_request
+
+    def table(specs):
+        t = (R * len(specs))()
+        for k, (field, method, n) in enumerate(specs):
+            t[k] = R(field=field, method=method, reserved=0, step=0.25, tol=1e-6, n=n)
+        return t
+
+    fp32, fp64 = _lib.PRECISIONS["fp32c"], _lib.PRECISIONS["fp64"]
+    a = table([(0x1000, 0, 1000), (0x1000, 0, 24)])
+    b = table([(0x1000, 0, 1000), (0x1000, 0, 1024)])
+    w = lambda t, p: L.fqsl_scott_batch_workspace_bytes(t, len(t), p)
+    assert w(a, fp32) > 0 and w(b, fp32) > w(a, fp32)
+    assert w(b, fp32) - w(a, fp32) >= 1000 * (8 * 4 + 96) - 512
+    assert w(b, fp64) > w(b, fp32)
+    assert L.fqsl_scott_batch_workspace_bytes(a, 0, fp32) == 0
+    h = ctypes.c_void_p()
+    bad = table([(None, 0, 10)])
+    assert L.fqsl_scott_batch_create(bad, 1, fp32, None, 0, None, ctypes.byref(h)) != _lib.FQSL_OK
+    assert "NULL field" in L.fqsl_last_error().decode()
