@@ -37,9 +37,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (plane 20, volume 2)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (plane 2, volume 1)")
-    ap.add_argument("--workload", default="plane", choices=["plane", "volume"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "volume", "batch"],
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
-                         "of the 512x256x256 rope-arcade field")
+                         "of the 512x256x256 rope-arcade field; batch: 256 uneven plane requests "
+                         "over 3 resident fields in one graph-captured run")
     ap.add_argument("--pixels", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch knobs instead of autotuning them")
@@ -134,6 +135,159 @@ def error_vs_oracle(out, idx, ref):
                     "tol x (1 + 1e-12) gives class match ~0.999 and ~0.67 within 1e-3"}
 
 
+def bench_batch(args, fq, torch, log):
+    """256 uneven Q-map requests (arbitrary planes of 64^2..1024^2 pixels, RK4
+    or RKF45 at tol 1e-7..1e-5) over three resident fields -- the quadrupole
+    64^3, the random multipole 256^3 (seed 0) and a 320x320x256 multipole
+    (seed 7) -- traced as one batch captured in a HIP graph.  One step = one
+    replay.  Beside it: the same requests traced one call at a time (FP32 and
+    FP64 per-line) and the FP64 oracle on a sample of the lines."""
+    from fastqsl_amd import tune
+    from fastqsl_amd.batch import ScottBatch
+    t0 = time.perf_counter()
+    arrs = [fq.synth.quadrupole_64(dtype=np.float64, device="cuda"),
+            fq.synth.multipole_256(seed=0, device="cuda"),
+            fq.synth.multipole(shape=(320, 320, 256), seed=7, device="cuda")]
+    fields = [fq.Field(*a) for a in arrs]
+    specs = fq.synth.batch_requests([f.shape for f in fields], count=256, seed=0)
+    reqs = []
+    for r in specs:
+        x, y, z = fq.synth.plane_seeds_abc_torch(r["A"], r["B"], r["C"], r["n"], r["n"])
+        reqs.append(dict(field=fields[r["field"]], x=x, y=y, z=z, method=r["method"],
+                         tol=r["tol"] or 1e-6, step=r["step"]))
+    sizes = np.array([r["n"] ** 2 for r in specs])
+    nlines = int(sizes.sum())
+    log(f"batch: {len(reqs)} requests, {nlines} lines, fields ready in "
+        f"{time.perf_counter() - t0:.1f} s")
+    outputs = ("q_0", "q_perp", "flag_start", "flag_end", "evals", "end_x", "end_y", "end_z")
+    tuning = None
+    if not args.no_autotune:
+        t1 = time.perf_counter()
+        cfg = tune.autotune_batch(reqs, precision=args.precision)
+        tuning = {"cfg": cfg, "seconds": time.perf_counter() - t1,
+                  "how": "coordinate descent over the leading requests (<= 2^23 lines), one "
+                         "plan per point"}
+        log(f"autotuned in {tuning['seconds']:.1f} s: {cfg}")
+    b = ScottBatch(reqs, precision=args.precision, outputs=outputs)
+    g = b.graph()
+    for _ in range(args.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    step_ms = []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        g.replay()
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+    done = b.times_ms()
+    ms_step = float(np.mean(step_ms))
+    value = nlines / (ms_step * 1e-3)
+    evals = int(sum(int(r["evals"].to(torch.int64).sum().item()) for r in b.results))
+    log(f"batch {args.precision} graph: {value:.4g} lines/s, {ms_step:.1f} ms per replay, "
+        f"{evals / nlines:.0f} evals/line")
+    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    extra = {}
+
+    def one_by_one(precision, schedule):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for r in reqs:
+            fq.trace_scott(r["field"], r["x"], r["y"], r["z"], method=r["method"], step=r["step"],
+                           tol=r["tol"], precision=precision, schedule=schedule,
+                           outputs=("q_0", "flag_start", "flag_end"))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t1
+
+    if not args.no_fp64_compare:
+        dt = one_by_one("fp32c", "persistent")
+        extra["one_call_per_request_fp32c"] = {"lines_per_s": nlines / dt, "ms": dt * 1e3}
+        dt = one_by_one("fp64", "per_line")
+        extra["fp64_per_line"] = {"lines_per_s": nlines / dt, "ms": dt * 1e3,
+                                  "how": "one FP64 one-thread-per-line launch per request"}
+        extra["speedup_vs_fp64_per_line"] = value / (nlines / dt)
+        log(f"one call per request: fp32c {extra['one_call_per_request_fp32c']['ms']:.0f} ms, "
+            f"fp64 per-line {dt * 1e3:.0f} ms")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        rng = np.random.default_rng(1)
+        sample = min(args.cpu_sample, 1 << 14)
+        pick = np.sort(rng.choice(nlines, size=sample, replace=False))
+        starts = np.concatenate([[0], np.cumsum(sizes)])
+        which = np.searchsorted(starts, pick, side="right") - 1
+        fnp = [tuple(np.asarray(c, np.float64) for c in a) for a in arrs]
+        nt = O.num_threads()
+        t_cpu = 0.0
+        cls_ok, dq = [], []
+        for k in np.unique(which):
+            li = pick[which == k] - starts[k]
+            r, sp = reqs[k], specs[k]
+            seeds = torch.stack([r["x"], r["y"], r["z"]], 1)[torch.as_tensor(li, device="cuda")].cpu().numpy()
+            t1 = time.perf_counter()
+            ref = O.trace_scott(fnp[sp["field"]], seeds, method=r["method"], s0=r["step"],
+                                tol=r["tol"], nthreads=nt)
+            t_cpu += time.perf_counter() - t1
+            res = b.results[k]
+            gq = res["q_0"][torch.as_tensor(li, device="cuda")].cpu().numpy().astype(np.float64)
+            fs = res["flag_start"][torch.as_tensor(li, device="cuda")].cpu().numpy()
+            fe = res["flag_end"][torch.as_tensor(li, device="cuda")].cpu().numpy()
+            c = (fs == ref["flag_start"]) & (fe == ref["flag_end"])
+            d = np.full(len(li), np.inf)
+            ok = np.isfinite(gq) & np.isfinite(ref["q0"]) & (gq > 0) & (ref["q0"] > 0)
+            d[ok] = np.abs(np.log10(gq[ok]) - np.log10(ref["q0"][ok]))
+            d[(np.isnan(gq) & np.isnan(ref["q0"])) | ((gq == 0) & (ref["q0"] == 0))] = 0.0
+            cls_ok.append(c)
+            dq.append(np.where(c, d, np.inf))
+        cls_ok = np.concatenate(cls_ok)
+        dq = np.concatenate(dq)
+        fin = dq[np.isfinite(dq)]
+        cpu = {"value": sample / t_cpu, "unit": "lines/s", "cores": nt, "kind": "port",
+               "sample": f"{sample} lines drawn uniformly (default_rng(1)) from the batch, each "
+                         "with its request's method/step/tolerance, FP64 C oracle",
+               "cpu_model": cpu_model()}
+        extra["error_vs_oracle"] = {"lines": sample, "class_match": float(cls_ok.mean()),
+                                    "frac_dlog10q_le_1e-3": float(np.mean(dq <= 1e-3)),
+                                    "dlog10q_median": float(np.median(fin)),
+                                    "dlog10q_p999": float(np.quantile(fin, 0.999))}
+        log(f"cpu baseline: {sample} lines in {t_cpu:.2f} s on {nt} threads")
+    nz = sizes > 0
+    rec = {
+        "metric": "Scott-Q field lines/s",
+        "value": value,
+        "unit": "lines/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "latency_ms": {"p50": float(np.percentile(done[nz], 50)),
+                       "p99": float(np.percentile(done[nz], 99)),
+                       "what": "per request, from the start of the batch run to its last line "
+                               "finishing (device clock), last replay"},
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32c" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "256 uneven requests (planes 64^2..1024^2 px, RK4/RKF45 tol "
+                               "1e-7..1e-5) over quadrupole 64^3, multipole 256^3 (seed 0), "
+                               "multipole 320x320x256 (seed 7); one graph-captured batch",
+                   "lines_per_step": nlines, "precision": args.precision,
+                   "schedule": "batch", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": evals * bpe / (ms_step * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": evals * bpe / (ms_step * 1e-3) / HBM_PEAK, "traffic": None,
+                     "evals_per_launch": evals, "bytes_per_eval": bpe,
+                     "note": "time per replay (host-synchronised), all groups"},
+        "cpu_baseline": cpu,
+        "tuning": tuning,
+        **extra,
+    }
+    print(json.dumps(rec), flush=True)
+    b.close()
+    for f in fields:
+        f.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -149,6 +303,12 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
 
+    if args.workload == "batch":
+        if args.steps is None:
+            args.steps = 3
+        if args.warmup is None:
+            args.warmup = 1
+        return bench_batch(args, fq, torch, log)
     volume = args.workload == "volume"
     if args.steps is None:
         args.steps = 2 if volume else 20

@@ -153,3 +153,46 @@ def autotune(field, x, y, z, method="rkf45", step=0.25, tol=1e-6, precision="fp3
                      "points": len(seen)}
         save_cache(data)
     return best
+
+
+BATCH_SPACE = {
+    "scott_waves": (2, 0),
+    "refill_min": (8, 4, 16, 32),
+    "grid_pct": (100, 75),
+}
+
+
+def autotune_batch(requests, precision="fp32c", outputs=("q_0", "flag_start", "flag_end"),
+                   max_lines=1 << 23, log=None):
+    """Pick the knobs for a batch of requests (``batch.ScottBatch``): the
+    stepping kernel's waves per SIMD and refill chunk are fixed when a batch is
+    planned, the grid size when it runs.  Measures on the leading requests up
+    to `max_lines` lines (each point plans a fresh batch and times its second
+    run), applies the best configuration and returns it."""
+    from .batch import ScottBatch
+    sub, lines = [], 0
+    for r in requests:
+        n = int(r["x"].numel()) if hasattr(r["x"], "numel") else len(r["x"])
+        if sub and lines + n > max_lines:
+            break
+        sub.append(r)
+        lines += n
+
+    def measure(cfg):
+        apply(cfg)
+        with ScottBatch(sub, precision=precision, outputs=outputs) as b:
+            b.run()
+            _lib.last_kernel_ms()
+            b.run()
+            return _lib.last_kernel_ms()
+
+    before = current()
+    try:
+        best, best_ms, _ = coordinate_descent(measure, space=BATCH_SPACE,
+                                              start={k: DEFAULTS[k] for k in BATCH_SPACE}, rounds=1,
+                                              log=log)
+    except Exception:
+        apply(before)
+        raise
+    apply(best)
+    return best
