@@ -37,10 +37,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (plane 20, volume 2)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (plane 2, volume 1)")
-    ap.add_argument("--workload", default="plane", choices=["plane", "volume", "batch"],
+    ap.add_argument("--rates", default="5,20,50,100,200",
+                    help="serve: Poisson arrival rates (requests/s) to sweep")
+    ap.add_argument("--duration", type=float, default=2.0,
+                    help="serve: seconds of arrivals per rate")
+    ap.add_argument("--workload", default="plane", choices=["plane", "volume", "batch", "serve"],
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
                          "of the 512x256x256 rope-arcade field; batch: 256 uneven plane requests "
-                         "over 3 resident fields in one graph-captured run")
+                         "over 3 resident fields in one graph-captured run; serve: Poisson "
+                         "arrivals of 128^2..2048^2 maps over 16 resident snapshots")
     ap.add_argument("--pixels", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch knobs instead of autotuning them")
@@ -288,6 +293,72 @@ def bench_batch(args, fq, torch, log):
         f.close()
 
 
+def bench_serve(args, fq, torch, log):
+    """Serving: Poisson arrivals of Q-map requests (z = 0 windows of 128^2..
+    2048^2 pixels, RKF45 tol 1e-6) against 16 resident snapshots of the
+    random-multipole field at 320x320x256 (each with a 1-5 % Fourier-mode
+    potential perturbation); continuous batching on two streams with HBM
+    admission, FP32-compensated.  Beside it: FP64 one-request-at-a-time with
+    the one-thread-per-line kernel (the plain baseline) at the low rates."""
+    from fastqsl_amd import serve
+    t0 = time.perf_counter()
+    shape = (320, 320, 256)
+    fields = [fq.Field(*fq.synth.snapshot(i, shape=shape, device="cuda")) for i in range(16)]
+    hbm = sum(f.nbytes for f in fields)
+    log(f"16 snapshots resident ({hbm / 1e9:.1f} GB) in {time.perf_counter() - t0:.1f} s")
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = []
+    # warm up the kernels and the allocator
+    serve.QServer(fields).run(serve.poisson_requests(100, 0.05, 16, shape, seed=99, min_count=4))
+    for rate in rates:
+        reqs = serve.poisson_requests(rate, args.duration, 16, shape, seed=int(rate))
+        st = serve.QServer(fields, precision=args.precision, streams=2).run(reqs)
+        row = {"rate": rate, "requests": st["requests"], "lines": st["lines"],
+               "lines_per_s": st["lines_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+               "batches": st["batches"], "mean_batch": st["mean_batch"]}
+        sweep.append(row)
+        log(f"rate {rate:g}/s: {st['requests']} req, {st['lines_per_s']:.3g} lines/s, "
+            f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, {st['batches']} batches")
+    base = []
+    if not args.no_fp64_compare:
+        for rate in [r for r in rates if r <= 20]:
+            reqs = serve.poisson_requests(rate, args.duration, 16, shape, seed=int(rate))
+            st = serve.QServer(fields, precision="fp64", mode="serial", schedule="per_line").run(reqs)
+            base.append({"rate": rate, "requests": st["requests"], "lines_per_s": st["lines_per_s"],
+                         "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"]})
+            log(f"fp64 plain serial, rate {rate:g}/s: {st['lines_per_s']:.3g} lines/s, "
+                f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms")
+    top = sweep[-1]
+    rec = {
+        "metric": "Scott-Q field lines/s",
+        "value": top["lines_per_s"],
+        "unit": "lines/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 1,
+        "ms_per_step": args.duration * 1e3,
+        "latency_ms": {"p50": top["p50_ms"], "p99": top["p99_ms"],
+                       "what": f"per request, arrival to results resident, at {top['rate']:g} req/s"},
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32c" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "Poisson arrivals of z=0 Q maps (128^2..2048^2 px, RKF45 tol 1e-6) "
+                               "over 16 resident 320x320x256 snapshots of the random multipole "
+                               "(1-5 % Fourier-mode perturbations); continuous batching, 2 streams",
+                   "precision": args.precision, "schedule": "serve", "parallelism": "single GPU",
+                   "duration_s_per_rate": args.duration},
+        "roofline": None,
+        "cpu_baseline": None,
+        "sweep": sweep,
+        "fp64_plain_serial": base,
+    }
+    print(json.dumps(rec), flush=True)
+    for f in fields:
+        f.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -303,6 +374,8 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
 
+    if args.workload == "serve":
+        return bench_serve(args, fq, torch, log)
     if args.workload == "batch":
         if args.steps is None:
             args.steps = 3

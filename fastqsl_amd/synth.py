@@ -226,3 +226,56 @@ def plane_seeds_abc_torch(A, B, C, n_u, n_v, device="cuda"):
     P = A + (B - A) * U[..., None] + (C - A) * V[..., None]
     P = P.reshape(-1, 3).to(torch.float32)
     return P[:, 0].contiguous(), P[:, 1].contiguous(), P[:, 2].contiguous()
+
+
+def fourier_perturbation(shape, seed, box=((-1, 1), (-1, 1), (0, 2)), modes=8, kmax=4,
+                         device=None):
+    """dB = -grad(dPhi) of a harmonic potential perturbation
+    dPhi = sum_m a_m cos(kx_m x + ky_m y + phi_m) exp(-k_m z), k_m = |(kx_m, ky_m)|
+    (each mode solves Laplace's equation, so dB is itself a potential field).
+    Draw order (default_rng(seed)): per mode, kx and ky as pi * integers in
+    [-kmax, kmax] (not both zero), a ~ N(0,1), phi ~ U[0, 2 pi).  Returns
+    unnormalised (dBx, dBy, dBz) torch float64 tensors of shape (nz, ny, nx)."""
+    torch = _torch()
+    device = _default_device(device)
+    rng = np.random.default_rng(seed)
+    xs, ys, zs = _axes(shape, box, device, torch)
+    Z, Y, X = torch.meshgrid(zs, ys, xs, indexing="ij")
+    dbx = torch.zeros_like(X)
+    dby = torch.zeros_like(X)
+    dbz = torch.zeros_like(X)
+    for _ in range(modes):
+        while True:
+            ix, iy = rng.integers(-kmax, kmax + 1, 2)
+            if ix or iy:
+                break
+        a = rng.normal()
+        phi = rng.uniform(0, 2 * math.pi)
+        kx, ky = math.pi * ix, math.pi * iy
+        k = math.hypot(kx, ky)
+        arg = kx * X + ky * Y + phi
+        decay = torch.exp(-k * Z)
+        s = a * torch.sin(arg) * decay
+        dbx += kx * s
+        dby += ky * s
+        dbz += a * k * torch.cos(arg) * decay
+    return dbx, dby, dbz
+
+
+def snapshot(index, shape=(320, 320, 256), base_seed=0, dtype=np.float32, device=None):
+    """Snapshot `index` of a field time series: the random multipole of
+    `base_seed` plus a Fourier-mode potential perturbation whose RMS |dB| on
+    the z = 0 plane is eps times that of the base field, eps ~ U[0.01, 0.05]
+    drawn first from default_rng(1000 + index) (the modes then from
+    default_rng(index))."""
+    torch = _torch()
+    device = _default_device(device)
+    eps = float(np.random.default_rng(1000 + index).uniform(0.01, 0.05))
+    q, pos = multipole_sources(base_seed, 40)
+    base = point_sources(shape, q, pos, dtype=np.float64, device=device)
+    bt = [torch.as_tensor(np.ascontiguousarray(b.transpose(2, 1, 0)), device=device) for b in base]
+    dbx, dby, dbz = fourier_perturbation(shape, index, device=device)
+    rms = lambda t: torch.sqrt(torch.mean(t[0] ** 2 + t[1] ** 2 + t[2] ** 2))
+    scale = eps * rms([b[0] for b in bt]) / rms([dbx[0], dby[0], dbz[0]])
+    out = [b + scale * d for b, d in zip(bt, (dbx, dby, dbz))]
+    return _to_numpy(*out, dtype)
