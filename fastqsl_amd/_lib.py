@@ -111,7 +111,8 @@ def check(rc: int) -> None:
         raise FqslError(f"libfqsl error {rc}: {msg}")
 
 
-TUNE_KNOBS = {"scott_waves": 0, "grid_pct": 1, "queues": 2, "refill_min": 3}
+TUNE_KNOBS = {"scott_waves": 0, "grid_pct": 1, "queues": 2, "refill_min": 3, "order": 4,
+              "order_buckets": 5}
 
 
 def tune_set(knob: str, value: int) -> None:

@@ -204,7 +204,7 @@ extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) 
 }
 
 // ------------------------------------------------------------- tuning knobs
-static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8};
+static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
@@ -214,6 +214,10 @@ extern "C" int fqsl_tune_set(int knob, int value) {
         return set_err(FQSL_ERR_INVALID, "queue count %d not in {1, 8}", value);
     if (knob == FQSL_TUNE_REFILL_MIN && (value < 1 || value > 64))
         return set_err(FQSL_ERR_INVALID, "refill group %d not in 1..64", value);
+    if (knob == FQSL_TUNE_ORDER && (value < 0 || value > 2))
+        return set_err(FQSL_ERR_INVALID, "order knob must be 0, 1 or 2 (got %d)", value);
+    if (knob == FQSL_TUNE_ORDER_BUCKETS && (value < 2 || value > 256 || (value & (value - 1))))
+        return set_err(FQSL_ERR_INVALID, "order buckets must be a power of two in [2, 256] (got %d)", value);
     if (knob == FQSL_TUNE_GRID_PCT && (value < 1 || value > 1000))
         return set_err(FQSL_ERR_INVALID, "grid percentage %d not in 1..1000", value);
     g_tune[knob] = value;
@@ -393,6 +397,132 @@ __device__ __forceinline__ bool claim(unsigned long long* queues, int nq, unsign
     return false;
 }
 
+// ------------------------------------------------------ longest-first order
+// Lines take 10 to 10^4 steps.  A persistent launch that hands them out in
+// seed order ends with a tail of long lines running in nearly empty
+// wavefronts (about a third of the launch on the 1024^2 multipole map).  A
+// coarse estimate of each line's length -- a plain RK4 trace of b with 4-cell
+// steps, at most kPredSteps per direction -- sorts the queue longest first
+// (counting sort into 2..256 length buckets; within a bucket lines keep
+// roughly their seed order, so fewer buckets keep more of the field locality
+// of neighbouring seeds), which removes most of that tail for ~3 % more work.
+// With many lines per lane the tail is short anyway and the lost locality
+// costs more than it saves, so by default lines are only ordered up to 64
+// lines per persistent lane.  The order only decides which lane traces a line
+// and when; the line's result is the same.
+constexpr int kPredSteps = 32;
+constexpr float kPredStep = 4.0f;
+constexpr int kOrderBuckets = 256;
+
+template <typename R>
+__device__ __forceinline__ Pt<R> pt_from_real(const R* x) {
+    Pt<R> p;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const R fl = flr(x[a]);
+        p.i[a] = to_int(fl);
+        p.r[a] = x[a] - fl;
+    }
+    return p;
+}
+
+// Length (cells) a coarse RK4 trace of +-b from p0 covers inside the box,
+// capped at 2 * kPredSteps * kPredStep.
+template <typename R, class F>
+__device__ float predict_length(const F& f, const float* p0) {
+    const R h = (R)kPredStep;
+    float total = 0.f;
+    for (int dir = -1; dir <= 1; dir += 2) {
+        R x[3] = {(R)p0[0], (R)p0[1], (R)p0[2]};
+        const R hs = h * (R)dir;
+        for (int st = 0; st < kPredSteps; ++st) {
+            R k1[3], k2[3], k3[3], k4[3], y[3];
+            unit_b<R>(f, pt_from_real<R>(x), k1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y[a] = x[a] + (R)0.5 * hs * k1[a];
+            unit_b<R>(f, pt_from_real<R>(y), k2);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y[a] = x[a] + (R)0.5 * hs * k2[a];
+            unit_b<R>(f, pt_from_real<R>(y), k3);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y[a] = x[a] + hs * k3[a];
+            unit_b<R>(f, pt_from_real<R>(y), k4);
+            bool inside = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                x[a] += hs / (R)6 * (k1[a] + (R)2 * k2[a] + (R)2 * k3[a] + k4[a]);
+                inside = inside && x[a] >= (R)0 && x[a] <= (R)(f.n[a] - 1);  // false for NaN
+            }
+            total += kPredStep;
+            if (!inside) break;
+        }
+    }
+    return total;
+}
+
+__global__ void k_zero_u32(unsigned* p, int count) {
+    for (int t = threadIdx.x; t < count; t += blockDim.x) p[t] = 0u;
+}
+
+// Bucket key per line (0 = longest) and the bucket counts.
+template <typename R, class F>
+__global__ void __launch_bounds__(256) k_order_predict(F f, Seeds s, unsigned long long n, int nbuckets,
+                                                       unsigned char* __restrict__ key, unsigned* counts) {
+    __shared__ unsigned hist[kOrderBuckets];
+    for (int t = threadIdx.x; t < kOrderBuckets; t += blockDim.x) hist[t] = 0u;
+    __syncthreads();
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float p0[3] = {s.x[i], s.y[i], s.z[i]};
+        const float len = predict_length<R, F>(f, p0);
+        int b = (int)(len * ((float)nbuckets / (2.f * kPredSteps * kPredStep)));
+        b = b < 0 ? 0 : (b > nbuckets - 1 ? nbuckets - 1 : b);
+        const int kb = nbuckets - 1 - b;  // bucket 0 holds the longest lines
+        key[i] = (unsigned char)kb;
+        atomicAdd(&hist[kb], 1u);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kOrderBuckets; t += blockDim.x)
+        if (hist[t]) atomicAdd(&counts[t], hist[t]);
+}
+
+// Exclusive scan of the bucket counts (one block).
+__global__ void __launch_bounds__(kOrderBuckets) k_order_scan(unsigned* counts) {
+    __shared__ unsigned v[kOrderBuckets];
+    v[threadIdx.x] = counts[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned acc = 0;
+        for (int t = 0; t < kOrderBuckets; ++t) {
+            const unsigned c = v[t];
+            v[t] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    counts[threadIdx.x] = v[threadIdx.x];
+}
+
+// order[offset(bucket) + rank] = line, 256 lines per block step: one global
+// atomic per (block step, bucket) reserves the block's range in a bucket.
+__global__ void __launch_bounds__(256) k_order_scatter(const unsigned char* __restrict__ key, unsigned long long n,
+                                                       unsigned* offs, unsigned* __restrict__ order) {
+    __shared__ unsigned hist[kOrderBuckets], base[kOrderBuckets];
+    for (unsigned long long c0 = blockIdx.x * 256ull; c0 < n; c0 += gridDim.x * 256ull) {
+        hist[threadIdx.x] = 0u;
+        __syncthreads();
+        const unsigned long long i = c0 + threadIdx.x;
+        const bool valid = i < n;
+        const int kb = valid ? (int)key[i] : 0;
+        const unsigned rank = valid ? atomicAdd(&hist[kb], 1u) : 0u;
+        __syncthreads();
+        if (hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&offs[threadIdx.x], hist[threadIdx.x]);
+        __syncthreads();
+        if (valid) order[base[kb] + rank] = (unsigned)i;
+        __syncthreads();
+    }
+}
+
 // Seed preparation, one thread per seed: frame (u0, v0, b0z, |B0|) for the
 // stepping kernel and B at the seed (Interp3d rule) straight to the output.
 template <typename R, class F>
@@ -426,7 +556,7 @@ template <typename R, class F>
 __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                       const R* __restrict__ frames, ScottRec<R>* __restrict__ recs,
                                                       DevScalars ds, unsigned long long* queues, int nq,
-                                                      int refill_min) {
+                                                      int refill_min, const unsigned* __restrict__ order) {
     apply_scalars(f, P, n, ds);
     ScottLine<R> L;
     bool active = false;
@@ -447,6 +577,7 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
                 const int rank = __popcll(need & below);
                 if (!active && rank < avail) {
                     idx = base + (unsigned long long)rank;
+                    if (order) idx = order[idx];
                     ScottEnd<R> e;
                     const int st = scott_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx], e,
                                                  frames + (unsigned long long)kFrameWords * idx);
@@ -479,15 +610,16 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
     __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                \
     k_scott_persistent_w##W(F f, ScottParams P, Seeds s, unsigned long long n, const R* frames,       \
                             ScottRec<R>* recs, DevScalars ds, unsigned long long* queues, int nq,     \
-                            int refill_min) {                                                         \
-        scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min);            \
+                            int refill_min, const unsigned* order) {                                  \
+        scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order);     \
     }
 
 template <typename R, class F>
 __global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Seeds s, unsigned long long n,
                                                           const R* frames, ScottRec<R>* recs, DevScalars ds,
-                                                          unsigned long long* queues, int nq, int refill_min) {
-    scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min);
+                                                          unsigned long long* queues, int nq, int refill_min,
+                                                          const unsigned* order) {
+    scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order);
 }
 FQSL_PERSISTENT_KERNEL(2)
 FQSL_PERSISTENT_KERNEL(3)
@@ -690,18 +822,36 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         if (grid < 1) grid = 1;
         const int nq = g_tune[FQSL_TUNE_QUEUES];
         const int refill_min = g_tune[FQSL_TUNE_REFILL_MIN];
+        const int om = g_tune[FQSL_TUNE_ORDER];
+        const bool ordered = !ds.shape && n_hint < (1ull << 32) &&
+                             (om == 2 || (om == 1 && n_hint <= 64ull * 256ull * (unsigned long long)grid));
         const size_t frame_bytes = (size_t)kFrameWords * sizeof(R) * n_hint;
         const size_t rec_off = (frame_bytes + 255) & ~(size_t)255;
+        const size_t order_off = (rec_off + sizeof(ScottRec<R>) * n_hint + 255) & ~(size_t)255;
+        const size_t key_off = order_off + (ordered ? ((sizeof(unsigned) * n_hint + 255) & ~(size_t)255) : 0);
+        const size_t cnt_off = key_off + (ordered ? ((n_hint + 255) & ~(size_t)255) : 0);
+        const size_t ws_bytes = cnt_off + (ordered ? sizeof(unsigned) * kOrderBuckets : 0);
         void* ws = nullptr;
-        HIP_TRY(hipMallocAsync(&ws, rec_off + sizeof(ScottRec<R>) * n_hint, st));
+        HIP_TRY(hipMallocAsync(&ws, ws_bytes, st));
         R* frames = (R*)ws;
         ScottRec<R>* recs = (ScottRec<R>*)((char*)ws + rec_off);
+        unsigned* order = ordered ? (unsigned*)((char*)ws + order_off) : nullptr;
+        unsigned char* key = ordered ? (unsigned char*)ws + key_off : nullptr;
+        unsigned* counts = ordered ? (unsigned*)((char*)ws + cnt_off) : nullptr;
         k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) rc = record_start(st);
         if (e == hipSuccess && !rc) {
+            if (ordered) {
+                k_zero_u32<<<1, 256, 0, st>>>(counts, kOrderBuckets);
+                k_order_predict<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(
+                    f, s, n, g_tune[FQSL_TUNE_ORDER_BUCKETS], key, counts);
+                k_order_scan<<<1, kOrderBuckets, 0, st>>>(counts);
+                const int sg = grid_for((long)n_hint, 256) < 4096 ? grid_for((long)n_hint, 256) : 4096;
+                k_order_scatter<<<sg, 256, 0, st>>>(key, n, counts, order);
+            }
             k_scott_prep<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, frames, o, ds);
-            kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min);
+            kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min, order);
             k_scott_finish<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, n, frames, recs, o, ds);
             e = hipGetLastError();
             if (e == hipSuccess) rc = record_end(st);

@@ -10,7 +10,11 @@ picked per workload by measurement:
                    (the queue chunk: one atomic per chunk);
 * ``queues``       one seed queue, or one per XCD (8), the launch's blocks
                    spread over them;
-* ``grid_pct``     persistent grid size in % of one full occupancy wave.
+* ``grid_pct``     persistent grid size in % of one full occupancy wave;
+* ``order``        queue order: 2 = longest line first (by a coarse length
+                   estimate), 0 = seed order, 1 = longest first only up to 64
+                   lines per persistent lane;
+* ``order_buckets``length classes of that order (fewer keep more locality).
 
 ``autotune`` runs coordinate descent over these on the actual request, or on
 a contiguous slice of it when it is larger than ``sample`` lines (contiguous,
@@ -31,12 +35,15 @@ import numpy as np
 from . import _lib
 
 SPACE = {
+    "order": (1, 2, 0),
+    "order_buckets": (64, 16, 256),
     "scott_waves": (2, 0),
     "refill_min": (8, 4, 16, 32, 1),
     "queues": (1, 8),
     "grid_pct": (100, 75, 50),
 }
-DEFAULTS = {"scott_waves": 2, "refill_min": 8, "queues": 1, "grid_pct": 100}
+DEFAULTS = {"order": 1, "order_buckets": 64, "scott_waves": 2, "refill_min": 8, "queues": 1,
+            "grid_pct": 100}
 
 
 def cache_path() -> str:
@@ -156,6 +163,8 @@ def autotune(field, x, y, z, method="rkf45", step=0.25, tol=1e-6, precision="fp3
 
 
 BATCH_SPACE = {
+    "order": (1, 2, 0),
+    "order_buckets": (64, 16, 256),
     "scott_waves": (2, 0),
     "refill_min": (8, 4, 16, 32),
     "grid_pct": (100, 75),

@@ -250,7 +250,11 @@ int fqsl_TraceAllBline(int method, float *Bx, float *By, float *Bz, int *BshapeN
 #define FQSL_TUNE_GRID_PCT 1    /* persistent grid size in % of one full occupancy wave (default 100) */
 #define FQSL_TUNE_QUEUES 2      /* seed queues of the persistent Scott kernel: 1 (default) or 8 (one per XCD) */
 #define FQSL_TUNE_REFILL_MIN 3  /* idle lanes a wavefront gathers before it refills them (1..64, default 8) */
-#define FQSL_TUNE_COUNT 4
+#define FQSL_TUNE_ORDER 4       /* persistent Scott queue order: 0 = seed order, 2 = longest first by a
+                                   coarse length estimate, 1 (default) = longest first up to 64 lines per
+                                   persistent lane (ordering changes timing, never results) */
+#define FQSL_TUNE_ORDER_BUCKETS 5 /* length buckets of that order: power of two in [2, 256], default 64 */
+#define FQSL_TUNE_COUNT 6
 int fqsl_tune_set(int knob, int value);
 int fqsl_tune_get(int knob, int *value);
 

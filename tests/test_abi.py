@@ -120,6 +120,7 @@ def test_host_only_calls(L):
     assert _lib.tune_get("grid_pct") == 100
     assert _lib.tune_get("queues") == 1
     assert _lib.tune_get("refill_min") == 8
+    assert _lib.tune_get("order") == 1 and _lib.tune_get("order_buckets") == 64
     assert L.fqsl_tune_set(99, 1) != _lib.FQSL_OK
     assert L.fqsl_last_error().decode()
     v = ctypes.c_int(-1)

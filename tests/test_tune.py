@@ -6,7 +6,8 @@ from fastqsl_amd import _lib, tune
 
 
 def test_coordinate_descent_finds_the_separable_optimum():
-    target = {"scott_waves": 0, "refill_min": 16, "queues": 8, "grid_pct": 75}
+    target = {"order": 2, "order_buckets": 16, "scott_waves": 0, "refill_min": 16, "queues": 8,
+              "grid_pct": 75}
     cost = lambda c: 10 + sum(abs(hash((k, c[k])) % 7) * (c[k] != target[k]) + (c[k] != target[k])
                               for k in c)
     best, ms, seen = tune.coordinate_descent(cost)
