@@ -449,12 +449,15 @@ def main():
         run(args.precision, args.schedule)
     torch.cuda.synchronize()
     kernel_ms = []
+    trace_ms = []
     step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
         run(args.precision, args.schedule)
-        kernel_ms.append(fq.last_kernel_ms())        # waits for the map's end event
+        trace_ms.append(fq.last_kernel_ms())         # waits for the map's end event
+        # the stepping kernel alone (persistent schedule); the whole trace otherwise
+        kernel_ms.append(fq._lib.last_step_ms() if args.schedule == "persistent" else trace_ms[-1])
         step_ms.append((time.perf_counter() - ts) * 1e3)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -467,7 +470,7 @@ def main():
     bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
     achieved = evals * bpe / (kms * 1e-3)
     log(f"{args.precision}/{args.schedule}: {value:.4g} lines/s, {ms_step:.2f} ms/step, "
-        f"kernel {kms:.2f} ms, {evals / nlines:.1f} evals/line")
+        f"trace {np.mean(trace_ms):.2f} ms, stepping kernel {kms:.2f} ms, {evals / nlines:.1f} evals/line")
 
     extra = {}
     if (volume or args.per_line_fp32) and (args.precision, args.schedule) == ("fp32c", "persistent"):
@@ -535,7 +538,10 @@ def main():
                                  and args.pixels == 1024 else None),
                      "traffic_source": "profiles/scott_rkf45_multipole256_1024sq.md (rocprofv3 "
                                        "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
-                     "kernel_ms": kms, "evals_per_launch": evals, "bytes_per_eval": bpe},
+                     "kernel": ("stepping kernel k_scott_persistent_w2" if args.schedule == "persistent"
+                                else "k_scott_per_line"),
+                     "kernel_ms": kms, "trace_ms": float(np.mean(trace_ms)),
+                     "evals_per_launch": evals, "bytes_per_eval": bpe},
         "cpu_baseline": cpu,
         "tuning": tuning,
         "classes": {str(int(c)): int(k) for c, k in zip(classes, counts)},

@@ -84,6 +84,7 @@ PROTOTYPES = {
     "fqsl_scott_batch_times": (i32, [vp, ctypes.POINTER(ctypes.c_double), vp]),
     "fqsl_scott_batch_destroy": (i32, [vp]),
     "fqsl_last_kernel_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
+    "fqsl_last_step_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
     "fqsl_tune_set": (i32, [i32, i32]),
     "fqsl_tune_get": (i32, [i32, ctypes.POINTER(i32)]),
 }
@@ -128,4 +129,11 @@ def tune_get(knob: str) -> int:
 def last_kernel_ms() -> float:
     v = ctypes.c_float()
     check(lib().fqsl_last_kernel_ms(ctypes.byref(v)))
+    return float(v.value)
+
+
+def last_step_ms() -> float:
+    """Stepping-kernel time of the last persistent Scott trace (HIP events)."""
+    v = ctypes.c_float()
+    check(lib().fqsl_last_step_ms(ctypes.byref(v)))
     return float(v.value)

@@ -27,8 +27,10 @@ using namespace fqsl;
 
 // ------------------------------------------------------------ error state
 static thread_local char g_err[512] = "";
-static thread_local hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+static thread_local hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;  // whole trace
+static thread_local hipEvent_t g_evs0 = nullptr, g_evs1 = nullptr;  // stepping kernel only
 static thread_local bool g_ev_valid = false;
+static thread_local bool g_evs_valid = false;
 
 static int set_err(int code, const char* fmt, ...) {
     va_list ap;
@@ -775,13 +777,24 @@ static int record_start(hipStream_t st) {
     if (!g_ev0) {
         HIP_TRY(hipEventCreate(&g_ev0));
         HIP_TRY(hipEventCreate(&g_ev1));
+        HIP_TRY(hipEventCreate(&g_evs0));
+        HIP_TRY(hipEventCreate(&g_evs1));
     }
+    g_evs_valid = false;
     HIP_TRY(hipEventRecord(g_ev0, st));
     return FQSL_OK;
 }
 static int record_end(hipStream_t st) {
     HIP_TRY(hipEventRecord(g_ev1, st));
     g_ev_valid = true;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_last_step_ms(float* ms) {
+    if (!ms) return set_err(FQSL_ERR_INVALID, "ms is NULL");
+    if (!g_evs_valid) return set_err(FQSL_ERR_INVALID, "the last trace on this thread had no separate stepping kernel");
+    HIP_TRY(hipEventSynchronize(g_evs1));
+    HIP_TRY(hipEventElapsedTime(ms, g_evs0, g_evs1));
     return FQSL_OK;
 }
 
@@ -851,7 +864,10 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
                 k_order_scatter<<<sg, 256, 0, st>>>(key, n, counts, order);
             }
             k_scott_prep<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, frames, o, ds);
+            e = hipEventRecord(g_evs0, st);
             kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min, order);
+            if (e == hipSuccess) e = hipEventRecord(g_evs1, st);
+            if (e == hipSuccess) g_evs_valid = true;
             k_scott_finish<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, n, frames, recs, o, ds);
             e = hipGetLastError();
             if (e == hipSuccess) rc = record_end(st);

@@ -264,6 +264,12 @@ int fqsl_tune_get(int knob, int *value);
  * events recorded around it on its stream (synchronises on the end event). */
 int fqsl_last_kernel_ms(float *ms);
 
+/* Duration in milliseconds of the stepping kernel alone of the most recent
+ * persistent fqsl_trace_scott on this thread (HIP events around that launch;
+ * synchronises on its end event).  FQSL_ERR_INVALID if that trace had no
+ * separate stepping kernel (per-line schedule). */
+int fqsl_last_step_ms(float *ms);
+
 #ifdef __cplusplus
 }
 #endif
