@@ -1,7 +1,7 @@
 // fqsl_kernels.hip -- gfx950 kernels and the C ABI of libfqsl.so.
 //
 // Kernels:
-//   k_build_cells          SoA float field -> 128-byte cell stencils
+//   k_build_cells          SoA float field -> 128-byte cells of trilinear coefficients
 //   k_scott_persistent     Scott tracer, persistent grid, per-lane refill
 //                          from a device work queue (one line per lane)
 //   k_scott_per_line       Scott tracer, one thread per line
@@ -83,11 +83,12 @@ __global__ void k_build_cells(const float* __restrict__ bx, const float* __restr
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const float* b = comp[q];
-            float v[8];
+            float v[8], c8[8];
 #pragma unroll
             for (int c = 0; c < 8; ++c) v[c] = b[base + (c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy];
-            o[2 * q] = make_float4(v[0], v[1], v[2], v[3]);
-            o[2 * q + 1] = make_float4(v[4], v[5], v[6], v[7]);
+            cell_coefs(v, c8);
+            o[2 * q] = make_float4(c8[0], c8[1], c8[2], c8[3]);
+            o[2 * q + 1] = make_float4(c8[4], c8[5], c8[6], c8[7]);
         }
         o[6] = make_float4(0.f, 0.f, 0.f, 0.f);
         o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -560,6 +561,7 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
                                                       DevScalars ds, unsigned long long* queues, int nq,
                                                       int refill_min, const unsigned* __restrict__ order) {
     apply_scalars(f, P, n, ds);
+    P.ctl = scott_ctl(P.tol);  // on the device, so every path forms the same bits
     ScottLine<R> L;
     bool active = false;
     bool drained = false;
@@ -656,6 +658,7 @@ template <typename R, class F>
 __device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                     ScottOut out, DevScalars ds) {
     apply_scalars(f, P, n, ds);
+    P.ctl = scott_ctl(P.tol);  // on the device, so every path forms the same bits
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
         ScottLine<R> L;
@@ -937,7 +940,7 @@ extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params
     if (!x || !y || !z) return set_err(FQSL_ERR_INVALID, "NULL seed array");
     if (!(p->step > 0) || !(p->step <= 100.0)) return set_err(FQSL_ERR_INVALID, "step %g outside (0, 100]", p->step);
     if (p->method == FQSL_METHOD_RKF45 && !(p->tol > 0)) return set_err(FQSL_ERR_INVALID, "tol %g must be > 0", p->tol);
-    ScottParams P{p->method, p->step, p->tol};
+    ScottParams P{p->method, p->step, p->tol, 0.0};  // ctl is formed on the device
     Seeds s{x, y, z};
     ScottOut o{out->start_x, out->start_y, out->start_z, out->end_x, out->end_y, out->end_z,
                out->flag_start, out->flag_end, out->b_this_x, out->b_this_y, out->b_this_z,
@@ -1053,7 +1056,7 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
                                                  ScottRec<R>* __restrict__ recs, BatchCtl c, int qi,
                                                  int refill_min) {
     ScottLine<R> L;
-    ScottParams P{method, 0.0, 0.0};
+    ScottParams P{method, 0.0, 0.0, 0.0};
     bool active = false;
     bool drained = false;
     unsigned long long idx = 0;
@@ -1076,6 +1079,7 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
                     const unsigned long long li = idx - g.offs[r];
                     P.s0 = g.reqs[r].step;
                     P.tol = g.reqs[r].tol;
+                    P.ctl = scott_ctl(P.tol);
                     const Seeds s = g.reqs[r].s;
                     ScottEnd<R> e;
                     const int st = scott_init<R>(L, f, P, s.x[li], s.y[li], s.z[li], e,
@@ -1427,7 +1431,7 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
     if (rc) return rc;
     if (precision != FQSL_PREC_FP32C)
         return set_err(FQSL_ERR_UNSUPPORTED, "the in-place float field supports FQSL_PREC_FP32C only");
-    ScottParams P{FQSL_METHOD_RKF45, 0.0, 0.0};
+    ScottParams P{FQSL_METHOD_RKF45, 0.0, 0.0, 0.0};
     Seeds s{inp_x, inp_y, inp_z};
     ScottOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
                q_0, q_perp, LineLen, nullptr};

@@ -114,6 +114,7 @@ template <typename R> FQSL_HD R dot3(const R* a, const R* b) { return a[0] * b[0
 // Plain SoA layout, x fastest (the reference's layout).
 template <typename T>
 struct FieldSoA {
+    static constexpr bool kCoef = false;  // load() yields corner values
     const T* bx;
     const T* by;
     const T* bz;
@@ -133,11 +134,30 @@ struct FieldSoA {
     }
 };
 
-// Cell-stencil layout: each cell (i, j, k), i < nx-1 ..., stores its eight
-// corners for the three components contiguously as 8 float4 (128 B, one cache
-// line): [Bx c0-3][Bx c4-7][By c0-3][By c4-7][Bz c0-3][Bz c4-7][pad][pad],
-// corner q = a + 2b + 4c.  One interpolation is six 16-byte loads from one line.
+// Trilinear coefficients of one cell from its corners v[a + 2b + 4c]:
+//   B(x, y, z) = c0 + c1 x + c2 y + c3 z + c4 xy + c5 yz + c6 xz + c7 xyz
+// for (x, y, z) in [0, 1]^3, formed in double and rounded once.
+template <typename T>
+FQSL_HD void cell_coefs(const T* v, float* c) {
+    const double v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3], v4 = v[4], v5 = v[5], v6 = v[6], v7 = v[7];
+    c[0] = (float)v0;
+    c[1] = (float)(v1 - v0);
+    c[2] = (float)(v2 - v0);
+    c[3] = (float)(v4 - v0);
+    c[4] = (float)((v3 - v2) - (v1 - v0));
+    c[5] = (float)((v6 - v4) - (v2 - v0));
+    c[6] = (float)((v5 - v4) - (v1 - v0));
+    c[7] = (float)(((v7 - v6) - (v5 - v4)) - ((v3 - v2) - (v1 - v0)));
+}
+
+// Cell layout: each cell (i, j, k), i < nx-1 ..., stores the trilinear
+// coefficients c0..c7 of its three components contiguously as 8 float4
+// (128 B, one cache line): [Bx c0-3][Bx c4-7][By c0-3][By c4-7][Bz c0-3]
+// [Bz c4-7][pad][pad].  One interpolation is six 16-byte loads from one line,
+// and the coefficient form gives B and its three edge derivatives in 11
+// FMAs per component (the corner form needs ~22).
 struct FieldCell32 {
+    static constexpr bool kCoef = true;  // load() yields coefficients
     const float4* cells;
     int n[3];
     template <typename R>
@@ -217,6 +237,33 @@ FQSL_HD void cell_eval(const R c[3][8], R wx, R wy, R wz, R B[3], R d[3][3]) {
     }
 }
 
+// The same from trilinear coefficients (FieldCell32): with t = c1 + c4 y,
+// u = c6 + c7 y, A = c0 + c2 y + x t, C = c3 + c5 y + x u:
+//   B = A + z C,  dB/dz = C,  dB/dx = t + z u,  dB/dy = (c2 + c4 x) + z (c5 + c7 x).
+template <typename R>
+FQSL_HD void coef_eval(const R c[3][8], R x, R y, R z, R B[3], R d[3][3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const R t = c[q][1] + c[q][4] * y, u = c[q][6] + c[q][7] * y;
+        const R A = (c[q][0] + c[q][2] * y) + x * t;
+        const R C = (c[q][3] + c[q][5] * y) + x * u;
+        B[q] = A + z * C;
+        d[2][q] = C;
+        d[0][q] = t + z * u;
+        d[1][q] = (c[q][2] + c[q][4] * x) + z * (c[q][5] + c[q][7] * x);
+    }
+}
+
+template <typename R>
+FQSL_HD void coef_value(const R c[3][8], R x, R y, R z, R B[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const R A = (c[q][0] + c[q][2] * y) + x * (c[q][1] + c[q][4] * y);
+        const R C = (c[q][3] + c[q][5] * y) + x * (c[q][6] + c[q][7] * y);
+        B[q] = A + z * C;
+    }
+}
+
 // Value of the trilinear interpolant only.
 template <typename R>
 FQSL_HD void cell_value(const R c[3][8], R wx, R wy, R wz, R B[3]) {
@@ -227,6 +274,19 @@ FQSL_HD void cell_value(const R c[3][8], R wx, R wy, R wz, R B[3]) {
         const R f0 = e00 + wy * (e10 - e00), f1 = e01 + wy * (e11 - e01);
         B[q] = f0 + wz * (f1 - f0);
     }
+}
+
+// Dispatch on the field view's cell representation.
+template <class F, typename R>
+FQSL_HD void field_eval(const R c[3][8], R wx, R wy, R wz, R B[3], R d[3][3]) {
+    if constexpr (F::kCoef) coef_eval<R>(c, wx, wy, wz, B, d);
+    else cell_eval<R>(c, wx, wy, wz, B, d);
+}
+
+template <class F, typename R>
+FQSL_HD void field_value(const R c[3][8], R wx, R wy, R wz, R B[3]) {
+    if constexpr (F::kCoef) coef_value<R>(c, wx, wy, wz, B);
+    else cell_value<R>(c, wx, wy, wz, B);
 }
 
 // (b(x+h e_j) - b(x-h e_j)) / 2h from B and the covered increments.
@@ -300,7 +360,7 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     R cc[3][8];
     f.template load<R>(c[0], c[1], c[2], cc);
     R B[3], d[3][3];
-    cell_eval<R>(cc, w[0], w[1], w[2], B, d);
+    field_eval<F, R>(cc, w[0], w[1], w[2], B, d);
     const R inb = rsq(dot3(B, B));
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
@@ -327,7 +387,7 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
             ci[j] += dj;
             R cn[3][8], Bn[3], dn[3][3];
             f.template load<R>(ci[0], ci[1], ci[2], cn);
-            cell_eval<R>(cn, w[0], w[1], w[2], Bn, dn);
+            field_eval<F, R>(cn, w[0], w[1], w[2], Bn, dn);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 if (dj > 0) ap[q] += coef * dn[j][q];
@@ -350,7 +410,7 @@ FQSL_HD void unit_b(const F& f, const Pt<R>& p, R b[3]) {
     }
     R c[3][8], B[3];
     f.template load<R>(ci[0], ci[1], ci[2], c);
-    cell_value<R>(c, w[0], w[1], w[2], B);
+    field_value<F, R>(c, w[0], w[1], w[2], B);
     const R inb = rsq(dot3(B, B));
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
@@ -369,7 +429,7 @@ FQSL_HD void raw_b(const F& f, const double* x, double B[3]) {
     }
     R c[3][8], Bv[3];
     f.template load<R>(ci[0], ci[1], ci[2], c);
-    cell_value<R>(c, w[0], w[1], w[2], Bv);
+    field_value<F, R>(c, w[0], w[1], w[2], Bv);
     B[0] = Bv[0]; B[1] = Bv[1]; B[2] = Bv[2];
 }
 
@@ -480,7 +540,30 @@ struct ScottParams {
     int method;     // kRKF45 or kRK4
     double s0;      // initial (RK4: fixed) step, cells
     double tol;     // RKF45 absolute per-step tolerance, cells
+    double ctl;     // (tol / 11.09)^0.2, the tolerance factor of the step controller
+                    // (formed once per kernel on the device: scott_ctl)
 };
+
+// (tol / 11.09)^0.2: the controller's scale (tol / err / 11.09)^0.2
+// (TraceBlineScott.cu:413) is ctl * (err^2)^(-1/10).
+FQSL_HD double scott_ctl(double tol) { return tol > 0 ? pow(tol / 11.09, 0.2) : 0.0; }
+
+// e2^(-1/10) to double precision without a double pow: a float estimate
+// y0 = 2^(-0.1 log2 e2) (relative error up to ~1e-6, from the float log2 of
+// values up to |100|) refined by two Newton steps for y^10 e2 = 1,
+// y <- y (1.1 - 0.1 y^10 e2); each step maps a relative error d to 5.5 d^2,
+// so the result is exact to double rounding.  Zero, NaN, infinite and extreme
+// inputs take the library pow.
+FQSL_HD double pow_minus_tenth(double e2) {
+    if (!(e2 > 1e-30 && e2 < 1e30)) return pow(e2, -0.1);
+    double y = (double)exp2f(-0.1f * log2f((float)e2));
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double y2 = y * y, y4 = y2 * y2, y8 = y4 * y4;
+        y = y * (1.1 - 0.1 * (y8 * y2 * e2));
+    }
+    return y;
+}
 
 // Status of a line after init / attempt.
 enum : int { kTracing = 0, kFinished = 1, kAborted = 2 };
@@ -685,8 +768,11 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
             e2 += ec * ec;
         }
         L.evals += 6;
-        // step controller (TraceBlineScott.cu:413-419), FP64
-        const double scale = pow(P.tol / sqrt(e2) / 11.09, 0.2);
+        // step controller (TraceBlineScott.cu:413-419), FP64: the FP64 path
+        // evaluates the reference's expression as written; the FP32 path the
+        // same quantity as ctl * e2^(-1/10) (double-accurate, no pow/sqrt/div)
+        const double scale = sizeof(R) == 4 ? P.ctl * pow_minus_tenth(e2)
+                                            : pow(P.tol / sqrt(e2) / 11.09, 0.2);
         if (scale < 0.618) {
             L.s = L.s * 0.618;
             if (!scott_can_step(L, scott_len_lim(f))) {  // attempt budget spent: limit exit
@@ -724,7 +810,10 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
     Pos<R> xb = L.x;
     xb.add(inc);
     const double xbv[3] = {xb.get(0), xb.get(1), xb.get(2)};
-    const double dL = sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
+    // step length: the increment is R, so its norm is taken in R; the sum over
+    // steps is kept in double
+    const double dL = sizeof(R) == 4 ? (double)sqrtf((float)(inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2]))
+                                     : sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
     L.len += dL;
     const int flag = check_flag(f.n, xbv);
     if (flag > 0) {

@@ -55,7 +55,7 @@ extern "C" {
 /* field layouts held on the device (bit mask for fqsl_field_create) */
 #define FQSL_LAYOUT_SOA32 1  /* three float arrays, x fastest */
 #define FQSL_LAYOUT_SOA64 2  /* three double arrays, x fastest */
-#define FQSL_LAYOUT_CELL32 4 /* 128-byte cell stencils (8 corners x 3 components, float) */
+#define FQSL_LAYOUT_CELL32 4 /* 128-byte cells: 8 trilinear coefficients x 3 components, float */
 #define FQSL_LAYOUT_ALL 7
 
 const char *fqsl_last_error(void);

@@ -82,3 +82,12 @@ def trace_plain(f, seeds, ncross=(0, 0, 1), precision=0, method=0, s0=0.25, tol_
                      P(o["start"]), P(o["end"]), P(o["flag_start"], _ip), P(o["flag_end"], _ip),
                      P(o["length"]), P(o["evals"], _lp))
     return o
+
+
+def pow_minus_tenth(e2):
+    e = np.ascontiguousarray(np.asarray(e2, np.float64).reshape(-1))
+    out = np.empty_like(e)
+    L = lib()
+    L.hc_pow_minus_tenth.argtypes = [ctypes.c_long, _dp, _dp]
+    L.hc_pow_minus_tenth(e.size, P(e), P(out))
+    return out

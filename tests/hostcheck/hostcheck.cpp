@@ -11,7 +11,7 @@
 using namespace fqsl;
 
 namespace {
-// host copy of the cell-stencil layout
+// host copy of the cell layout (trilinear coefficients)
 std::vector<float4> build_cells(const float* bx, const float* by, const float* bz, int nx, int ny, int nz) {
     const long cx = nx - 1, cy = ny - 1, cz = nz - 1, nxy = (long)nx * ny;
     std::vector<float4> cells(8 * cx * cy * cz);
@@ -20,10 +20,11 @@ std::vector<float4> build_cells(const float* bx, const float* by, const float* b
         const long i = t % cx, j = (t / cx) % cy, k = t / (cx * cy);
         const long base = k * nxy + j * nx + i;
         for (int q = 0; q < 3; ++q) {
-            float v[8];
+            float v[8], c8[8];
             for (int c = 0; c < 8; ++c) v[c] = comp[q][base + (c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy];
-            cells[8 * t + 2 * q] = make_float4(v[0], v[1], v[2], v[3]);
-            cells[8 * t + 2 * q + 1] = make_float4(v[4], v[5], v[6], v[7]);
+            cell_coefs(v, c8);
+            cells[8 * t + 2 * q] = make_float4(c8[0], c8[1], c8[2], c8[3]);
+            cells[8 * t + 2 * q + 1] = make_float4(c8[4], c8[5], c8[6], c8[7]);
         }
         cells[8 * t + 6] = make_float4(0, 0, 0, 0);
         cells[8 * t + 7] = make_float4(0, 0, 0, 0);
@@ -57,7 +58,7 @@ extern "C" {
 int hc_trace_scott(const double* bx, const double* by, const double* bz, int nx, int ny, int nz, int precision,
                    int method, double s0, double tol, long n, const float* seeds, double* start, double* end,
                    int* fs, int* fe, double* q0, double* qp, double* len, long* evals) {
-    ScottParams P{method, s0, tol};
+    ScottParams P{method, s0, tol, scott_ctl(tol)};
     const long nn = (long)nx * ny * nz;
     if (precision == 1) {
         FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
@@ -69,6 +70,11 @@ int hc_trace_scott(const double* bx, const double* by, const double* bz, int nx,
     FieldCell32 f{cells.data(), {nx, ny, nz}};
     run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
     return 0;
+}
+
+// The FP32 path's step-controller power e2^(-1/10) at n values.
+void hc_pow_minus_tenth(long n, const double* e2, double* out) {
+    for (long i = 0; i < n; ++i) out[i] = pow_minus_tenth(e2[i]);
 }
 
 // grad b by the stencil formula at n points (double SoA); out[9*i + 3*j + c].

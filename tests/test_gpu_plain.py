@@ -222,7 +222,11 @@ def test_dropin_scott_kernel_matches_direct_api(fq):
     rng = np.random.default_rng(1)
     seeds = np.column_stack([rng.uniform(0, 63, 3000), rng.uniform(0, 47, 3000),
                              np.r_[np.zeros(2000), rng.uniform(0, 39, 1000)]]).astype(np.float32)
-    with fq.Field(*field) as f:
+    # the drop-in reads the caller's float arrays in place (corner form of the
+    # interpolant); the handle API on the same in-place SoA layout must give
+    # the same bits.  (The default cell layout holds trilinear coefficients,
+    # whose float rounding differs in the last bits.)
+    with fq.Field(*field, layouts=("soa32", "soa64")) as f:
         # the drop-in's tolerance is 1e-3 * tol_coef with tol_coef a float32
         tol = 1e-3 * float(np.float32(1e-3))
         ref = fq.trace_scott(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rkf45", step=0.25, tol=tol)
@@ -252,7 +256,7 @@ def test_dropin_plain_kernels_match_direct_api(fq):
     n = seeds.shape[0]
     bx, by, bz = (np.asfortranarray(b.astype(np.float32)) for b in field)
     for which, ctor in (("rkf45", fq.kernels.compileTraceBlineAdaptive), ("rk4", fq.kernels.compileTraceBlineRK4)):
-        with fq.Field(*field) as f:
+        with fq.Field(*field, layouts=("soa32", "soa64")) as f:   # the drop-in's in-place layout
             ref = fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method=which, step=0.125,
                                  tol_coef=0.5, precision="fp32c")
             torch.cuda.synchronize()

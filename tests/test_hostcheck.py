@@ -96,3 +96,18 @@ def test_plain_tracer_matches_the_oracle(oracle, quad, precision):
     tol = 1e-8 if precision == 1 else 2e-3
     assert np.abs(r["end"] - ref["end"])[cond].max() < tol
     assert np.abs(r["start"] - ref["start"])[cond].max() < tol
+
+
+def test_controller_power_is_double_accurate():
+    """The FP32 path evaluates the step controller's (tol/err/11.09)^0.2 as
+    ctl * e2^(-1/10) with a float estimate refined by two Newton steps; it
+    must agree with the double power to double rounding over the range, and
+    keep pow's special values (0 -> inf, NaN -> NaN).  A float-only estimate
+    would be off by up to ~1e-6, one Newton step by ~3e-12."""
+    rng = np.random.default_rng(14)
+    e2 = 10.0 ** rng.uniform(-29.9, 29.9, 20000)
+    got = hc.pow_minus_tenth(e2)
+    np.testing.assert_allclose(got, e2 ** -0.1, rtol=1e-15, atol=0)
+    sp = hc.pow_minus_tenth([0.0, np.nan, np.inf, 1e-40, 1e40])
+    assert sp[0] == np.inf and np.isnan(sp[1]) and sp[2] == 0.0
+    np.testing.assert_allclose(sp[3:], np.array([1e-40, 1e40]) ** -0.1, rtol=1e-15)
