@@ -276,6 +276,31 @@ FQSL_HD void cell_value(const R c[3][8], R wx, R wy, R wz, R B[3]) {
     }
 }
 
+// Edge derivative along axis J only (the d[J] of cell_eval / coef_eval).
+template <int J, typename R>
+FQSL_HD void coef_edge(const R c[3][8], R x, R y, R z, R d[3]) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (J == 0) d[q] = (c[q][1] + c[q][4] * y) + z * (c[q][6] + c[q][7] * y);
+        else if (J == 1) d[q] = (c[q][2] + c[q][4] * x) + z * (c[q][5] + c[q][7] * x);
+        else d[q] = (c[q][3] + c[q][5] * y) + x * (c[q][6] + c[q][7] * y);
+    }
+}
+
+template <int J, typename R>
+FQSL_HD void cell_edge(const R c[3][8], R wx, R wy, R wz, R d[3]) {
+    // corner bit of axis J and the two other axes (u, v) with their weights
+    constexpr int bj = 1 << J, bu = J == 0 ? 2 : 1, bv = J == 2 ? 2 : 4;
+    const R wu = J == 0 ? wy : wx, wv = J == 2 ? wy : wz;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const R e00 = c[q][bj] - c[q][0], e10 = c[q][bj + bu] - c[q][bu];
+        const R e01 = c[q][bj + bv] - c[q][bv], e11 = c[q][bj + bu + bv] - c[q][bu + bv];
+        const R f0 = e00 + wu * (e10 - e00), f1 = e01 + wu * (e11 - e01);
+        d[q] = f0 + wv * (f1 - f0);
+    }
+}
+
 // Dispatch on the field view's cell representation.
 template <class F, typename R>
 FQSL_HD void field_eval(const R c[3][8], R wx, R wy, R wz, R B[3], R d[3][3]) {
@@ -287,6 +312,12 @@ template <class F, typename R>
 FQSL_HD void field_value(const R c[3][8], R wx, R wy, R wz, R B[3]) {
     if constexpr (F::kCoef) coef_value<R>(c, wx, wy, wz, B);
     else cell_value<R>(c, wx, wy, wz, B);
+}
+
+template <class F, int J, typename R>
+FQSL_HD void field_edge(const R c[3][8], R wx, R wy, R wz, R d[3]) {
+    if constexpr (F::kCoef) coef_edge<J, R>(c, wx, wy, wz, d);
+    else cell_edge<J, R>(c, wx, wy, wz, d);
 }
 
 // (b(x+h e_j) - b(x-h e_j)) / 2h from B and the covered increments.
@@ -329,6 +360,43 @@ FQSL_HD Pt<R> pt_from_double(const double* x) {
     return p;
 }
 
+// Row J of grad b at p from the cell's B and edge derivative dj along J:
+// the closed form on a simple axis, else the exact difference quotient, which
+// takes the neighbour cell's edge derivative where the +-h sample crosses a
+// grid plane (crossing on both sides at once would need 2h > 1 cell).
+template <int J, typename R, class F>
+FQSL_HD void scott_axis(const F& f, const Pt<R>& p, const int* c, const R* w, bool simple, const R B[3],
+                        const R b[3], R inb, const R dj[3], R out[3]) {
+    if (simple) {
+        const R bd = dot3(b, dj);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) out[q] = (dj[q] - b[q] * bd) * inb;
+        return;
+    }
+    const Axis<R> ax = axis_setup<R>(p.i[J], p.r[J], f.n[J]);
+    R ap[3], am[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        ap[q] = ax.ap * dj[q];
+        am[q] = ax.am * dj[q];
+    }
+    if (ax.apn != (R)0 || ax.amp != (R)0) {  // the +-h sample crosses a grid plane
+        const bool up = ax.apn != (R)0;
+        const R coef = up ? ax.apn : ax.amp;
+        int ci[3] = {c[0], c[1], c[2]};
+        ci[J] += up ? 1 : -1;
+        R cn[3][8], dn[3];
+        f.template load<R>(ci[0], ci[1], ci[2], cn);
+        field_edge<F, J, R>(cn, w[0], w[1], w[2], dn);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (up) ap[q] += coef * dn[q];
+            else am[q] += coef * dn[q];
+        }
+    }
+    fd_quotient<R>(B, ap, am, out);
+}
+
 // Scott right-hand side at p: b = B/|B| and J[j][q] = (difference quotient of
 // b_q along axis j), i.e. grad_unit_vec_B (TraceBlineScott.cu:204-226).
 //
@@ -364,38 +432,9 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     const R inb = rsq(dot3(B, B));
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        if (simple[j]) {
-            const R bd = dot3(b, d[j]);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) J[j][q] = (d[j][q] - b[q] * bd) * inb;
-            continue;
-        }
-        const Axis<R> ax = axis_setup<R>(p.i[j], p.r[j], f.n[j]);
-        R ap[3], am[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            ap[q] = ax.ap * d[j][q];
-            am[q] = ax.am * d[j][q];
-        }
-        if (ax.apn != (R)0 || ax.amp != (R)0) {  // the +-h sample crosses a grid plane
-            // (crossing on both sides at once would need 2h > 1 cell)
-            const int dj = ax.apn != (R)0 ? 1 : -1;
-            const R coef = ax.apn != (R)0 ? ax.apn : ax.amp;
-            int ci[3] = {c[0], c[1], c[2]};
-            ci[j] += dj;
-            R cn[3][8], Bn[3], dn[3][3];
-            f.template load<R>(ci[0], ci[1], ci[2], cn);
-            field_eval<F, R>(cn, w[0], w[1], w[2], Bn, dn);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (dj > 0) ap[q] += coef * dn[j][q];
-                else am[q] += coef * dn[j][q];
-            }
-        }
-        fd_quotient<R>(B, ap, am, J[j]);
-    }
+    scott_axis<0, R, F>(f, p, c, w, simple[0], B, b, inb, d[0], J[0]);
+    scott_axis<1, R, F>(f, p, c, w, simple[1], B, b, inb, d[1], J[1]);
+    scott_axis<2, R, F>(f, p, c, w, simple[2], B, b, inb, d[2], J[2]);
 }
 
 // Unit vector b only (plain tracer RHS: Interp3dxyzn(..., true)).
