@@ -1,5 +1,7 @@
 """Drop-in for ``fastqslpy.FastQSL`` (fastqslpy/FastQSL.py) plus the Qcut
-assembly the demo notebooks do inline (demo/Q_RKF45CuPy.ipynb cells [10]-[11]).
+assembly the demo notebooks do inline (demo/Q_RKF45CuPy.ipynb cells [10]-[11])
+and the current density for the twist number (curl_of_B_grid, diff_3point,
+copy_boundary, FastQSL.py:120-163).
 
 ``QCalcPlane``, ``CookPseudoLine`` and ``ResReshape`` keep the reference's
 positional signatures and return values.  Inputs may be torch CUDA tensors or
@@ -186,3 +188,87 @@ def Qcut(cut_start_x, cut_start_y, cut_start_z, flag_cut_start,
     if Q_all is None:
         return d.out(target)
     return Q_all
+
+
+# ----------------------------------------------------------- current density
+def _xfast_dev(a, dtype=None):
+    """(nx, ny, nz) array or tensor -> x-fastest CUDA tensor, its dtype code,
+    and whether the caller passed NumPy."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        t = a.to(device="cuda")
+        is_np = False
+    else:
+        t = torch.as_tensor(np.asarray(a), device="cuda")
+        is_np = True
+    if dtype is not None:
+        t = t.to(dtype)
+    elif t.dtype not in (torch.float32, torch.float64):
+        t = t.to(torch.float64)
+    t = t.permute(2, 1, 0).contiguous()
+    code = _lib.DTYPE_F64 if t.dtype == torch.float64 else _lib.DTYPE_F32
+    return t, code, is_np
+
+
+def _axis(a):
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        return a.to(device="cuda", dtype=torch.float64).reshape(-1).contiguous()
+    return torch.as_tensor(np.asarray(a, np.float64).reshape(-1), device="cuda")
+
+
+def _back(t, is_np):
+    """x-fastest (nz, ny, nx) device tensor -> (nx, ny, nz) view, NumPy if asked."""
+    v = t.permute(2, 1, 0)
+    return v.cpu().numpy() if is_np else v
+
+
+def curl_of_B_grid(Bx, By, Bz, arr_x, arr_y, arr_z):
+    """curl B on the node grid of axes arr_x/y/z (FastQSL.py:120-131): float64
+    (nx, ny, nz) arrays, three-point differences on non-uniform spacing inside,
+    boundary nodes copied from the nearest interior node.  Axes are used in
+    float64."""
+    torch = _torch()
+    bx, code, is_np = _xfast_dev(Bx)
+    by, _, _ = _xfast_dev(By, bx.dtype)
+    bz, _, _ = _xfast_dev(Bz, bx.dtype)
+    nx, ny, nz = (int(v) for v in np.shape(Bx))
+    ax, ay, az = _axis(arr_x), _axis(arr_y), _axis(arr_z)
+    if (ax.numel(), ay.numel(), az.numel()) != (nx, ny, nz):
+        raise ValueError("axis lengths must match the field shape")
+    out = [torch.empty((nz, ny, nx), dtype=torch.float64, device="cuda") for _ in range(3)]
+    _lib.check(_lib.lib().fqsl_curl(bx.data_ptr(), by.data_ptr(), bz.data_ptr(), nx, ny, nz, code,
+                                    ax.data_ptr(), ay.data_ptr(), az.data_ptr(), out[0].data_ptr(),
+                                    out[1].data_ptr(), out[2].data_ptr(), _stream()))
+    return [_back(o, is_np) for o in out]
+
+
+def diff_3point(Cube, arr_x, axis):
+    """Three-point derivative along `axis` at the interior nodes
+    (FastQSL.py:143-163): a float64 (nx-2, ny-2, nz-2) array."""
+    torch = _torch()
+    c, code, is_np = _xfast_dev(Cube)
+    nx, ny, nz = (int(v) for v in np.shape(Cube))
+    ax = _axis(arr_x)
+    if ax.numel() != (nx, ny, nz)[axis]:
+        raise ValueError("axis length must match the cube along that axis")
+    out = torch.empty((nz - 2, ny - 2, nx - 2), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.lib().fqsl_diff_3point(c.data_ptr(), nx, ny, nz, code, ax.data_ptr(), int(axis),
+                                           out.data_ptr(), _stream()))
+    return _back(out, is_np)
+
+
+def copy_boundary(arr):
+    """Boundary nodes take the value of the nearest interior node
+    (FastQSL.py:133-141); in place for a CUDA float64 tensor (returned), a
+    copy for anything else."""
+    torch = _torch()
+    inplace = isinstance(arr, torch.Tensor) and arr.is_cuda and arr.dtype == torch.float64
+    t, _, is_np = _xfast_dev(arr, torch.float64)
+    nx, ny, nz = (int(v) for v in np.shape(arr))
+    _lib.check(_lib.lib().fqsl_copy_boundary(t.data_ptr(), nx, ny, nz, _stream()))
+    res = _back(t, is_np)
+    if inplace:
+        arr.copy_(res)
+        return arr
+    return res

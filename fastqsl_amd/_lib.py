@@ -37,7 +37,7 @@ class fqsl_scott_params(ctypes.Structure):
 class fqsl_scott_out(ctypes.Structure):
     _fields_ = [(n, vp) for n in ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
                                   "flag_start", "flag_end", "b_this_x", "b_this_y", "b_this_z",
-                                  "q_0", "q_perp", "line_len", "evals")]
+                                  "q_0", "q_perp", "line_len", "evals", "twist")]
 
 
 class fqsl_plain_params(ctypes.Structure):
@@ -50,7 +50,7 @@ class fqsl_plain_out(ctypes.Structure):
     _fields_ = [(n, vp) for n in ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
                                   "flag_start", "flag_end", "b_this_x", "b_this_y", "b_this_z",
                                   "b_flag", "b_start_x", "b_start_y", "b_start_z", "b_end_x",
-                                  "b_end_y", "b_end_z", "line_len", "evals")]
+                                  "b_end_y", "b_end_z", "line_len", "evals", "twist")]
 
 
 class fqsl_scott_request(ctypes.Structure):
@@ -67,6 +67,10 @@ PROTOTYPES = {
     "fqsl_field_destroy": (i32, [vp]),
     "fqsl_field_bytes": (ctypes.c_size_t, [vp]),
     "fqsl_field_shape": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+    "fqsl_field_set_current": (i32, [vp, vp, vp, vp, i32, i32]),
+    "fqsl_curl": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "fqsl_diff_3point": (i32, [vp, i32, i32, i32, i32, vp, i32, vp, vp]),
+    "fqsl_copy_boundary": (i32, [vp, i32, i32, i32, vp]),
     "fqsl_trace_scott": (i32, [vp, ctypes.POINTER(fqsl_scott_params), u64, vp, vp, vp,
                                ctypes.POINTER(fqsl_scott_out), vp]),
     "fqsl_trace_plain": (i32, [vp, ctypes.POINTER(fqsl_plain_params), u64, vp, vp, vp,

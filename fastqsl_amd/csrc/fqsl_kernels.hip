@@ -57,6 +57,8 @@ struct fqsl_field {
     float* soa32[3];
     double* soa64[3];
     float4* cell32;
+    float* cur32[3];   // current density J (twist), float and double SoA
+    double* cur64[3];
     size_t bytes;
 };
 
@@ -106,6 +108,8 @@ static void field_free(fqsl_field* f) {
     for (int q = 0; q < 3; ++q) {
         if (f->soa32[q]) (void)hipFree(f->soa32[q]);
         if (f->soa64[q]) (void)hipFree(f->soa64[q]);
+        if (f->cur32[q]) (void)hipFree(f->cur32[q]);
+        if (f->cur64[q]) (void)hipFree(f->cur64[q]);
     }
     if (f->cell32) (void)hipFree(f->cell32);
     delete f;
@@ -197,6 +201,48 @@ extern "C" int fqsl_field_destroy(fqsl_field* f) {
 }
 
 extern "C" size_t fqsl_field_bytes(const fqsl_field* f) { return f ? f->bytes : 0; }
+
+extern "C" int fqsl_field_set_current(fqsl_field* f, const void* jx, const void* jy, const void* jz, int dtype,
+                                      int on_device) {
+    if (!f) return set_err(FQSL_ERR_INVALID, "field is NULL");
+    if (!jx || !jy || !jz) return set_err(FQSL_ERR_INVALID, "current component pointer is NULL");
+    if (dtype != FQSL_DTYPE_F32 && dtype != FQSL_DTYPE_F64) return set_err(FQSL_ERR_INVALID, "dtype %d", dtype);
+    const long nn = (long)f->n[0] * f->n[1] * f->n[2];
+    const size_t esz = dtype == FQSL_DTYPE_F64 ? 8 : 4;
+    const void* src[3] = {jx, jy, jz};
+    for (int q = 0; q < 3; ++q) {
+        if (f->cur32[q]) { (void)hipFree(f->cur32[q]); f->cur32[q] = nullptr; f->bytes -= nn * 4; }
+        if (f->cur64[q]) { (void)hipFree(f->cur64[q]); f->cur64[q] = nullptr; f->bytes -= nn * 8; }
+    }
+    for (int q = 0; q < 3; ++q) {
+        void* staged = const_cast<void*>(src[q]);
+        if (!on_device) {
+            HIP_TRY(hipMalloc(&staged, nn * esz));
+            hipError_t e = hipMemcpy(staged, src[q], nn * esz, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                (void)hipFree(staged);
+                return set_err(FQSL_ERR_HIP, "hipMemcpy(current): %s", hipGetErrorString(e));
+            }
+        }
+        hipError_t e = hipMalloc(&f->cur32[q], nn * 4);
+        if (e == hipSuccess) e = hipMalloc(&f->cur64[q], nn * 8);
+        if (e == hipSuccess) {
+            if (dtype == FQSL_DTYPE_F64) {
+                e = hipMemcpy(f->cur64[q], staged, nn * 8, hipMemcpyDeviceToDevice);
+                if (e == hipSuccess) k_convert_to_f32<<<grid_for(nn, 256), 256>>>((const double*)staged, f->cur32[q], nn);
+            } else {
+                e = hipMemcpy(f->cur32[q], staged, nn * 4, hipMemcpyDeviceToDevice);
+                if (e == hipSuccess) k_convert_to_f64<<<grid_for(nn, 256), 256>>>((const float*)staged, f->cur64[q], nn);
+            }
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+        }
+        if (!on_device) (void)hipFree(staged);
+        if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "current layout: %s", hipGetErrorString(e));
+        f->bytes += nn * 12;
+    }
+    return FQSL_OK;
+}
 
 extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) {
     if (!f) return set_err(FQSL_ERR_INVALID, "field is NULL");
@@ -301,6 +347,7 @@ struct ScottOut {
     float *btx, *bty, *btz, *q0, *qp;
     double* len;
     uint32_t* evals;
+    double* tw;
 };
 
 struct PlainOut {
@@ -311,6 +358,7 @@ struct PlainOut {
     float *bsx, *bsy, *bsz, *bex, *bey, *bez;
     double* len;
     uint32_t* evals;
+    double* tw;
 };
 
 template <typename R, class F>
@@ -333,6 +381,7 @@ __device__ __forceinline__ void scott_write(const ScottLine<R>& L, const ScottEn
     if (o.qp) o.qp[i] = (float)qp;
     if (o.len) o.len[i] = L.len;
     if (o.evals) o.evals[i] = L.evals;
+    if (o.tw) o.tw[i] = L.tw;
     if (o.btx) {
         float bt[3];
         b_clamped<R>(f, L.p0, bt);
@@ -362,6 +411,7 @@ struct ScottRec {
     double len;
     int32_t status;
     uint32_t evals;
+    double tw;
 };
 
 // Work queues of the persistent kernels: kQueues counters, queue q owning
@@ -555,12 +605,13 @@ __global__ void __launch_bounds__(256) k_scott_prep(F f, ScottParams P, Seeds s,
 // queue entries once at least `refill_min` lanes of the wavefront are idle (or
 // all are), so the setup of refilled lines runs once for the group.  A wave
 // exits once every queue is empty and all its lanes are idle.
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                       const R* __restrict__ frames, ScottRec<R>* __restrict__ recs,
                                                       DevScalars ds, unsigned long long* queues, int nq,
-                                                      int refill_min, const unsigned* __restrict__ order) {
+                                                      int refill_min, const unsigned* __restrict__ order, FJ fj) {
     apply_scalars(f, P, n, ds);
+    fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     P.ctl = scott_ctl(P.tol);  // on the device, so every path forms the same bits
     ScottLine<R> L;
     bool active = false;
@@ -586,7 +637,7 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
                     const int st = scott_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx], e,
                                                  frames + (unsigned long long)kFrameWords * idx);
                     if (st != kTracing) {
-                        recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                        recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                     } else {
                         active = true;
                     }
@@ -599,9 +650,9 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
         }
         if (active) {
             ScottEnd<R> e;
-            const int st = scott_attempt<R>(L, f, P, e);
+            const int st = scott_attempt<R, F, FJ>(L, f, P, e, fj);
             if (st != kTracing) {
-                recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                 active = false;
             }
         }
@@ -610,20 +661,20 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
 
 // W = waves per SIMD the register allocation targets.
 #define FQSL_PERSISTENT_KERNEL(W)                                                                     \
-    template <typename R, class F>                                                                    \
+    template <typename R, class F, class FJ>                                                          \
     __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                \
     k_scott_persistent_w##W(F f, ScottParams P, Seeds s, unsigned long long n, const R* frames,       \
                             ScottRec<R>* recs, DevScalars ds, unsigned long long* queues, int nq,     \
-                            int refill_min, const unsigned* order) {                                  \
-        scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order);     \
+                            int refill_min, const unsigned* order, FJ fj) {                           \
+        scott_persistent_body<R, F, FJ>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj); \
     }
 
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Seeds s, unsigned long long n,
                                                           const R* frames, ScottRec<R>* recs, DevScalars ds,
                                                           unsigned long long* queues, int nq, int refill_min,
-                                                          const unsigned* order) {
-    scott_persistent_body<R, F>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order);
+                                                          const unsigned* order, FJ fj) {
+    scott_persistent_body<R, F, FJ>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj);
 }
 FQSL_PERSISTENT_KERNEL(2)
 FQSL_PERSISTENT_KERNEL(3)
@@ -649,37 +700,39 @@ __global__ void __launch_bounds__(256) k_scott_finish(F f, ScottParams P, unsign
         if (o.qp) o.qp[i] = (float)qp;
         if (o.len) o.len[i] = r.len;
         if (o.evals) o.evals[i] = r.evals;
-        }
+        if (o.tw) o.tw[i] = r.tw;
+    }
 }
 
 // One thread per line (grid-stride), the reference's mapping
 // (TraceBlineScott.cu:530-553).
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s, unsigned long long n,
-                                                    ScottOut out, DevScalars ds) {
+                                                    ScottOut out, DevScalars ds, FJ fj) {
     apply_scalars(f, P, n, ds);
+    fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     P.ctl = scott_ctl(P.tol);  // on the device, so every path forms the same bits
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
         ScottLine<R> L;
         ScottEnd<R> e;
         int st = scott_init<R>(L, f, P, s.x[i], s.y[i], s.z[i], e);
-        while (st == kTracing) st = scott_attempt<R>(L, f, P, e);
+        while (st == kTracing) st = scott_attempt<R, F, FJ>(L, f, P, e, fj);
         scott_write<R>(L, e, st, f, i, out, ds);
     }
 }
 
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seeds s, unsigned long long n,
-                                                        ScottOut out, DevScalars ds) {
-    scott_per_line_body<R, F>(f, P, s, n, out, ds);
+                                                        ScottOut out, DevScalars ds, FJ fj) {
+    scott_per_line_body<R, F, FJ>(f, P, s, n, out, ds, fj);
 }
 
 // FP32 variant compiled for two waves per SIMD (like the persistent kernel).
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_scott_per_line_w2(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds) {
-    scott_per_line_body<R, F>(f, P, s, n, out, ds);
+k_scott_per_line_w2(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds, FJ fj) {
+    scott_per_line_body<R, F, FJ>(f, P, s, n, out, ds, fj);
 }
 
 // ------------------------------------------------------------ plain kernels
@@ -694,6 +747,7 @@ __device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, c
     if (o.fe) o.fe[i] = L.flag_e;
     if (o.len) o.len[i] = L.len_total;
     if (o.evals) o.evals[i] = L.evals;
+    if (o.tw) o.tw[i] = L.tw;
     float b[3];
     if (o.bsx) { b_clamped<R>(f, xs, b); o.bsx[i] = b[0]; o.bsy[i] = b[1]; o.bsz[i] = b[2]; }
     if (o.bex) { b_clamped<R>(f, xe, b); o.bex[i] = b[0]; o.bey[i] = b[1]; o.bez[i] = b[2]; }
@@ -717,11 +771,12 @@ __device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsign
     P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];
 }
 
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Seeds s, unsigned long long n,
                                                           PlainOut out, DevScalars ds, const float* ncross,
-                                                          unsigned long long* queue) {
+                                                          unsigned long long* queue, FJ fj) {
     apply_plain_scalars(f, P, n, ds, ncross);
+    fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     PlainLine<R> L;
     bool active = false;
     bool drained = false;
@@ -753,7 +808,7 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
             continue;
         }
         if (active) {
-            plain_attempt<R>(L, f, P);
+            plain_attempt<R, F, FJ>(L, f, P, fj);
             if (L.done) {
                 plain_write<R>(L, f, P, idx, out, ds);
                 active = false;
@@ -762,15 +817,16 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
     }
 }
 
-template <typename R, class F>
+template <typename R, class F, class FJ>
 __global__ void __launch_bounds__(256) k_plain_per_line(F f, PlainParams P, Seeds s, unsigned long long n,
-                                                        PlainOut out, DevScalars ds, const float* ncross) {
+                                                        PlainOut out, DevScalars ds, const float* ncross, FJ fj) {
     apply_plain_scalars(f, P, n, ds, ncross);
+    fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
         PlainLine<R> L;
         plain_init<R>(L, f, P, s.x[i], s.y[i], s.z[i]);
-        while (!L.done) plain_attempt<R>(L, f, P);
+        while (!L.done) plain_attempt<R, F, FJ>(L, f, P, fj);
         plain_write<R>(L, f, P, i, out, ds);
     }
 }
@@ -818,9 +874,10 @@ static int persistent_grid(K kernel, int* grid) {
     return FQSL_OK;
 }
 
-template <typename R, class F>
+template <typename R, class F, class FJ = NoCurrent>
 static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsigned long long n, const ScottOut& o,
-                        const DevScalars& ds, int schedule, hipStream_t st, unsigned long long n_hint) {
+                        const DevScalars& ds, int schedule, hipStream_t st, unsigned long long n_hint,
+                        const FJ& fj = FJ{}) {
     if (schedule == FQSL_SCHED_PERSISTENT) {
         // n_hint: number of lines, known on the host (sizes the workspace)
         if (n_hint == 0) return FQSL_OK;
@@ -828,9 +885,9 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         int rc = queue_slot(&q);
         if (rc) return rc;
         const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES] : 0;
-        auto kern = waves == 2 ? k_scott_persistent_w2<R, F>
-                  : waves == 3 ? k_scott_persistent_w3<R, F>
-                               : k_scott_persistent<R, F>;
+        auto kern = waves == 2 ? k_scott_persistent_w2<R, F, FJ>
+                  : waves == 3 ? k_scott_persistent_w3<R, F, FJ>
+                               : k_scott_persistent<R, F, FJ>;
         int grid = 0;
         rc = persistent_grid(kern, &grid);
         if (rc) return rc;
@@ -868,7 +925,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
             }
             k_scott_prep<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, frames, o, ds);
             e = hipEventRecord(g_evs0, st);
-            kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min, order);
+            kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min, order, fj);
             if (e == hipSuccess) e = hipEventRecord(g_evs1, st);
             if (e == hipSuccess) g_evs_valid = true;
             k_scott_finish<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, n, frames, recs, o, ds);
@@ -884,38 +941,38 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
         if (rc) return rc;
-        auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F>
-                                                                        : k_scott_per_line<R, F>;
-        kern<<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds);
+        auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F, FJ>
+                                                                        : k_scott_per_line<R, F, FJ>;
+        kern<<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, fj);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
     return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
 }
 
-template <typename R, class F>
+template <typename R, class F, class FJ = NoCurrent>
 static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsigned long long n, const PlainOut& o,
                         const DevScalars& ds, const float* ncross, int schedule, hipStream_t st,
-                        unsigned long long n_hint) {
+                        unsigned long long n_hint, const FJ& fj = FJ{}) {
     if (schedule == FQSL_SCHED_PERSISTENT) {
         unsigned long long* q = nullptr;
         int rc = queue_slot(&q);
         if (rc) return rc;
         int grid = 0;
-        rc = persistent_grid(k_plain_persistent<R, F>, &grid);
+        rc = persistent_grid(k_plain_persistent<R, F, FJ>, &grid);
         if (rc) return rc;
         k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
         HIP_TRY(hipGetLastError());
         rc = record_start(st);
         if (rc) return rc;
-        k_plain_persistent<R, F><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q);
+        k_plain_persistent<R, F, FJ><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q, fj);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
         if (rc) return rc;
-        k_plain_per_line<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, ncross);
+        k_plain_per_line<R, F, FJ><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, ncross, fj);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
@@ -944,25 +1001,148 @@ extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params
     Seeds s{x, y, z};
     ScottOut o{out->start_x, out->start_y, out->start_z, out->end_x, out->end_y, out->end_z,
                out->flag_start, out->flag_end, out->b_this_x, out->b_this_y, out->b_this_z,
-               out->q_0, out->q_perp, out->line_len, out->evals};
+               out->q_0, out->q_perp, out->line_len, out->evals, out->twist};
     if ((o.sx && (!o.sy || !o.sz)) || (o.ex && (!o.ey || !o.ez)) || (o.btx && (!o.bty || !o.btz)))
         return set_err(FQSL_ERR_INVALID, "x/y/z output arrays must be all set or all NULL");
     DevScalars ds{};
     hipStream_t st = (hipStream_t)stream;
+    const int* nn = field->n;
+    if (o.tw && !field->cur32[0]) return set_err(FQSL_ERR_INVALID, "twist requested but the field has no current density");
     if (p->precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
-        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {field->n[0], field->n[1], field->n[2]}};
+        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) {
+            FieldSoA<double> fj{field->cur64[0], field->cur64[1], field->cur64[2], {nn[0], nn[1], nn[2]}};
+            return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n, fj);
+        }
         return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n);
     }
+    const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
     if (field->cell32) {
-        FieldCell32 f{field->cell32, {field->n[0], field->n[1], field->n[2]}};
+        FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, fj);
         return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
     }
     if (field->soa32[0]) {
-        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {field->n[0], field->n[1], field->n[2]}};
+        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, fj);
         return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
     }
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
+// ---------------------------------------------------- current density (curl)
+// curl B on the node grid of non-uniform axes (FastQSL.curl_of_B_grid,
+// fastqslpy/FastQSL.py:120-163): at an interior node the derivative along an
+// axis is the three-point rule
+//   ((c[+1] - c[0]) / dr * dl / (dr + dl) + (c[0] - c[-1]) / dl * dr / (dr + dl))
+// with dl, dr the spacings to the neighbours; boundary nodes take the value
+// of the nearest interior node (copy_boundary, :133-141, whose six plane
+// copies leave every boundary node with the interior value at the clamped
+// index).  The differences of B are taken in B's own type and everything
+// after in double, in the reference's operation order, so the results are
+// the same bits as its array arithmetic.
+#pragma clang fp contract(off)
+template <typename T>
+__device__ __forceinline__ double diff3(const T* __restrict__ c, long idx, long stride, const double* __restrict__ ax,
+                                        int i) {
+    const double dl = ax[i] - ax[i - 1], dr = ax[i + 1] - ax[i];
+    const T dp = c[idx + stride] - c[idx];
+    const T dm = c[idx] - c[idx - stride];
+    return (double)dp / dr * dl / (dr + dl) + (double)dm / dl * dr / (dr + dl);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_curl(const T* __restrict__ bx, const T* __restrict__ by,
+                                              const T* __restrict__ bz, int nx, int ny, int nz,
+                                              const double* __restrict__ ax, const double* __restrict__ ay,
+                                              const double* __restrict__ az, double* __restrict__ cx,
+                                              double* __restrict__ cy, double* __restrict__ cz) {
+    const long nxy = (long)nx * ny, nn = nxy * nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nn; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / nxy);
+        const int ci = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
+        const int cj = j < 1 ? 1 : (j > ny - 2 ? ny - 2 : j);
+        const int ck = k < 1 ? 1 : (k > nz - 2 ? nz - 2 : k);
+        const long c = (long)ck * nxy + (long)cj * nx + ci;
+        cx[t] = diff3(bz, c, nx, ay, cj) - diff3(by, c, nxy, az, ck);
+        cy[t] = diff3(bx, c, nxy, az, ck) - diff3(bz, c, 1, ax, ci);
+        cz[t] = diff3(by, c, 1, ax, ci) - diff3(bx, c, nx, ay, cj);
+    }
+}
+
+// diff_3point (FastQSL.py:143-163): the derivative along `axis` at every
+// interior node, out[(k-1)][(j-1)][(i-1)] (x fastest, (nx-2)(ny-2)(nz-2) values).
+template <typename T>
+__global__ void __launch_bounds__(256) k_diff3(const T* __restrict__ cube, int nx, int ny, int nz,
+                                               const double* __restrict__ ax, int axis, double* __restrict__ out) {
+    const long mx = nx - 2, my = ny - 2, mxy = mx * my, nn = mxy * (nz - 2), nxy = (long)nx * ny;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nn; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % mx) + 1, j = (int)((t / mx) % my) + 1, k = (int)(t / mxy) + 1;
+        const long c = (long)k * nxy + (long)j * nx + i;
+        out[t] = axis == 0 ? diff3(cube, c, 1, ax, i) : (axis == 1 ? diff3(cube, c, nx, ax, j) : diff3(cube, c, nxy, ax, k));
+    }
+}
+#pragma clang fp contract(on)
+
+// copy_boundary in place: every boundary node takes the clamped interior value.
+__global__ void __launch_bounds__(256) k_copy_boundary(double* __restrict__ a, int nx, int ny, int nz) {
+    const long nxy = (long)nx * ny, nn = nxy * nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nn; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / nxy);
+        const int ci = i < 1 ? 1 : (i > nx - 2 ? nx - 2 : i);
+        const int cj = j < 1 ? 1 : (j > ny - 2 ? ny - 2 : j);
+        const int ck = k < 1 ? 1 : (k > nz - 2 ? nz - 2 : k);
+        if (ci != i || cj != j || ck != k) a[t] = a[(long)ck * nxy + (long)cj * nx + ci];
+    }
+}
+
+static int check_grid3(int nx, int ny, int nz, int dtype) {
+    if (nx < 3 || ny < 3 || nz < 3) return set_err(FQSL_ERR_INVALID, "each axis needs >= 3 nodes (%d,%d,%d)", nx, ny, nz);
+    if (dtype != FQSL_DTYPE_F32 && dtype != FQSL_DTYPE_F64) return set_err(FQSL_ERR_INVALID, "dtype %d", dtype);
+    if ((long)nx * ny * nz > (1L << 31) - 1) return set_err(FQSL_ERR_INVALID, "grid exceeds int32 indexing");
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_curl(const void* bx, const void* by, const void* bz, int nx, int ny, int nz, int dtype,
+                         const double* ax, const double* ay, const double* az, double* cx, double* cy, double* cz,
+                         void* stream) {
+    int rc = check_grid3(nx, ny, nz, dtype);
+    if (rc) return rc;
+    if (!bx || !by || !bz || !ax || !ay || !az || !cx || !cy || !cz) return set_err(FQSL_ERR_INVALID, "NULL array");
+    hipStream_t st = (hipStream_t)stream;
+    const int g = grid_for((long)nx * ny * nz, 256);
+    if (dtype == FQSL_DTYPE_F32)
+        k_curl<float><<<g, 256, 0, st>>>((const float*)bx, (const float*)by, (const float*)bz, nx, ny, nz, ax, ay, az, cx,
+                                         cy, cz);
+    else
+        k_curl<double><<<g, 256, 0, st>>>((const double*)bx, (const double*)by, (const double*)bz, nx, ny, nz, ax, ay,
+                                          az, cx, cy, cz);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_diff_3point(const void* cube, int nx, int ny, int nz, int dtype, const double* ax, int axis,
+                                double* out, void* stream) {
+    int rc = check_grid3(nx, ny, nz, dtype);
+    if (rc) return rc;
+    if (axis < 0 || axis > 2) return set_err(FQSL_ERR_INVALID, "axis %d", axis);
+    if (!cube || !ax || !out) return set_err(FQSL_ERR_INVALID, "NULL array");
+    hipStream_t st = (hipStream_t)stream;
+    const int g = grid_for((long)(nx - 2) * (ny - 2) * (nz - 2), 256);
+    if (dtype == FQSL_DTYPE_F32) k_diff3<float><<<g, 256, 0, st>>>((const float*)cube, nx, ny, nz, ax, axis, out);
+    else k_diff3<double><<<g, 256, 0, st>>>((const double*)cube, nx, ny, nz, ax, axis, out);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_copy_boundary(double* a, int nx, int ny, int nz, void* stream) {
+    int rc = check_grid3(nx, ny, nz, FQSL_DTYPE_F64);
+    if (rc) return rc;
+    if (!a) return set_err(FQSL_ERR_INVALID, "NULL array");
+    k_copy_boundary<<<grid_for((long)nx * ny * nz, 256), 256, 0, (hipStream_t)stream>>>(a, nx, ny, nz);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
 }
 
 // ----------------------------------------------------------------- batches
@@ -1085,7 +1265,7 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
                     const int st = scott_init<R>(L, f, P, s.x[li], s.y[li], s.z[li], e,
                                                  frames + (unsigned long long)kFrameWords * idx);
                     if (st != kTracing) {
-                        recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                        recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                         batch_line_done(c, r);
                     } else {
                         active = true;
@@ -1101,7 +1281,7 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
             ScottEnd<R> e;
             const int st = scott_attempt<R>(L, f, P, e);
             if (st != kTracing) {
-                recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals};
+                recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                 active = false;
                 batch_line_done(c, batch_find(g.offs, g.r0, g.r1, idx));
             }
@@ -1143,6 +1323,7 @@ __global__ void __launch_bounds__(256) k_batch_finish(F f, BatchGroupArgs g, con
         if (o.qp) o.qp[li] = (float)qp;
         if (o.len) o.len[li] = rec.len;
         if (o.evals) o.evals[li] = rec.evals;
+        if (o.tw) o.tw[li] = rec.tw;
     }
 }
 
@@ -1200,6 +1381,7 @@ int check_request(const fqsl_scott_request& q, int precision, int k) {
     if (!(q.step > 0) || !(q.step <= 100.0)) return set_err(FQSL_ERR_INVALID, "request %d: step %g outside (0, 100]", k, q.step);
     if (q.method == FQSL_METHOD_RKF45 && !(q.tol > 0)) return set_err(FQSL_ERR_INVALID, "request %d: tol %g must be > 0", k, q.tol);
     if (q.n && (!q.x || !q.y || !q.z)) return set_err(FQSL_ERR_INVALID, "request %d: NULL seed array", k);
+    if (q.out.twist) return set_err(FQSL_ERR_UNSUPPORTED, "request %d: twist is not available in batches", k);
     const fqsl_scott_out& o = q.out;
     if ((o.start_x && (!o.start_y || !o.start_z)) || (o.end_x && (!o.end_y || !o.end_z)) ||
         (o.b_this_x && (!o.b_this_y || !o.b_this_z)))
@@ -1290,7 +1472,7 @@ extern "C" int fqsl_scott_batch_create(const fqsl_scott_request* reqs, int nreq,
         table[k] = BatchReq{Seeds{q.x, q.y, q.z},
                             ScottOut{o.start_x, o.start_y, o.start_z, o.end_x, o.end_y, o.end_z, o.flag_start,
                                      o.flag_end, o.b_this_x, o.b_this_y, o.b_this_z, o.q_0, o.q_perp, o.line_len,
-                                     o.evals},
+                                     o.evals, nullptr},
                             q.step, q.method == FQSL_METHOD_RKF45 ? q.tol : 0.0, q.n};
         lines += q.n;
     }
@@ -1389,23 +1571,32 @@ extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params
     PlainOut o{out->start_x, out->start_y, out->start_z, out->end_x, out->end_y, out->end_z,
                out->flag_start, out->flag_end, out->b_this_x, out->b_this_y, out->b_this_z, out->b_flag,
                out->b_start_x, out->b_start_y, out->b_start_z, out->b_end_x, out->b_end_y, out->b_end_z,
-               out->line_len, out->evals};
+               out->line_len, out->evals, out->twist};
     if ((o.sx && (!o.sy || !o.sz)) || (o.ex && (!o.ey || !o.ez)) || (o.btx && (!o.bty || !o.btz)) ||
         (o.bsx && (!o.bsy || !o.bsz)) || (o.bex && (!o.bey || !o.bez)))
         return set_err(FQSL_ERR_INVALID, "x/y/z output arrays must be all set or all NULL");
     DevScalars ds{};
     hipStream_t st = (hipStream_t)stream;
+    const int* nn = field->n;
+    if (o.tw && !field->cur32[0]) return set_err(FQSL_ERR_INVALID, "twist requested but the field has no current density");
     if (p->precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
-        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {field->n[0], field->n[1], field->n[2]}};
+        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) {
+            FieldSoA<double> fj{field->cur64[0], field->cur64[1], field->cur64[2], {nn[0], nn[1], nn[2]}};
+            return launch_plain<double>(f, P, s, n, o, ds, nullptr, p->schedule, st, n, fj);
+        }
         return launch_plain<double>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
     }
+    const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
     if (field->cell32) {
-        FieldCell32 f{field->cell32, {field->n[0], field->n[1], field->n[2]}};
+        FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n, fj);
         return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
     }
     if (field->soa32[0]) {
-        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {field->n[0], field->n[1], field->n[2]}};
+        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n, fj);
         return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
     }
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
@@ -1424,9 +1615,16 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
                                        int* flag_end, float* B_this_x, float* B_this_y, float* B_this_z, float* q_0,
                                        float* q_perp, float* s_len, unsigned long long* N, double* LineLen,
                                        float* tol_coef, int precision, int schedule, void* stream) {
-    (void)curB_x; (void)curB_y; (void)curB_z; (void)inp_cross_dir;
+    (void)inp_cross_dir;
     if (!Bx || !By || !Bz || !BshapeN || !inp_x || !inp_y || !inp_z || !s_len || !N || !tol_coef)
         return set_err(FQSL_ERR_INVALID, "NULL required argument");
+    bool twist_on = false;
+    if (curB_flag) {  // the switch is a device scalar: read it (synchronises the stream)
+        HIP_TRY(hipMemcpyAsync(&twist_on, curB_flag, sizeof(bool), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    if (twist_on && (!curB_x || !curB_y || !curB_z || !twist))
+        return set_err(FQSL_ERR_INVALID, "curB_flag is set: curB_x/y/z and twist must be device arrays");
     int rc = check_common(FQSL_METHOD_RKF45, precision, schedule);
     if (rc) return rc;
     if (precision != FQSL_PREC_FP32C)
@@ -1434,7 +1632,7 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
     ScottParams P{FQSL_METHOD_RKF45, 0.0, 0.0, 0.0};
     Seeds s{inp_x, inp_y, inp_z};
     ScottOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
-               q_0, q_perp, LineLen, nullptr};
+               q_0, q_perp, LineLen, nullptr, twist_on ? twist : nullptr};
     DevScalars ds{BshapeN, s_len, N, tol_coef, curB_flag, twist};
     FieldSoA<float> f{Bx, By, Bz, {2, 2, 2}};
     unsigned long long n_hint = (unsigned long long)kPerLineMaxBlocks * 256;
@@ -1442,6 +1640,10 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
         // the persistent path sizes its workspace from N: read it (synchronises the stream)
         HIP_TRY(hipMemcpyAsync(&n_hint, N, sizeof(n_hint), hipMemcpyDeviceToHost, (hipStream_t)stream));
         HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    if (twist_on) {
+        const FieldSoA<float> fj{curB_x, curB_y, curB_z, {2, 2, 2}};  // shape taken from B's on the device
+        return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream, n_hint, fj);
     }
     return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream, n_hint);
 }
@@ -1454,9 +1656,15 @@ extern "C" int fqsl_TraceAllBline(int method, float* Bx, float* By, float* Bz, i
                                   float* B_start_x, float* B_start_y, float* B_start_z, float* B_end_x,
                                   float* B_end_y, float* B_end_z, float* s_len, unsigned long long* N,
                                   double* LineLen, float* tol_coef, int precision, int schedule, void* stream) {
-    (void)curB_x; (void)curB_y; (void)curB_z;
     if (!Bx || !By || !Bz || !BshapeN || !inp_x || !inp_y || !inp_z || !inp_cross_dir || !s_len || !N)
         return set_err(FQSL_ERR_INVALID, "NULL required argument");
+    bool twist_on = false;
+    if (curB_flag) {  // the switch is a device scalar: read it (synchronises the stream)
+        HIP_TRY(hipMemcpyAsync(&twist_on, curB_flag, sizeof(bool), hipMemcpyDeviceToHost, (hipStream_t)stream));
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    }
+    if (twist_on && (!curB_x || !curB_y || !curB_z || !twist))
+        return set_err(FQSL_ERR_INVALID, "curB_flag is set: curB_x/y/z and twist must be device arrays");
     if (method == FQSL_METHOD_RKF45 && !tol_coef) return set_err(FQSL_ERR_INVALID, "RKF45 needs tol_coef");
     int rc = check_common(method, precision, schedule);
     if (rc) return rc;
@@ -1467,11 +1675,16 @@ extern "C" int fqsl_TraceAllBline(int method, float* Bx, float* By, float* Bz, i
     P.ncross[0] = P.ncross[1] = P.ncross[2] = 0.f;
     Seeds s{inp_x, inp_y, inp_z};
     PlainOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
-               B_flag, B_start_x, B_start_y, B_start_z, B_end_x, B_end_y, B_end_z, LineLen, nullptr};
+               B_flag, B_start_x, B_start_y, B_start_z, B_end_x, B_end_y, B_end_z, LineLen, nullptr,
+               twist_on ? twist : nullptr};
     DevScalars ds{BshapeN, s_len, N, method == FQSL_METHOD_RKF45 ? tol_coef : nullptr, curB_flag, twist};
     FieldSoA<float> f{Bx, By, Bz, {2, 2, 2}};
-    return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, (hipStream_t)stream,
-                               (unsigned long long)kPerLineMaxBlocks * 256);
+    const unsigned long long n_hint = (unsigned long long)kPerLineMaxBlocks * 256;
+    if (twist_on) {
+        const FieldSoA<float> fj{curB_x, curB_y, curB_z, {2, 2, 2}};  // shape taken from B's on the device
+        return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, (hipStream_t)stream, n_hint, fj);
+    }
+    return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, (hipStream_t)stream, n_hint);
 }
 
 // ---------------------------------------------- neighbour-difference assembly

@@ -44,6 +44,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define FQSL_HD __host__ __device__ __forceinline__
 #define FQSL_COLD __host__ __device__ __attribute__((noinline))
 
@@ -489,6 +491,27 @@ FQSL_HD void b_clamped(const F& f, const float* x, float out[3]) {
     out[0] = (float)B[0]; out[1] = (float)B[1]; out[2] = (float)B[2];
 }
 
+// ------------------------------------------------------------------ twist
+// Current-density field for the twist number: none (the tracers' default).
+struct NoCurrent {
+    int n[3];
+};
+
+template <class FJ>
+constexpr bool has_current() { return !std::is_same<FJ, NoCurrent>::value; }
+
+constexpr double kQuarterOverPi = 0.25 / 3.14159265358979323846;
+
+// J.B / |B|^2 / 4pi at x from the raw interpolants of J and B
+// (TraceBlineScott.cu:424-426, TraceBline.cu:326-328); B is returned too.
+template <typename R, class F, class FJ>
+FQSL_HD double twist_rate(const F& f, const FJ& fj, const double* x, double B[3]) {
+    double J[3];
+    raw_b<R>(fj, x, J);
+    raw_b<R>(f, x, B);
+    return dot3(J, B) / dot3(B, B) * kQuarterOverPi;
+}
+
 // checkFlag (TraceBlineScott.cu:310-332) in double.
 FQSL_HD int check_flag(const int* n, const double* p) {
     const double X = p[0], Y = p[1], Z = p[2];
@@ -624,6 +647,7 @@ struct ScottLine {
     Pos<R> x;
     R u[3], v[3];
     double s, len;
+    double tw;                 // twist accumulated over both directions
     int steps, attempts, dir;  // dir: -1 backward (first), +1 forward
     uint32_t evals;
     ScottEnd<R> st;            // start (backward) footpoint once known
@@ -725,6 +749,7 @@ FQSL_HD int scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float 
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
     L.fr = frame;
     L.len = 0.0;
+    L.tw = 0.0;
     L.steps = 0; L.attempts = 0; L.evals = 0;
     L.s = P.s0;
     L.dir = -1;
@@ -768,8 +793,9 @@ FQSL_HD int scott_side_done(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
 // One integrator attempt (one pass of the while loop at
 // TraceBlineScott.cu:407-470).  Returns kTracing, kFinished (both footpoints
 // known: start in L.st, end in e) or kAborted (class 42: Q = 0).
-template <typename R, class F>
-FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, ScottEnd<R>& e) {
+template <typename R, class F, class FJ = NoCurrent>
+FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, ScottEnd<R>& e,
+                          const FJ& fj = FJ{}) {
     const R sc = (R)(L.s * L.dir);
     R inc[9];
     L.attempts += 1;
@@ -854,9 +880,16 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
     const double dL = sizeof(R) == 4 ? (double)sqrtf((float)(inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2]))
                                      : sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
     L.len += dL;
+    // twist at the step's end point (TraceBlineScott.cu:423-427)
+    double tw_rate = 0.0, Bend[3] = {0.0, 0.0, 0.0};
+    if constexpr (has_current<FJ>()) {
+        tw_rate = twist_rate<R>(f, fj, xbv, Bend);
+        L.tw += tw_rate * dL;
+    }
     const int flag = check_flag(f.n, xbv);
     if (flag > 0) {
         L.len -= dL;
+        if constexpr (has_current<FJ>()) L.tw -= tw_rate * dL;  // :432-434
         if (flag <= 6) {
             const int dim = (flag - 1) / 2;
             const double p1 = xa[dim], p2 = xbv[dim];
@@ -873,11 +906,22 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
                     ul[a] = (R)((double)L.u[a] * wa + (double)(L.u[a] + inc[3 + a]) * wb);
                     vl[a] = (R)((double)L.v[a] * wa + (double)(L.v[a] + inc[6 + a]) * wb);
                 }
-                // arc length to the face with b at the last in-box state
-                R ba[3];
-                const R zero3[3] = {0, 0, 0};
-                unit_b<R>(f, L.x.stage(zero3), ba);
-                L.len += fabs(pm - p1) / (1e-4 + fabs((double)ba[dim]));
+                // arc length to the face: with twist on, the reference's B_P1
+                // is the raw B at the step's end point (:425) and also scales
+                // the landing twist (:455-457); with it off B_P1 is unset and b
+                // at the last in-box state is used
+                double bdim;
+                if constexpr (has_current<FJ>()) {
+                    bdim = Bend[dim];
+                } else {
+                    R ba[3];
+                    const R zero3[3] = {0, 0, 0};
+                    unit_b<R>(f, L.x.stage(zero3), ba);
+                    bdim = (double)ba[dim];
+                }
+                const double arc = fabs(pm - p1) / (1e-4 + fabs(bdim));
+                L.len += arc;
+                if constexpr (has_current<FJ>()) L.tw += tw_rate * arc;
                 scott_record(L, e, xl, ul, vl, flag);
             } else {
                 scott_record(L, e, xa, L.u, L.v, flag);
@@ -971,6 +1015,7 @@ struct PlainLine {
     int steps, attempts, dir, step_lim;  // dir +1 forward (first), -1 backward
     int flag_e, flag_s;
     double xe[3], xs[3];
+    double tw;  // twist, forward plus backward
     bool done;
     uint32_t evals;
 };
@@ -1016,6 +1061,7 @@ FQSL_HD void plain_init(PlainLine<R>& L, const F& f, const PlainParams& P, float
     L.len_lim = (rkf ? kPlainRkf45LenRatio : kPlainRk4LenRatio) * (f.n[0] + f.n[1] + f.n[2]);
     L.step_lim = rkf ? kPlainRkf45StepLim : kPlainRk4StepLim;
     L.len_total = 0.0;
+    L.tw = 0.0;
     L.evals = 0;
     L.done = false;
     L.flag_e = L.flag_s = kClassLimit;
@@ -1057,8 +1103,8 @@ FQSL_COLD void rk4_boundary(const F f, const double* p, double s, int dim, doubl
     for (int a = 0; a < 3; ++a) out[a] = p[a] + (1. / 6.) * s * (k[0][a] + 2.0 * k[1][a] + 2.0 * k[2][a] + k[3][a]);
 }
 
-template <typename R, class F>
-FQSL_HD void plain_attempt(PlainLine<R>& L, const F& f, const PlainParams& P) {
+template <typename R, class F, class FJ = NoCurrent>
+FQSL_HD void plain_attempt(PlainLine<R>& L, const F& f, const PlainParams& P, const FJ& fj = FJ{}) {
     const R sc = (R)(L.s * L.dir);
     R inc[3];
     L.attempts += 1;
@@ -1140,9 +1186,20 @@ FQSL_HD void plain_attempt(PlainLine<R>& L, const F& f, const PlainParams& P) {
     const double xbv[3] = {xb.get(0), xb.get(1), xb.get(2)};
     const double dl = sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
     L.len += dl;
+    // twist at the step's start point (TraceBline.cu:325-329,
+    // TraceBlineAdaptive.cu:373-377)
+    double tw_rate = 0.0, Jp1[3] = {0.0, 0.0, 0.0};
+    if constexpr (has_current<FJ>()) {
+        double Bp1[3];
+        raw_b<R>(fj, xa, Jp1);
+        raw_b<R>(f, xa, Bp1);
+        tw_rate = dot3(Jp1, Bp1) / dot3(Bp1, Bp1) * kQuarterOverPi;
+        L.tw += tw_rate * dl;
+    }
     const int flag = check_flag(f.n, xbv);
     if (flag > 0) {
         L.len -= dl;
+        if constexpr (has_current<FJ>()) L.tw -= tw_rate * dl;  // TraceBline.cu:334-336
         if (flag <= 6) {
             const int dim = (flag - 1) / 2;
             const double pa = xa[dim], pb = xbv[dim];
@@ -1160,7 +1217,17 @@ FQSL_HD void plain_attempt(PlainLine<R>& L, const F& f, const PlainParams& P) {
                     L.evals += 4;
                 }
                 plain_record(L, out, flag);
-                L.len += fabs(pm - pa) / (1e-4 + fabs((double)b1[dim]));
+                const double arc = fabs(pm - pa) / (1e-4 + fabs((double)b1[dim]));
+                L.len += arc;
+                // RK4 only (the RKF45 tracer's term is commented out): J.b/|b|^2
+                // with b the normalised B at p1, as the reference re-assigns
+                // B_P1 before it (TraceBline.cu:362-365)
+                if constexpr (has_current<FJ>()) {
+                    if (P.method == kRK4) {
+                        const double bd[3] = {(double)b1[0], (double)b1[1], (double)b1[2]};
+                        L.tw += dot3(Jp1, bd) / dot3(bd, bd) * kQuarterOverPi * arc;
+                    }
+                }
             } else {
                 plain_record(L, xa, flag);
             }

@@ -16,19 +16,43 @@ def _host_xfast(b, dtype):
     return np.ascontiguousarray(a.transpose(2, 1, 0), dtype=dtype)
 
 
+def _device_xfast(comps, dtype=None):
+    """Three (nx, ny, nz) arrays -> (pointers, dtype code, on-device flag,
+    buffers to keep alive until the library has copied them)."""
+    torch = _maybe_torch(comps[0])
+    if torch is not None and comps[0].is_cuda:
+        cs = [t.permute(2, 1, 0).contiguous() for t in comps]
+        if dtype is not None:
+            cs = [c.to(getattr(torch, np.dtype(dtype).name)) for c in cs]
+        if cs[0].dtype not in (torch.float32, torch.float64):
+            cs = [c.float() for c in cs]
+        dt = _lib.DTYPE_F64 if cs[0].dtype == torch.float64 else _lib.DTYPE_F32
+        return [c.data_ptr() for c in cs], dt, 1, cs
+    if torch is not None:
+        comps = [t.detach().cpu().numpy() for t in comps]
+    src_dt = np.dtype(dtype) if dtype is not None else np.asarray(comps[0]).dtype
+    if src_dt not in (np.float32, np.float64):
+        src_dt = np.dtype(np.float32)
+    cs = [_host_xfast(b, src_dt) for b in comps]
+    dt = _lib.DTYPE_F64 if src_dt == np.float64 else _lib.DTYPE_F32
+    return [c.ctypes.data for c in cs], dt, 0, cs
+
+
 class Field:
     """Bx, By, Bz of shape (nx, ny, nz) (x fastest in memory, the layout the
     reference gets from ``np.transpose(sav, (2, 1, 0))``), uploaded to the
     current device with the requested layouts:
 
     * ``soa64`` three double arrays (FP64 tracer),
-    * ``cell32`` 128-byte cell stencils (FP32 tracer),
+    * ``cell32`` 128-byte cells of trilinear coefficients (FP32 tracer),
     * ``soa32`` three float arrays.
 
+    ``current`` = (Jx, Jy, Jz) on the same grid (e.g. from
+    ``FastQSL.curl_of_B_grid``) enables the twist output of the tracers.
     Accepts NumPy arrays or torch tensors (CPU or GPU, shape (nx, ny, nz)).
     """
 
-    def __init__(self, bx, by, bz, layouts=("soa64", "cell32"), dtype=None):
+    def __init__(self, bx, by, bz, layouts=("soa64", "cell32"), dtype=None, current=None):
         L = _lib.lib()
         mask = 0
         for name in layouts:
@@ -36,34 +60,23 @@ class Field:
                      "cell32": _lib.LAYOUT_CELL32}[name]
         self.layouts = tuple(layouts)
         self._h = ctypes.c_void_p()
-        self._keep = None
-        torch = _maybe_torch(bx)
-        if torch is not None and bx.is_cuda:
-            comps = [t.permute(2, 1, 0).contiguous() for t in (bx, by, bz)]
-            if dtype is not None:
-                comps = [c.to(getattr(torch, np.dtype(dtype).name)) for c in comps]
-            if comps[0].dtype not in (torch.float32, torch.float64):
-                comps = [c.float() for c in comps]
-            dt = _lib.DTYPE_F64 if comps[0].dtype == torch.float64 else _lib.DTYPE_F32
-            shape = tuple(bx.shape)
-            ptrs = [c.data_ptr() for c in comps]
-            on_dev = 1
-            self._keep = comps
-        else:
-            if torch is not None:
-                bx, by, bz = (t.detach().cpu().numpy() for t in (bx, by, bz))
-            src_dt = np.dtype(dtype) if dtype is not None else np.asarray(bx).dtype
-            if src_dt not in (np.float32, np.float64):
-                src_dt = np.dtype(np.float32)
-            comps = [_host_xfast(b, src_dt) for b in (bx, by, bz)]
-            dt = _lib.DTYPE_F64 if src_dt == np.float64 else _lib.DTYPE_F32
-            shape = np.asarray(bx).shape
-            ptrs = [c.ctypes.data for c in comps]
-            on_dev = 0
-        self.shape = tuple(int(s) for s in shape)
+        self.has_current = False
+        self.shape = tuple(int(v) for v in np.shape(bx))
+        ptrs, dt, on_dev, keep = _device_xfast((bx, by, bz), dtype)
         _lib.check(L.fqsl_field_create(ptrs[0], ptrs[1], ptrs[2], *self.shape, dt, on_dev, mask,
                                        ctypes.byref(self._h)))
-        self._keep = None
+        del keep
+        if current is not None:
+            self.set_current(*current)
+
+    def set_current(self, jx, jy, jz):
+        """Attach the current density J (same grid as B) for the twist number."""
+        if tuple(int(v) for v in np.shape(jx)) != self.shape:
+            raise ValueError(f"current of shape {np.shape(jx)} on a field of shape {self.shape}")
+        ptrs, dt, on_dev, keep = _device_xfast((jx, jy, jz))
+        _lib.check(_lib.lib().fqsl_field_set_current(self.handle, ptrs[0], ptrs[1], ptrs[2], dt, on_dev))
+        del keep
+        self.has_current = True
 
     @property
     def handle(self):

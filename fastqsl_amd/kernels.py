@@ -15,7 +15,9 @@ as CuPy arrays are) or NumPy arrays (copied to the device and the outputs
 copied back into them); scalars are one-element arrays as in the reference.
 Memory is read as laid out, exactly like the reference: field arrays must
 hold x fastest (the ``np.transpose(sav, (2, 1, 0))`` layout).  ``grid`` and
-``block`` are accepted and ignored -- the library sizes its own launch.
+``block`` are accepted and ignored -- the library sizes its own launch.  With
+``curB_flag`` set, ``curB_x/y/z`` (float32, B's layout) give the current
+density and ``twist`` (float64, one per line) receives the twist number.
 
 Nothing is compiled at run time: the kernels are in the prebuilt
 ``libfqsl.so``; these functions only bind it.
@@ -111,10 +113,11 @@ class _Kernel:
             raise TypeError(f"{self.name} takes {n_expected} arguments ({len(args)} given)")
         flag = args[8]
         flag_val = bool(flag.reshape(-1)[0].item()) if hasattr(flag, "reshape") else bool(flag)
-        if flag_val:
-            raise NotImplementedError("twist (curB_flag = True) is not implemented in fastqsl_amd")
-        twist_i = 7  # dummy arrays of any dtype are accepted for twist, as the notebooks pass them
-        args = args[:twist_i] + (np.zeros(1, np.float64),) + args[twist_i + 1:]
+        if not flag_val:
+            # twist off: the notebooks pass float32 dummies for curB and twist,
+            # which the library never reads
+            twist_i = 7
+            args = args[:twist_i] + (np.zeros(1, np.float64),) + args[twist_i + 1:]
         b = _Binding(self.params, args)
         st = stream if stream is not None else torch.cuda.current_stream()
         L = _lib.lib()

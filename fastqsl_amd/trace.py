@@ -18,6 +18,8 @@ SCOTT_OUTPUTS = ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z", "fl
 PLAIN_OUTPUTS = ("start_x", "start_y", "start_z", "end_x", "end_y", "end_z", "flag_start",
                  "flag_end", "b_this_x", "b_this_y", "b_this_z", "b_flag", "b_start_x",
                  "b_start_y", "b_start_z", "b_end_x", "b_end_y", "b_end_z", "line_len", "evals")
+# requested explicitly (``outputs=... + ("twist",)``); needs Field(..., current=...)
+OPTIONAL_OUTPUTS = ("twist",)
 
 
 def _torch():
@@ -29,7 +31,7 @@ def _dtype_of(name):
     torch = _torch()
     if name.startswith("flag") or name == "b_flag":
         return torch.int32
-    if name == "line_len":
+    if name in ("line_len", "twist"):
         return torch.float64
     if name == "evals":
         return torch.int32  # uint32 on the device; counts stay far below 2^31
@@ -70,11 +72,12 @@ def trace_scott(field: Field, x, y, z, method="rkf45", step=0.25, tol=1e-6, prec
     n = xs.numel()
     res = dict(out) if out is not None else {}
     for name in outputs:
-        if name not in SCOTT_OUTPUTS:
+        if name not in SCOTT_OUTPUTS + OPTIONAL_OUTPUTS:
             raise ValueError(f"unknown output {name}")
         if name not in res:
             res[name] = torch.empty(n, dtype=_dtype_of(name), device=xs.device)
-    o = _lib.fqsl_scott_out(**{k: (res[k].data_ptr() if k in res else None) for k in SCOTT_OUTPUTS})
+    o = _lib.fqsl_scott_out(**{k: (res[k].data_ptr() if k in res and k in outputs else None)
+                               for k in SCOTT_OUTPUTS + OPTIONAL_OUTPUTS})
     p = _lib.fqsl_scott_params(method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision],
                                schedule=_lib.SCHEDULES[schedule], reserved=0, step=float(step),
                                tol=float(tol))
@@ -95,11 +98,12 @@ def trace_plain(field: Field, x, y, z, ncross=(0.0, 0.0, 1.0), method="rkf45", s
     n = xs.numel()
     res = dict(out) if out is not None else {}
     for name in outputs:
-        if name not in PLAIN_OUTPUTS:
+        if name not in PLAIN_OUTPUTS + OPTIONAL_OUTPUTS:
             raise ValueError(f"unknown output {name}")
         if name not in res:
             res[name] = torch.empty(n, dtype=_dtype_of(name), device=xs.device)
-    o = _lib.fqsl_plain_out(**{k: (res[k].data_ptr() if k in res else None) for k in PLAIN_OUTPUTS})
+    o = _lib.fqsl_plain_out(**{k: (res[k].data_ptr() if k in res and k in outputs else None)
+                               for k in PLAIN_OUTPUTS + OPTIONAL_OUTPUTS})
     nc = (ctypes.c_float * 3)(*[float(v) for v in ncross])
     p = _lib.fqsl_plain_params(method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision],
                                schedule=_lib.SCHEDULES[schedule], reserved=0, step=float(step),

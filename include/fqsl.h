@@ -74,6 +74,27 @@ int fqsl_field_destroy(fqsl_field *field);
 /* Device bytes held by the field's layouts. */
 size_t fqsl_field_bytes(const fqsl_field *field);
 int fqsl_field_shape(const fqsl_field *field, int *nx, int *ny, int *nz);
+/* Attach the current density J (three arrays on the field's grid, x fastest)
+ * for the twist number: the tracers' curB_x/y/z (TraceBline.cu:301).  Kept in
+ * float and double.  Synchronous; replaces an earlier one. */
+int fqsl_field_set_current(fqsl_field *field, const void *jx, const void *jy, const void *jz, int dtype,
+                           int on_device);
+
+/* ----------------------------------------------------- current density */
+/* curl B on the node grid of axes ax/ay/az (device double arrays of nx, ny,
+ * nz coordinates), FastQSL.curl_of_B_grid (fastqslpy/FastQSL.py:120-131):
+ * three-point differences on non-uniform spacing at interior nodes, boundary
+ * nodes copied from the nearest interior node.  B in dtype, curl in double;
+ * all arrays x fastest, nx, ny, nz >= 3. */
+int fqsl_curl(const void *bx, const void *by, const void *bz, int nx, int ny, int nz, int dtype,
+              const double *ax, const double *ay, const double *az, double *cx, double *cy, double *cz,
+              void *stream);
+/* diff_3point (FastQSL.py:143-163): derivative along axis (0, 1, 2) at the
+ * interior nodes, out of (nx-2)(ny-2)(nz-2) doubles, x fastest. */
+int fqsl_diff_3point(const void *cube, int nx, int ny, int nz, int dtype, const double *ax, int axis,
+                     double *out, void *stream);
+/* copy_boundary (FastQSL.py:133-141) in place on a double array. */
+int fqsl_copy_boundary(double *a, int nx, int ny, int nz, void *stream);
 
 /* ------------------------------------------------------------ Scott tracer */
 typedef struct {
@@ -96,6 +117,8 @@ typedef struct {
     float *q_0, *q_perp;                /* TraceBlineScott.cu:495-508 */
     double *line_len;
     uint32_t *evals;                    /* RHS evaluations spent on the line */
+    double *twist;                      /* twist number (needs the field's current density,
+                                           fqsl_field_set_current; TraceBlineScott.cu:423-457) */
 } fqsl_scott_out;
 
 /* Trace n seeds (device float arrays x, y, z) with the Scott method and
@@ -171,6 +194,8 @@ typedef struct {
     float *b_end_x, *b_end_y, *b_end_z;
     double *line_len;
     uint32_t *evals;
+    double *twist;   /* twist number, forward plus backward (TraceBline.cu:325-365 for RK4,
+                        TraceBlineAdaptive.cu:373-384 for RKF45); needs the current density */
 } fqsl_plain_out;
 
 /* Position-only tracing of n seeds, forward then backward.  Replaces a launch
@@ -214,8 +239,9 @@ int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, co
 /* Same parameters, order and meaning as the CUDA kernels the fastqslpy host
  * code launches (all device pointers; scalars as one-element device arrays),
  * plus the stream.  The field arrays are read in place (SoA float layout).
- * Twist (curB_flag set) is not implemented: callers must pass curB_flag false;
- * twist[] and the curB arrays are never accessed. */
+ * With curB_flag set (read from the device: one stream synchronisation) the
+ * twist number is written to twist[] from the current density curB_x/y/z
+ * (float, same layout as B); with it clear, twist[] and curB are not read. */
 
 /* TraceAllBlineScott (TraceBlineScott.cu:521-528).  tol_coef scales the
  * per-step tolerance 1e-3 * tol_coef.  precision / schedule select the path;

@@ -31,7 +31,15 @@
  *   - Scott tolerance is an explicit absolute tolerance (TraceBlineScott.cu:378
  *     multiplies an uninitialised tol_this).
  *   - The Scott landing arc-length term uses the unit vector b at the last
- *     in-box state (TraceBlineScott.cu:448 reads an uninitialised B_P1).
+ *     in-box state (TraceBlineScott.cu:448 reads an uninitialised B_P1) when
+ *     twist is off; with twist on B_P1 is the raw B at the step's end point
+ *     (:425), which the landing length and twist terms then use.
+ *   - Twist (J.B/|B|^2/4pi along the line, TraceBline.cu:325-365,
+ *     TraceBlineAdaptive.cu:373-384, TraceBlineScott.cu:423-457): a face exit
+ *     whose step has no extent along the exit axis adds no landing twist (the
+ *     reference's p_mid is unset there); the RKF45 plain tracer adds none at
+ *     all (its landing term is commented out); a seed below z=-1e-8 has
+ *     twist 0.
  *   - A direction that stops on the length or step limit keeps class 0 and its
  *     last state as endpoint; the line's Q is NaN.
  *   - A seed below z=-1e-8 gets class -1 on both sides and Q = NaN.
@@ -46,6 +54,10 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -282,10 +294,19 @@ static void scott_rk4(const field_t *f, const state9 *y, state9 *out, double s, 
 typedef struct {
     double start[3], end[3];
     int flag_start, flag_end;
-    double q0, qperp, len;
+    double q0, qperp, len, twist;
     double bthis[3];
     long evals, steps;
 } scott_result;
+
+/* J.B / |B|^2 / 4 pi at p from the current-density field cur and B (both the
+ * raw Interp3dxyzn, TraceBlineScott.cu:424-426), and that B. */
+static double twist_density(const field_t *f, const field_t *cur, const double *p, double *Bout) {
+    double J[3];
+    interp_vec(cur, p, J, 0);
+    interp_vec(f, p, Bout, 0);
+    return dot3(J, Bout) / dot3(Bout, Bout) / 4.0 / M_PI;
+}
 
 static void normalize3(double *a) {
     double n = len3(a);
@@ -293,8 +314,8 @@ static void normalize3(double *a) {
 }
 
 /* TraceBlineScott (TraceBlineScott.cu:334-519) with the Q assembly at :475-508. */
-static void trace_scott_line(const field_t *f, const double *p0, int method, double s0,
-                             double tol, int gm, scott_result *res) {
+static void trace_scott_line(const field_t *f, const field_t *cur, const double *p0, int method,
+                             double s0, double tol, int gm, scott_result *res) {
     const int *n = f->n;
     double len_lim = SCOTT_LEN_RATIO * (n[0] + n[1] + n[2]);
     double B0[3], b0[3], Nb0, u0[3], v0[3];
@@ -303,7 +324,9 @@ static void trace_scott_line(const field_t *f, const double *p0, int method, dou
     int z0, dir, flags[2] = {0, 0};
     state9 ends[2];
     int aborted = 0, abort_side = -1;
+    double twist = 0.0;
 
+    res->twist = 0.0;
     interp_clamped(f, p0, res->bthis);           /* TraceBlineScott.cu:577-579 */
     memset(ends, 0, sizeof(ends));
     if (p0[2] < -1e-8) {                           /* TraceBlineScott.cu:366 */
@@ -364,22 +387,39 @@ static void trace_scott_line(const field_t *f, const double *p0, int method, dou
             d[0] = b.x[0] - a.x[0]; d[1] = b.x[1] - a.x[1]; d[2] = b.x[2] - a.x[2];
             dL = len3(d);
             len += dL;
+            /* twist: J.B/|B|^2/4pi at the step's end point times its length
+             * (TraceBlineScott.cu:423-427) */
+            double tw_rate = 0.0, Bend[3] = {0.0, 0.0, 0.0};
+            if (cur) {
+                tw_rate = twist_density(f, cur, b.x, Bend);
+                twist += tw_rate * dL;
+            }
             flag = check_flag(n, b.x);
             if (flag > 0) {
                 len -= dL;
+                if (cur) twist -= tw_rate * dL;          /* :432-434 */
                 if (flag <= 6) {
                     int dim = (flag - 1) / 2;
                     double p1 = a.x[dim], p2 = b.x[dim];
                     if (fabs(p1 - p2) > 0) {
                         double pm = (flag % 2 == 1) ? 0.0 : (double)n[dim];
-                        double ba[3];
+                        double ba[3], arc;
                         int c;
                         const double *pa = (const double *)&a, *pb = (const double *)&b;
                         double *pe = (double *)&ends[side];
                         for (c = 0; c < 9; ++c) pe[c] = (pa[c] * (p2 - pm) + pb[c] * (pm - p1)) / (p2 - p1);
-                        interp_vec(f, a.x, ba, 1);
-                        len += fabs(pm - p1) / (1e-4 + fabs(ba[dim]));
+                        /* the landing term's B: with twist on, the reference's
+                         * B_P1 is the raw B at the step's end point (:425); with
+                         * it off B_P1 is never set and b at the last in-box state
+                         * is used (decision register) */
+                        if (cur) memcpy(ba, Bend, sizeof(ba));
+                        else interp_vec(f, a.x, ba, 1);
+                        arc = fabs(pm - p1) / (1e-4 + fabs(ba[dim]));
+                        len += arc;
+                        if (cur) twist += tw_rate * arc;  /* :455-457 */
                     } else {
+                        /* no landing segment: the reference's p_mid is unset
+                         * here; no length and no twist are added */
                         ends[side] = a;
                     }
                     flags[side] = flag;
@@ -447,6 +487,7 @@ static void trace_scott_line(const field_t *f, const double *p0, int method, dou
     res->flag_start = flags[0];
     res->flag_end = flags[1];
     res->len = len;
+    res->twist = twist;
     res->evals = evals;
     res->steps = steps;
 }
@@ -506,15 +547,18 @@ static void rk4_boundary(const field_t *f, const double *p, double *out, double 
 }
 
 /* TraceBlineAdap (TraceBlineAdaptive.cu:333-426) / TraceBline
- * (TraceBline.cu:300-378), twist off, one direction. */
-static void trace_plain_dir(const field_t *f, const double *p0, int method, double s,
+ * (TraceBline.cu:300-378), one direction; with cur, the twist
+ * J.B/|B|^2/4pi integrated along the accepted steps (at each step's start
+ * point, TraceBline.cu:325-329, TraceBlineAdaptive.cu:373-377). */
+static void trace_plain_dir(const field_t *f, const field_t *cur, const double *p0, int method, double s,
                             double tol_coef, const double *ncross, double dir,
-                            double *pout, int *flag_out, double *len_out, long *evals_out) {
+                            double *pout, int *flag_out, double *len_out, long *evals_out,
+                            double *twist_out) {
     const int *n = f->n;
     double ratio = method == M_RKF45 ? RKF45_PLAIN_LEN_RATIO : RK4_PLAIN_LEN_RATIO;
     long step_lim = method == M_RKF45 ? RKF45_PLAIN_STEP_LIM : RK4_PLAIN_STEP_LIM;
     double len_lim = ratio * (n[0] + n[1] + n[2]);
-    double p1[3], p2[3], b0[3], tol = 0.0, len = 0.0;
+    double p1[3], p2[3], b0[3], tol = 0.0, len = 0.0, twist = 0.0;
     long steps = 0, attempts = 0, evals = 0;
     int flag = 0;
     memcpy(p1, p0, sizeof(p1));
@@ -545,9 +589,18 @@ static void trace_plain_dir(const field_t *f, const double *p0, int method, doub
         d[0] = p1[0] - p2[0]; d[1] = p1[1] - p2[1]; d[2] = p1[2] - p2[2];
         dl = len3(d);
         len += dl;
+        double Jp1[3] = {0.0, 0.0, 0.0}, tw_rate = 0.0;
+        if (cur) {
+            double Bp1[3];
+            interp_vec(cur, p1, Jp1, 0);
+            interp_vec(f, p1, Bp1, 0);
+            tw_rate = dot3(Jp1, Bp1) / dot3(Bp1, Bp1) / 4.0 / M_PI;
+            twist += tw_rate * dl;
+        }
         flag = check_flag(n, p2);
         if (flag > 0) {
             len -= dl;
+            if (cur) twist -= tw_rate * dl;             /* TraceBline.cu:334-336 */
             if (flag <= 6) {
                 int dim = (flag - 1) / 2, i;
                 double pa = p1[dim], pb = p2[dim];
@@ -565,7 +618,14 @@ static void trace_plain_dir(const field_t *f, const double *p0, int method, doub
                         memcpy(pout, q, sizeof(q));
                     }
                     len += fabs(pm - pa) / (1e-4 + fabs(bp1[dim]));
+                    /* RK4 only (the RKF45 tracer has this term commented out):
+                     * the landing arc times J.b/|b|^2 with b the normalised B at
+                     * p1, as the reference re-assigns B_P1 (TraceBline.cu:362-365) */
+                    if (cur && method == M_RK4)
+                        twist += dot3(Jp1, bp1) / dot3(bp1, bp1) / 4.0 / M_PI * fabs(pm - pa) / (1e-4 + fabs(bp1[dim]));
                 } else {
+                    /* no landing segment; the reference's p_mid is unset here:
+                     * no twist is added (decision register) */
                     memcpy(pout, p1, sizeof(p1));
                 }
             } else {
@@ -579,6 +639,7 @@ static void trace_plain_dir(const field_t *f, const double *p0, int method, doub
     *flag_out = flag;
     *len_out = len;
     *evals_out = evals;
+    if (twist_out) *twist_out = twist;
 }
 
 /* ================================================================ C ABI */
@@ -632,14 +693,17 @@ int orc_trace_scott(const double *bx, const double *by, const double *bz, int nx
                     int grad_mode, int nthreads,
                     double *start, double *end, int *flag_start, int *flag_end,
                     double *q0, double *qperp, double *len, double *bthis,
-                    long *evals, long *steps) {
+                    long *evals, long *steps,
+                    const double *jx, const double *jy, const double *jz, double *twist) {
     field_t f = {bx, by, bz, {nx, ny, nz}};
+    field_t cur = {jx, jy, jz, {nx, ny, nz}};
     long i;
     if (nx < 2 || ny < 2 || nz < 2) return -1;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_or_default(nthreads))
     for (i = 0; i < nseeds; ++i) {
         scott_result r;
-        trace_scott_line(&f, seeds + 3 * i, method, s0, tol, grad_mode, &r);
+        trace_scott_line(&f, jx ? &cur : NULL, seeds + 3 * i, method, s0, tol, grad_mode, &r);
+        if (twist) twist[i] = r.twist;
         memcpy(start + 3 * i, r.start, sizeof(r.start));
         memcpy(end + 3 * i, r.end, sizeof(r.end));
         flag_start[i] = r.flag_start;
@@ -680,18 +744,22 @@ int orc_trace_plain(const double *bx, const double *by, const double *bz, int nx
                     double tol_coef, int nthreads,
                     double *start, double *end, int *flag_start, int *flag_end,
                     double *bstart, double *bend, double *bthis, int *bflag,
-                    double *len, long *evals) {
+                    double *len, long *evals,
+                    const double *jx, const double *jy, const double *jz, double *twist) {
     field_t f = {bx, by, bz, {nx, ny, nz}};
+    field_t curf = {jx, jy, jz, {nx, ny, nz}};
+    const field_t *cur = jx ? &curf : NULL;
     long i;
     if (nx < 2 || ny < 2 || nz < 2) return -1;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_or_default(nthreads))
     for (i = 0; i < nseeds; ++i) {
         const double *p0 = seeds + 3 * i;
-        double lf, lb, bt[3];
+        double lf, lb, bt[3], tf = 0.0, tb = 0.0;
         long ef, eb;
-        trace_plain_dir(&f, p0, method, s0, tol_coef, ncross, 1.0, end + 3 * i, &flag_end[i], &lf, &ef);
+        trace_plain_dir(&f, cur, p0, method, s0, tol_coef, ncross, 1.0, end + 3 * i, &flag_end[i], &lf, &ef, &tf);
         interp_clamped(&f, end + 3 * i, bend + 3 * i);
-        trace_plain_dir(&f, p0, method, s0, tol_coef, ncross, -1.0, start + 3 * i, &flag_start[i], &lb, &eb);
+        trace_plain_dir(&f, cur, p0, method, s0, tol_coef, ncross, -1.0, start + 3 * i, &flag_start[i], &lb, &eb, &tb);
+        if (twist) twist[i] = tf + tb;
         interp_clamped(&f, start + 3 * i, bstart + 3 * i);
         len[i] = lf + lb;
         interp_clamped(&f, p0, bt);

@@ -44,12 +44,14 @@ def lib():
         L.orc_trace_scott.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_long, _dp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, ctypes.c_int,
-                                      _dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _lp, _lp]
+                                      _dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _lp, _lp,
+                                      _dp, _dp, _dp, _dp]
         L.orc_trace_plain.restype = ctypes.c_int
         L.orc_trace_plain.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_long, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int,
-                                      _dp, _dp, _ip, _ip, _dp, _dp, _dp, _ip, _dp, _lp]
+                                      _dp, _dp, _ip, _ip, _dp, _dp, _dp, _ip, _dp, _lp,
+                                      _dp, _dp, _dp, _dp]
         L.orc_interp.restype = None
         L.orc_interp.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_long, _dp, _dp, ctypes.c_int]
@@ -79,6 +81,10 @@ def _flat_field(bx, by, bz):
 
 def _ptr(a, t=_dp):
     return a.ctypes.data_as(t)
+
+
+def _optr(a, t=_dp):
+    return None if a is None else _ptr(a, t)
 
 
 def _seeds(seeds):
@@ -129,46 +135,53 @@ def scott_step(field, state9, s, method="rkf45", grad_mode=0):
     return out, e
 
 
-def trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6, grad_mode=0, nthreads=0):
-    """Scott tracer (TraceAllBlineScott, TraceBlineScott.cu:521-595) in FP64."""
+def trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6, grad_mode=0, nthreads=0, current=None):
+    """Scott tracer (TraceAllBlineScott, TraceBlineScott.cu:521-595) in FP64.
+    With ``current`` = (Jx, Jy, Jz) the result also has the twist number."""
     (bx, by, bz), shape = _flat_field(*field)
+    cur = _flat_field(*current)[0] if current is not None else (None, None, None)
     s = _seeds(seeds)
     n = s.shape[0]
     out = dict(start=np.empty((n, 3)), end=np.empty((n, 3)),
                flag_start=np.empty(n, np.int32), flag_end=np.empty(n, np.int32),
                q0=np.empty(n), qperp=np.empty(n), length=np.empty(n),
-               b_this=np.empty((n, 3)), evals=np.empty(n, np.int64), steps=np.empty(n, np.int64))
+               b_this=np.empty((n, 3)), evals=np.empty(n, np.int64), steps=np.empty(n, np.int64),
+               twist=np.zeros(n))
     rc = lib().orc_trace_scott(_ptr(bx), _ptr(by), _ptr(bz), *shape, n, _ptr(s), METHODS[method],
                                float(s0), float(tol), int(grad_mode), int(nthreads),
                                _ptr(out["start"]), _ptr(out["end"]),
                                _ptr(out["flag_start"], _ip), _ptr(out["flag_end"], _ip),
                                _ptr(out["q0"]), _ptr(out["qperp"]), _ptr(out["length"]),
                                _ptr(out["b_this"]), _ptr(out["evals"], _lp),
-                               _ptr(out["steps"], _lp))
+                               _ptr(out["steps"], _lp), _optr(cur[0]), _optr(cur[1]), _optr(cur[2]),
+                               _ptr(out["twist"]) if current is not None else None)
     if rc != 0:
         raise ValueError("oracle rejected the field shape")
     return out
 
 
 def trace_plain(field, seeds, ncross=(0.0, 0.0, 1.0), method="rkf45", s0=0.25, tol_coef=1.0,
-                nthreads=0):
+                nthreads=0, current=None):
     """Position-only tracer (TraceAllBline, TraceBlineAdaptive.cu:429-510 /
-    TraceBline.cu:383-464) in FP64."""
+    TraceBline.cu:383-464) in FP64; with ``current`` also the twist number."""
     (bx, by, bz), shape = _flat_field(*field)
+    cur = _flat_field(*current)[0] if current is not None else (None, None, None)
     s = _seeds(seeds)
     n = s.shape[0]
     nc = np.ascontiguousarray(np.asarray(ncross, dtype=np.float64))
     out = dict(start=np.empty((n, 3)), end=np.empty((n, 3)),
                flag_start=np.empty(n, np.int32), flag_end=np.empty(n, np.int32),
                b_start=np.empty((n, 3)), b_end=np.empty((n, 3)), b_this=np.empty((n, 3)),
-               b_flag=np.empty(n, np.int32), length=np.empty(n), evals=np.empty(n, np.int64))
+               b_flag=np.empty(n, np.int32), length=np.empty(n), evals=np.empty(n, np.int64),
+               twist=np.zeros(n))
     rc = lib().orc_trace_plain(_ptr(bx), _ptr(by), _ptr(bz), *shape, n, _ptr(s), _ptr(nc),
                                METHODS[method], float(s0), float(tol_coef), int(nthreads),
                                _ptr(out["start"]), _ptr(out["end"]),
                                _ptr(out["flag_start"], _ip), _ptr(out["flag_end"], _ip),
                                _ptr(out["b_start"]), _ptr(out["b_end"]), _ptr(out["b_this"]),
                                _ptr(out["b_flag"], _ip), _ptr(out["length"]),
-                               _ptr(out["evals"], _lp))
+                               _ptr(out["evals"], _lp), _optr(cur[0]), _optr(cur[1]), _optr(cur[2]),
+                               _ptr(out["twist"]) if current is not None else None)
     if rc != 0:
         raise ValueError("oracle rejected the field shape")
     return out

@@ -50,20 +50,23 @@ def grad_b(f, pts):
     return out
 
 
-def trace_scott(f, seeds, precision=0, method=0, s0=0.25, tol=1e-6):
+def trace_scott(f, seeds, precision=0, method=0, s0=0.25, tol=1e-6, current=None):
     (bx, by, bz), sh = _field(f)
+    j = _field(current)[0] if current is not None else (None, None, None)
     s = np.ascontiguousarray(np.asarray(seeds, np.float32).reshape(-1, 3))
     n = s.shape[0]
     o = dict(start=np.empty((n, 3)), end=np.empty((n, 3)), flag_start=np.empty(n, np.int32),
              flag_end=np.empty(n, np.int32), q0=np.empty(n), qperp=np.empty(n), length=np.empty(n),
-             evals=np.empty(n, np.int64))
+             evals=np.empty(n, np.int64), twist=np.zeros(n))
     L = lib()
     L.hc_trace_scott.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, _fp,
-                                 _dp, _dp, _ip, _ip, _dp, _dp, _dp, _lp]
+                                 _dp, _dp, _ip, _ip, _dp, _dp, _dp, _lp, _dp, _dp, _dp, _dp]
+    op = lambda a: None if a is None else P(a)
     L.hc_trace_scott(P(bx), P(by), P(bz), *sh, precision, method, s0, tol, n, P(s, _fp),
                      P(o["start"]), P(o["end"]), P(o["flag_start"], _ip), P(o["flag_end"], _ip),
-                     P(o["q0"]), P(o["qperp"]), P(o["length"]), P(o["evals"], _lp))
+                     P(o["q0"]), P(o["qperp"]), P(o["length"]), P(o["evals"], _lp),
+                     op(j[0]), op(j[1]), op(j[2]), P(o["twist"]) if current is not None else None)
     return o
 
 

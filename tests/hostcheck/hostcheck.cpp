@@ -32,14 +32,16 @@ std::vector<float4> build_cells(const float* bx, const float* by, const float* b
     return cells;
 }
 
-template <typename R, class F>
+template <typename R, class F, class FJ = NoCurrent>
 void run_scott(const F& f, const ScottParams& P, long n, const float* seeds, double* out_start, double* out_end,
-               int* fs, int* fe, double* q0, double* qp, double* len, long* evals) {
+               int* fs, int* fe, double* q0, double* qp, double* len, long* evals, const FJ& fj = FJ{},
+               double* twist = nullptr) {
     for (long i = 0; i < n; ++i) {
         ScottLine<R> L;
         ScottEnd<R> e;
         int st = scott_init<R>(L, f, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2], e);
-        while (st == kTracing) st = scott_attempt<R>(L, f, P, e);
+        while (st == kTracing) st = scott_attempt<R, F, FJ>(L, f, P, e, fj);
+        if (twist) twist[i] = L.tw;
         double nb0 = 0.0;
         {
             R u0[3], v0[3], b0z;
@@ -55,20 +57,33 @@ void run_scott(const F& f, const ScottParams& P, long n, const float* seeds, dou
 
 extern "C" {
 // precision 0: float + cell layout from the float field; 1: double SoA.
+// jx/jy/jz (may be NULL): current density for the twist output.
 int hc_trace_scott(const double* bx, const double* by, const double* bz, int nx, int ny, int nz, int precision,
                    int method, double s0, double tol, long n, const float* seeds, double* start, double* end,
-                   int* fs, int* fe, double* q0, double* qp, double* len, long* evals) {
+                   int* fs, int* fe, double* q0, double* qp, double* len, long* evals, const double* jx,
+                   const double* jy, const double* jz, double* twist) {
     ScottParams P{method, s0, tol, scott_ctl(tol)};
     const long nn = (long)nx * ny * nz;
     if (precision == 1) {
         FieldSoA<double> f{bx, by, bz, {nx, ny, nz}};
-        run_scott<double>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
+        if (jx) {
+            FieldSoA<double> fj{jx, jy, jz, {nx, ny, nz}};
+            run_scott<double>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals, fj, twist);
+        } else {
+            run_scott<double>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
+        }
         return 0;
     }
     std::vector<float> fx(bx, bx + nn), fy(by, by + nn), fz(bz, bz + nn);
     std::vector<float4> cells = build_cells(fx.data(), fy.data(), fz.data(), nx, ny, nz);
     FieldCell32 f{cells.data(), {nx, ny, nz}};
-    run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
+    if (jx) {
+        std::vector<float> gx(jx, jx + nn), gy(jy, jy + nn), gz(jz, jz + nn);
+        FieldSoA<float> fj{gx.data(), gy.data(), gz.data(), {nx, ny, nz}};
+        run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals, fj, twist);
+    } else {
+        run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
+    }
     return 0;
 }
 

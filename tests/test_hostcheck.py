@@ -111,3 +111,27 @@ def test_controller_power_is_double_accurate():
     sp = hc.pow_minus_tenth([0.0, np.nan, np.inf, 1e-40, 1e40])
     assert sp[0] == np.inf and np.isnan(sp[1]) and sp[2] == 0.0
     np.testing.assert_allclose(sp[3:], np.array([1e-40, 1e40]) ** -0.1, rtol=1e-15)
+
+
+@pytest.mark.parametrize("precision,rtol", [(1, 1e-9), (0, 2e-4)])
+def test_twist_matches_the_oracle(oracle, precision, rtol):
+    """The tracer header's twist (J, B raw at each step's end point, landing
+    term with the raw B) against the oracle, with J from the grid curl."""
+    from oracle import curl
+    f = synth.multipole((48, 40, 36), seed=6, dtype=np.float64, device="cpu")
+    cur = curl.curl_of_B_grid(*f, *(np.arange(n, dtype=np.float64) for n in f[0].shape))
+    rng = np.random.default_rng(15)
+    seeds = np.column_stack([rng.uniform(2, 45, 200), rng.uniform(2, 37, 200),
+                             np.r_[np.zeros(150), rng.uniform(0, 30, 50)]]).astype(np.float32)
+    ref = oracle.trace_scott(f, seeds, method="rk4", s0=0.25, current=cur)
+    refp = [oracle.trace_scott(f, seeds.astype(np.float64) + sg * np.array([1e-6, -1e-6, 0.0]), method="rk4",
+                               s0=0.25, current=cur) for sg in (1.0, -1.0)]
+    cond = parity.conditioned(ref, *refp)
+    for p in refp:
+        cond &= np.abs(p["twist"] - ref["twist"]) <= 1e-6 * (1 + np.abs(ref["twist"]))
+    r = hc.trace_scott(f, seeds, precision=precision, method=1, s0=0.25, current=cur)
+    ok = cond & parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    assert ok.mean() > 0.95
+    scale = np.maximum(np.abs(ref["twist"]), 1e-3)
+    assert np.quantile(np.abs(r["twist"] - ref["twist"])[ok] / scale[ok], 0.99) < rtol
+    assert np.mean(np.abs(r["twist"]) > 1e-6) > 0.5
