@@ -243,9 +243,10 @@ def test_dropin_scott_kernel_matches_direct_api(fq):
             for mine, theirs in (("sx", "start_x"), ("ey", "end_y"), ("fs", "flag_start"), ("fe", "flag_end"),
                                  ("q0", "q_0"), ("qp", "q_perp"), ("btz", "b_this_z"), ("line_len", "line_len")):
                 assert np.array_equal(got[mine], ref[theirs], equal_nan=True), (use_torch, sched, mine)
+    # with curB_flag set the twist output must be a float64 array (here a float32 dummy)
     a = _scott_args(field, seeds[:10], False)
     a["flag_twist"][0] = True
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(TypeError, match="twist"):
         kern((1, 1), (64, 1, 1), tuple(a.values()))
 
 

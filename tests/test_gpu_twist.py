@@ -114,19 +114,30 @@ def test_plain_twist_matches_oracle(fq, oracle, method):
     ref = oracle.trace_plain(f, seeds, current=cur, **kw)
     refp = [oracle.trace_plain(f, seeds.astype(np.float64) + sg * np.array([1e-6, -1e-6, 0.0]), current=cur, **kw)
             for sg in (1.0, -1.0)]
-    cond = parity.conditioned(ref, *refp, key="end")
-    for p in refp:
-        cond &= np.abs(p["twist"] - ref["twist"]) <= 1e-6 * (1 + np.abs(ref["twist"]))
     with fq.Field(*f, current=cur) as field:
         r = fq.trace_plain(field, seeds[:, 0], seeds[:, 1], seeds[:, 2], method=method, step=0.25,
                            tol_coef=1e-3, precision="fp64", outputs=("flag_start", "flag_end", "twist"))
         torch.cuda.synchronize()
         r = {k: v.cpu().numpy() for k, v in r.items() if k != "_seeds"}
-    ok = cond & parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
-    assert ok.mean() > 0.9
-    rel = np.abs(r["twist"] - ref["twist"])[ok] / np.maximum(np.abs(ref["twist"]), 1e-3)[ok]
-    # RKF45 step sequences fork on round-off even in FP64; RK4 is deterministic
-    assert np.quantile(rel, 0.99) < (1e-8 if method == "rk4" else 1e-2)
+    cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    rel = lambda a: np.abs(a - ref["twist"]) / np.maximum(np.abs(ref["twist"]), 1e-3)
+    if method == "rk4":   # deterministic: line by line on well-conditioned lines
+        cond = parity.conditioned(ref, *refp, key="end")
+        for p in refp:
+            cond &= np.abs(p["twist"] - ref["twist"]) <= 1e-6 * (1 + np.abs(ref["twist"]))
+        ok = cond & cls
+        assert ok.mean() > 0.9
+        assert np.quantile(rel(r["twist"])[ok], 0.99) < 1e-8
+    else:
+        # RKF45 step sequences fork on round-off even in FP64: the GPU's twist
+        # is as close to the oracle's as the oracle is to itself under a
+        # 1e-6-cell seed move
+        self_spread = np.maximum(rel(refp[0]["twist"]), rel(refp[1]["twist"]))
+        both = cls & parity.class_match(refp[0]["flag_start"], refp[0]["flag_end"], ref["flag_start"],
+                                        ref["flag_end"])
+        assert cls.mean() > 0.97
+        for q in (0.5, 0.9):
+            assert np.quantile(rel(r["twist"])[both], q) <= 1.5 * np.quantile(self_spread[both], q) + 1e-9
 
 
 def test_dropin_twist_matches_handle_api(fq):
