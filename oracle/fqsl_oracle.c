@@ -25,6 +25,12 @@
  *                                             TraceBlineAdaptive.cu:333-426,
  *                                             TraceBline.cu:300-378
  *   orc_trace_plain   TraceAllBline           TraceBlineAdaptive.cu:429-510
+ *   lookup_bisect     GetIndexBiSearch        TraceBlineAdaptiveStretch.cu:43-64
+ *   lookup_uniform    GetIndexUniStretch      TraceBlineAdaptiveStretch.cu:67-74
+ *   stretch_cell      Interp3dxyzn clamp and weights
+ *                                             TraceBlineAdaptiveStretch.cu:80-124
+ *   trace_stretch_dir TraceBlineAdap          TraceBlineAdaptiveStretch.cu:292-419
+ *   orc_trace_stretch TraceAllBline           TraceBlineAdaptiveStretch.cu:422-508
  *
  * Where the reference reads uninitialised memory or leaves outputs undefined,
  * this file takes one documented resolution, shared with the GPU kernels:
@@ -46,6 +52,10 @@
  *   - A class-42 exit sets both classes of an unfinished line to 42, keeps the
  *     last in-box state as that side's endpoint (the seed for a side never
  *     traced) and sets Q = 0 (TraceBlineScott.cu:460-465).
+ *   - Stretched grids: the reference's cell rules and its landing plane
+ *     (coordinate 0 / n_dim, not the box faces) are reproduced; the exact
+ *     rule (the cell that holds the point, landing on the faces) is an
+ *     alternative the caller selects.
  *   - Every line stops after FQSL_ATTEMPT_CAP integrator attempts (accepted or
  *     rejected) with class 0; the reference has no such cap, and no line of a
  *     field with finite values reaches it.
@@ -75,9 +85,18 @@
 
 enum { M_RKF45 = 0, M_RK4 = 1 };
 
+/* A stretched grid: node coordinates per axis (strictly increasing) and the
+ * rule that picks the interpolation cell (TraceBlineAdaptiveStretch.cu). */
+enum { LOOKUP_BISECT = 0, LOOKUP_UNIFORM = 1, LOOKUP_EXACT = 2 };
+typedef struct {
+    const double *a[3];
+    int rule;
+} axes_t;
+
 typedef struct {
     const double *bx, *by, *bz;
     int n[3];
+    const axes_t *ax;   /* NULL: grid-index coordinates */
 } field_t;
 
 typedef struct { double x[3], u[3], v[3]; } state9;
@@ -102,6 +121,53 @@ static void axis_cell(double p, int n, int *idx, double *r) {
     *idx = ii; *r = rr;
 }
 
+/* GetIndexBiSearch (TraceBlineAdaptiveStretch.cu:43-64): bisection that
+ * stops once the bracket spans at most three cells and then decides between
+ * the bracket's middle node and the one below it only, so a point in the
+ * bracket's top cell is given the cell below (and extrapolated from it). */
+static int lookup_bisect(double p, int n, const double *a) {
+    int lo = 0, hi = n - 1, mid = n / 2, r;
+    while (hi - lo > 3) {
+        if (p > a[mid]) lo = mid; else hi = mid;
+        mid = (lo + hi) / 2;
+    }
+    r = (p > a[mid]) ? mid : mid - 1;
+    return r > n - 2 ? n - 2 : r;
+}
+
+/* GetIndexUniStretch (TraceBlineAdaptiveStretch.cu:67-74): the cell index of
+ * an evenly spaced axis, with the span divided by n (not n - 1). */
+static int lookup_uniform(double p, int n, const double *a) {
+    double r = floor((p - a[0]) / ((a[n - 1] - a[0]) / (double)n));
+    if (r < 0.0) return 0;
+    if (r > n - 2) return n - 2;
+    return (int)r;
+}
+
+/* The cell that contains p: a[c] < p <= a[c+1], c in [0, n-2] (linear scan;
+ * test infrastructure). */
+static int lookup_exact(double p, int n, const double *a) {
+    int c = 0;
+    while (c < n - 2 && p > a[c + 1]) ++c;
+    return c;
+}
+
+/* Interp3dxyzn of the stretched tracer (TraceBlineAdaptiveStretch.cu:80-194):
+ * the point is first pulled into [a0 + 1e-6, a_{n-1} - 1e-6] (NaN goes to the
+ * low bound), then the rule picks the cell c and the weight is
+ * (p - a[c]) / (a[c+1] - a[c]), outside [0, 1] when the cell does not hold p. */
+static void stretch_cell(const axes_t *ax, int axis, double p, int n, int *idx, double *r) {
+    const double *a = ax->a[axis];
+    int c;
+    p = (p > a[0]) ? p : a[0] + 1e-6;
+    p = (p < a[n - 1]) ? p : a[n - 1] - 1e-6;
+    if (ax->rule == LOOKUP_BISECT) c = lookup_bisect(p, n, a);
+    else if (ax->rule == LOOKUP_UNIFORM) c = lookup_uniform(p, n, a);
+    else c = lookup_exact(p, n, a);
+    *idx = c;
+    *r = (p - a[c]) / (a[c + 1] - a[c]);
+}
+
 static void interp_vec(const field_t *f, const double *p, double *out, int normalize) {
     int ix, iy, iz;
     double rx, ry, rz, w[8];
@@ -109,9 +175,15 @@ static void interp_vec(const field_t *f, const double *p, double *out, int norma
     long base;
     long off[8];
     int c;
-    axis_cell(p[0], f->n[0], &ix, &rx);
-    axis_cell(p[1], f->n[1], &iy, &ry);
-    axis_cell(p[2], f->n[2], &iz, &rz);
+    if (f->ax) {
+        stretch_cell(f->ax, 0, p[0], f->n[0], &ix, &rx);
+        stretch_cell(f->ax, 1, p[1], f->n[1], &iy, &ry);
+        stretch_cell(f->ax, 2, p[2], f->n[2], &iz, &rz);
+    } else {
+        axis_cell(p[0], f->n[0], &ix, &rx);
+        axis_cell(p[1], f->n[1], &iy, &ry);
+        axis_cell(p[2], f->n[2], &iz, &rz);
+    }
     w[0] = (1. - rx) * (1. - ry) * (1. - rz);
     w[1] = (1. - rx) * (1. - ry) * (rz);
     w[2] = (1. - rx) * (ry) * (1. - rz);
@@ -199,18 +271,23 @@ static void grad_b_analytic(const field_t *f, const double *p, double J[3][3]) {
 
 /* checkFlag (TraceBlineScott.cu:310-332): 0 inside (note the strict y > 0),
  * 1..6 = x min/max, y min/max, z min/max, 42 otherwise (edges, NaN). */
-static int check_flag(const int *n, const double *p) {
+static int check_flag_box(const double *lo, const double *hi, const double *p) {
     double X = p[0], Y = p[1], Z = p[2];
-    double mx = n[0] - 1., my = n[1] - 1., mz = n[2] - 1.;
+    double x0 = lo[0], y0 = lo[1], z0 = lo[2], mx = hi[0], my = hi[1], mz = hi[2];
     int f = 42;
-    if (X >= 0. && Y > 0. && Z >= 0. && X <= mx && Y <= my && Z <= mz) return 0;
-    if (X < 0. && Y >= 0. && Z >= 0. && X < mx && Y < my && Z < mz) f = 1;
-    if (X >= 0. && Y >= 0. && Z >= 0. && X >= mx && Y < my && Z < mz) f = 2;
-    if (X >= 0. && Y < 0. && Z >= 0. && X < mx && Y < my && Z < mz) f = 3;
-    if (X >= 0. && Y >= 0. && Z >= 0. && X < mx && Y >= my && Z < mz) f = 4;
-    if (X >= 0. && Y >= 0. && Z < 0. && X < mx && Y < my && Z < mz) f = 5;
-    if (X >= 0. && Y >= 0. && Z >= 0. && X < mx && Y < my && Z >= mz) f = 6;
+    if (X >= x0 && Y > y0 && Z >= z0 && X <= mx && Y <= my && Z <= mz) return 0;
+    if (X < x0 && Y >= y0 && Z >= z0 && X < mx && Y < my && Z < mz) f = 1;
+    if (X >= x0 && Y >= y0 && Z >= z0 && X >= mx && Y < my && Z < mz) f = 2;
+    if (X >= x0 && Y < y0 && Z >= z0 && X < mx && Y < my && Z < mz) f = 3;
+    if (X >= x0 && Y >= y0 && Z >= z0 && X < mx && Y >= my && Z < mz) f = 4;
+    if (X >= x0 && Y >= y0 && Z < z0 && X < mx && Y < my && Z < mz) f = 5;
+    if (X >= x0 && Y >= y0 && Z >= z0 && X < mx && Y < my && Z >= mz) f = 6;
     return f;
+}
+
+static int check_flag(const int *n, const double *p) {
+    const double lo[3] = {0., 0., 0.}, hi[3] = {n[0] - 1., n[1] - 1., n[2] - 1.};
+    return check_flag_box(lo, hi, p);
 }
 
 /* ------------------------------------------------------------ Scott RHS */
@@ -642,6 +719,102 @@ static void trace_plain_dir(const field_t *f, const field_t *cur, const double *
     if (twist_out) *twist_out = twist;
 }
 
+/* ------------------------------------------------ stretched-grid tracer */
+#define STRETCH_LEN_RATIO 2.0   /* MAX_STEP_RATIO, TraceBlineAdaptiveStretch.cu:11 */
+
+/* TraceBlineAdap of the stretched tracer (TraceBlineAdaptiveStretch.cu:
+ * 292-419), one direction, positions in the axes' coordinates: the RKF45
+ * step and controller of the uniform tracer, the box [a0, a_{n-1}] per axis,
+ * len_lim = 2 (sum of the three spans), no step-count limit, and a face
+ * landing on the plane p_mid = 0 (min face) or n_dim (max face) as the
+ * reference writes it (:386) -- or on the face itself (a0 / a_{n-1}) when
+ * the axes use the exact lookup.  With cur, the twist of each step at its
+ * start point and, unlike the uniform RKF45 tracer, the landing term
+ * J.b/|b|^2 with b the normalised B at p1 (:407-410). */
+static void trace_stretch_dir(const field_t *f, const field_t *cur, const double *p0, double s,
+                              double tol_coef, const double *ncross, double dir,
+                              double *pout, int *flag_out, double *len_out, long *evals_out,
+                              double *twist_out) {
+    const int *n = f->n;
+    const axes_t *ax = f->ax;
+    double lo[3], hi[3];
+    double p1[3], p2[3], b0[3], tol, len = 0.0, twist = 0.0, len_lim;
+    long attempts = 0, evals = 0;
+    int flag = 0, a;
+    for (a = 0; a < 3; ++a) { lo[a] = ax->a[a][0]; hi[a] = ax->a[a][n[a] - 1]; }
+    len_lim = STRETCH_LEN_RATIO * ((hi[0] + hi[1] + hi[2]) - (lo[0] + lo[1] + lo[2]));
+    memcpy(p1, p0, sizeof(p1));
+    memcpy(pout, p0, sizeof(p1));
+    interp_vec(f, p0, b0, 1);
+    {                                                  /* :311-317 */
+        double bn = fabs(dot3(b0, ncross));
+        tol = (bn <= 0.05) ? TOL_BASE / 8e3 : TOL_BASE * pow(bn, 1.5);
+        tol = tol * tol_coef;
+    }
+    while (flag == 0 && len < len_lim) {
+        double d[3], dl, err, scale;
+        if (attempts >= FQSL_ATTEMPT_CAP) break;
+        ++attempts;
+        err = plain_rkf45(f, p1, p2, s * dir);
+        scale = pow(tol / err / 11.09, 0.2);
+        evals += 6;
+        if (scale < 0.618) { s = s * 0.618; continue; }
+        s = s * scale;
+        if (s > 100.) s = 100.;
+        if (s < 1. / 10.) s = 1. / 10.;
+        d[0] = p1[0] - p2[0]; d[1] = p1[1] - p2[1]; d[2] = p1[2] - p2[2];
+        dl = len3(d);
+        len += dl;
+        double Jp1[3] = {0.0, 0.0, 0.0}, tw_rate = 0.0;
+        if (cur) {
+            double Bp1[3];
+            interp_vec(cur, p1, Jp1, 0);
+            interp_vec(f, p1, Bp1, 0);
+            tw_rate = dot3(Jp1, Bp1) / dot3(Bp1, Bp1) / 4.0 / M_PI;
+            twist += tw_rate * dl;
+        }
+        flag = check_flag_box(lo, hi, p2);
+        if (flag > 0) {
+            len -= dl;
+            if (cur) twist -= tw_rate * dl;
+            if (flag <= 6) {
+                int dim = (flag - 1) / 2, i;
+                double pa = p1[dim], pb = p2[dim];
+                if (fabs(pa - pb) > 1e-3) {
+                    double pm, bp1[3], bp2[3], arc;
+                    if (ax->rule == LOOKUP_EXACT) pm = (flag % 2 == 1) ? lo[dim] : hi[dim];
+                    else pm = (flag % 2 == 1) ? 0.0 : (double)n[dim];
+                    interp_vec(f, p1, bp1, 1);
+                    interp_vec(f, p2, bp2, 1);
+                    if (fabs(bp1[dim]) < 0.2 || fabs(bp2[dim]) < 0.2) {
+                        for (i = 0; i < 3; ++i) pout[i] = (p1[i] * (pb - pm) + p2[i] * (pm - pa)) / (pb - pa);
+                    } else {
+                        double q[3];
+                        rk4_boundary(f, p1, q, pm - pa, dim);
+                        evals += 4;
+                        memcpy(pout, q, sizeof(q));
+                    }
+                    arc = fabs(pm - pa) / (1e-4 + fabs(bp1[dim]));
+                    len += arc;
+                    if (cur) twist += dot3(Jp1, bp1) / dot3(bp1, bp1) / 4.0 / M_PI * arc;
+                } else {
+                    /* no landing segment (p_mid unset in the reference): no
+                     * landing twist */
+                    memcpy(pout, p1, sizeof(p1));
+                }
+            } else {
+                memcpy(pout, p2, sizeof(p2));
+            }
+        }
+        memcpy(p1, p2, sizeof(p1));
+    }
+    if (flag == 0) memcpy(pout, p1, sizeof(p1));    /* limit exit: last position */
+    *flag_out = flag;
+    *len_out = len;
+    *evals_out = evals;
+    if (twist_out) *twist_out = twist;
+}
+
 /* ================================================================ C ABI */
 static int nthreads_or_default(int nt) {
 #ifdef _OPENMP
@@ -658,7 +831,7 @@ int orc_num_threads(void) { return nthreads_or_default(0); }
  * 2: Interp3d clamped scalar rule). */
 void orc_interp(const double *bx, const double *by, const double *bz, int nx, int ny, int nz,
                 long npts, const double *pts, double *out, int mode) {
-    field_t f = {bx, by, bz, {nx, ny, nz}};
+    field_t f = {bx, by, bz, {nx, ny, nz}, NULL};
     long i;
     for (i = 0; i < npts; ++i) {
         if (mode == 2) interp_clamped(&f, pts + 3 * i, out + 3 * i);
@@ -670,7 +843,7 @@ void orc_interp(const double *bx, const double *by, const double *bz, int nx, in
  * reference's difference quotient, 1: closed form inside the cell. */
 void orc_grad_b(const double *bx, const double *by, const double *bz, int nx, int ny, int nz,
                 long npts, const double *pts, double *out, int mode) {
-    field_t f = {bx, by, bz, {nx, ny, nz}};
+    field_t f = {bx, by, bz, {nx, ny, nz}, NULL};
     long i;
     for (i = 0; i < npts; ++i) {
         double J[3][3];
@@ -695,8 +868,8 @@ int orc_trace_scott(const double *bx, const double *by, const double *bz, int nx
                     double *q0, double *qperp, double *len, double *bthis,
                     long *evals, long *steps,
                     const double *jx, const double *jy, const double *jz, double *twist) {
-    field_t f = {bx, by, bz, {nx, ny, nz}};
-    field_t cur = {jx, jy, jz, {nx, ny, nz}};
+    field_t f = {bx, by, bz, {nx, ny, nz}, NULL};
+    field_t cur = {jx, jy, jz, {nx, ny, nz}, NULL};
     long i;
     if (nx < 2 || ny < 2 || nz < 2) return -1;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_or_default(nthreads))
@@ -723,7 +896,7 @@ int orc_trace_scott(const double *bx, const double *by, const double *bz, int nx
  * tests only. */
 double orc_scott_step(const double *bx, const double *by, const double *bz, int nx, int ny, int nz,
                       const double *y, double *out, double s, int method, int grad_mode) {
-    field_t f = {bx, by, bz, {nx, ny, nz}};
+    field_t f = {bx, by, bz, {nx, ny, nz}, NULL};
     state9 a, b;
     memcpy(&a, y, sizeof(a));
     if (method == M_RKF45) {
@@ -746,8 +919,8 @@ int orc_trace_plain(const double *bx, const double *by, const double *bz, int nx
                     double *bstart, double *bend, double *bthis, int *bflag,
                     double *len, long *evals,
                     const double *jx, const double *jy, const double *jz, double *twist) {
-    field_t f = {bx, by, bz, {nx, ny, nz}};
-    field_t curf = {jx, jy, jz, {nx, ny, nz}};
+    field_t f = {bx, by, bz, {nx, ny, nz}, NULL};
+    field_t curf = {jx, jy, jz, {nx, ny, nz}, NULL};
     const field_t *cur = jx ? &curf : NULL;
     long i;
     if (nx < 2 || ny < 2 || nz < 2) return -1;
@@ -765,6 +938,61 @@ int orc_trace_plain(const double *bx, const double *by, const double *bz, int nx
         interp_clamped(&f, p0, bt);
         memcpy(bthis + 3 * i, bt, sizeof(bt));
         /* B_flag, TraceBlineAdaptive.cu:498-501 */
+        bflag[i] = fabs(bt[0] * ncross[0] + bt[1] * ncross[1] + bt[2] * ncross[2]) * 100. < len3(bt) ? 1 : 0;
+        if (evals) evals[i] = ef + eb;
+    }
+    return 0;
+}
+
+/* Cell index and weight along one stretched axis at n points
+ * (Interp3dxyzn's lookup, TraceBlineAdaptiveStretch.cu:80-124). */
+void orc_stretch_cell(const double *a, int n, int rule, long npts, const double *p, int *cell, double *w) {
+    axes_t ax = {{a, a, a}, rule};
+    long i;
+    for (i = 0; i < npts; ++i) stretch_cell(&ax, 0, p[i], n, cell + i, w + i);
+}
+
+/* Raw or normalised B on a stretched grid at n points. */
+void orc_interp_stretch(const double *bx, const double *by, const double *bz, int nx, int ny, int nz,
+                        const double *xa, const double *ya, const double *za, int rule,
+                        long npts, const double *pts, double *out, int normalize) {
+    axes_t ax = {{xa, ya, za}, rule};
+    field_t f = {bx, by, bz, {nx, ny, nz}, &ax};
+    long i;
+    for (i = 0; i < npts; ++i) interp_vec(&f, pts + 3 * i, out + 3 * i, normalize);
+}
+
+/* Stretched-grid RKF45 tracer over n seeds (TraceAllBline,
+ * TraceBlineAdaptiveStretch.cu:422-508): forward first, then backward, each
+ * from s0; B_start / B_end / B_this by the stretched Interp3dxyzn (raw). */
+int orc_trace_stretch(const double *bx, const double *by, const double *bz, int nx, int ny, int nz,
+                      const double *xa, const double *ya, const double *za, int rule,
+                      long nseeds, const double *seeds, const double *ncross, double s0,
+                      double tol_coef, int nthreads,
+                      double *start, double *end, int *flag_start, int *flag_end,
+                      double *bstart, double *bend, double *bthis, int *bflag,
+                      double *len, long *evals,
+                      const double *jx, const double *jy, const double *jz, double *twist) {
+    axes_t ax = {{xa, ya, za}, rule};
+    field_t f = {bx, by, bz, {nx, ny, nz}, &ax};
+    field_t curf = {jx, jy, jz, {nx, ny, nz}, &ax};
+    const field_t *cur = jx ? &curf : NULL;
+    long i;
+    if (nx < 2 || ny < 2 || nz < 2) return -1;
+    if (rule < LOOKUP_BISECT || rule > LOOKUP_EXACT) return -2;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_or_default(nthreads))
+    for (i = 0; i < nseeds; ++i) {
+        const double *p0 = seeds + 3 * i;
+        double lf, lb, bt[3], tf = 0.0, tb = 0.0;
+        long ef, eb;
+        trace_stretch_dir(&f, cur, p0, s0, tol_coef, ncross, 1.0, end + 3 * i, &flag_end[i], &lf, &ef, &tf);
+        interp_vec(&f, end + 3 * i, bend + 3 * i, 0);
+        trace_stretch_dir(&f, cur, p0, s0, tol_coef, ncross, -1.0, start + 3 * i, &flag_start[i], &lb, &eb, &tb);
+        interp_vec(&f, start + 3 * i, bstart + 3 * i, 0);
+        if (twist) twist[i] = tf + tb;
+        len[i] = lf + lb;
+        interp_vec(&f, p0, bt, 0);
+        memcpy(bthis + 3 * i, bt, sizeof(bt));
         bflag[i] = fabs(bt[0] * ncross[0] + bt[1] * ncross[1] + bt[2] * ncross[2]) * 100. < len3(bt) ? 1 : 0;
         if (evals) evals[i] = ef + eb;
     }

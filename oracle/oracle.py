@@ -64,6 +64,19 @@ def lib():
         L.orc_scott_step.restype = ctypes.c_double
         L.orc_scott_step.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.orc_stretch_cell.restype = None
+        L.orc_stretch_cell.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp, _ip, _dp]
+        L.orc_interp_stretch.restype = None
+        L.orc_interp_stretch.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         _dp, _dp, _dp, ctypes.c_int, ctypes.c_long, _dp, _dp,
+                                         ctypes.c_int]
+        L.orc_trace_stretch.restype = ctypes.c_int
+        L.orc_trace_stretch.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        _dp, _dp, _dp, ctypes.c_int,
+                                        ctypes.c_long, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int,
+                                        _dp, _dp, _ip, _ip, _dp, _dp, _dp, _ip, _dp, _lp,
+                                        _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -184,4 +197,68 @@ def trace_plain(field, seeds, ncross=(0.0, 0.0, 1.0), method="rkf45", s0=0.25, t
                                _ptr(out["twist"]) if current is not None else None)
     if rc != 0:
         raise ValueError("oracle rejected the field shape")
+    return out
+
+
+# cell rules of the stretched tracer: the reference's bisection
+# (GetIndexBiSearch), its evenly-spaced formula (GetIndexUniStretch), and the
+# cell that holds the point
+LOOKUPS = {"bisect": 0, "uniform": 1, "exact": 2}
+
+
+def _axes(axes):
+    return [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in axes]
+
+
+def stretch_cell(axis, pts, rule="bisect"):
+    """(cell, weight) along one stretched axis at each point."""
+    a = np.ascontiguousarray(np.asarray(axis, dtype=np.float64))
+    p = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1))
+    cell = np.empty(p.shape[0], np.int32)
+    w = np.empty(p.shape[0])
+    lib().orc_stretch_cell(_ptr(a), a.shape[0], LOOKUPS[rule], p.shape[0], _ptr(p), _ptr(cell, _ip),
+                           _ptr(w))
+    return cell, w
+
+
+def interp_stretch(field, axes, pts, rule="bisect", normalize=False):
+    """Interp3dxyzn of the stretched tracer at points in axis coordinates."""
+    (bx, by, bz), shape = _flat_field(*field)
+    xa, ya, za = _axes(axes)
+    p = _seeds(pts)
+    out = np.empty_like(p)
+    lib().orc_interp_stretch(_ptr(bx), _ptr(by), _ptr(bz), *shape, _ptr(xa), _ptr(ya), _ptr(za),
+                             LOOKUPS[rule], p.shape[0], _ptr(p), _ptr(out), int(bool(normalize)))
+    return out
+
+
+def trace_stretch(field, axes, seeds, ncross=(0.0, 0.0, 1.0), s0=0.01, tol_coef=1.0, rule="bisect",
+                  nthreads=0, current=None):
+    """Stretched-grid RKF45 tracer (TraceAllBline of
+    TraceBlineAdaptiveStretch.cu:422-508) in FP64; seeds and results in the
+    axes' coordinates; with ``current`` also the twist number."""
+    (bx, by, bz), shape = _flat_field(*field)
+    xa, ya, za = _axes(axes)
+    if (xa.shape[0], ya.shape[0], za.shape[0]) != tuple(shape):
+        raise ValueError("axis lengths must match the field shape")
+    cur = _flat_field(*current)[0] if current is not None else (None, None, None)
+    s = _seeds(seeds)
+    n = s.shape[0]
+    nc = np.ascontiguousarray(np.asarray(ncross, dtype=np.float64))
+    out = dict(start=np.empty((n, 3)), end=np.empty((n, 3)),
+               flag_start=np.empty(n, np.int32), flag_end=np.empty(n, np.int32),
+               b_start=np.empty((n, 3)), b_end=np.empty((n, 3)), b_this=np.empty((n, 3)),
+               b_flag=np.empty(n, np.int32), length=np.empty(n), evals=np.empty(n, np.int64),
+               twist=np.zeros(n))
+    rc = lib().orc_trace_stretch(_ptr(bx), _ptr(by), _ptr(bz), *shape, _ptr(xa), _ptr(ya), _ptr(za),
+                                 LOOKUPS[rule], n, _ptr(s), _ptr(nc), float(s0), float(tol_coef),
+                                 int(nthreads),
+                                 _ptr(out["start"]), _ptr(out["end"]),
+                                 _ptr(out["flag_start"], _ip), _ptr(out["flag_end"], _ip),
+                                 _ptr(out["b_start"]), _ptr(out["b_end"]), _ptr(out["b_this"]),
+                                 _ptr(out["b_flag"], _ip), _ptr(out["length"]),
+                                 _ptr(out["evals"], _lp), _optr(cur[0]), _optr(cur[1]), _optr(cur[2]),
+                                 _ptr(out["twist"]) if current is not None else None)
+    if rc != 0:
+        raise ValueError("oracle rejected the field shape or lookup rule")
     return out
