@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libfqsl.so")
 
 FQSL_OK = 0
+LOOKUPS = {"bisect": 0, "uniform": 1, "exact": 2}  # FQSL_LOOKUP_*
 DTYPE_F32, DTYPE_F64 = 0, 1
 METHODS = {"rkf45": 0, "rk4": 1}
 PRECISIONS = {"fp32c": 0, "fp64": 1}
@@ -68,6 +69,8 @@ PROTOTYPES = {
     "fqsl_field_bytes": (ctypes.c_size_t, [vp]),
     "fqsl_field_shape": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),
     "fqsl_field_set_current": (i32, [vp, vp, vp, vp, i32, i32]),
+    "fqsl_field_set_axes": (i32, [vp, vp, vp, vp, i32, i32, i32]),
+    "fqsl_field_stretched": (i32, [vp]),
     "fqsl_curl": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
     "fqsl_diff_3point": (i32, [vp, i32, i32, i32, i32, vp, i32, vp, vp]),
     "fqsl_copy_boundary": (i32, [vp, i32, i32, i32, vp]),
@@ -81,6 +84,7 @@ PROTOTYPES = {
     "fqsl_qcut_scatter": (i32, [u64] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),
     "fqsl_TraceAllBlineScott": (i32, [vp] * 30 + [i32, i32, vp]),
     "fqsl_TraceAllBline": (i32, [i32] + [vp] * 35 + [i32, i32, vp]),
+    "fqsl_TraceAllBlineStretch": (i32, [vp] * 39 + [i32, i32, vp]),
     "fqsl_scott_batch_workspace_bytes": (ctypes.c_size_t, [ctypes.POINTER(fqsl_scott_request), i32, i32]),
     "fqsl_scott_batch_create": (i32, [ctypes.POINTER(fqsl_scott_request), i32, i32, vp, ctypes.c_size_t, vp,
                                       ctypes.POINTER(vp)]),

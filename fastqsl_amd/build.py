@@ -9,7 +9,8 @@ from ._lib import HERE, LIBPATH
 
 ROOT = os.path.dirname(HERE)
 SOURCES = [os.path.join(HERE, "csrc", "fqsl_kernels.hip")]
-HEADERS = [os.path.join(HERE, "csrc", "fqsl_tracer.h"), os.path.join(ROOT, "include", "fqsl.h")]
+HEADERS = [os.path.join(HERE, "csrc", "fqsl_tracer.h"), os.path.join(HERE, "csrc", "fqsl_stretch.h"),
+           os.path.join(ROOT, "include", "fqsl.h")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

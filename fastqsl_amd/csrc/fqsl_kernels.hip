@@ -5,7 +5,8 @@
 //   k_scott_persistent     Scott tracer, persistent grid, per-lane refill
 //                          from a device work queue (one line per lane)
 //   k_scott_per_line       Scott tracer, one thread per line
-//   k_plain_persistent / k_plain_per_line   position-only tracer
+//   k_plain_persistent / k_plain_per_line   position-only tracer (grid-index
+//                          coordinates, or a stretched grid's own coordinates)
 //   k_qcalc_plane          QCalcPlane (FastQSL.py:65-117)
 //   k_pseudo_*             CookPseudoLine compaction (FastQSL.py:4-57)
 //   k_qcut_scatter         Qcut + merge (demo/Q_RKF45CuPy.ipynb cells [10]-[11])
@@ -21,6 +22,7 @@
 #include <vector>
 
 #include "fqsl.h"
+#include "fqsl_stretch.h"
 #include "fqsl_tracer.h"
 
 using namespace fqsl;
@@ -60,6 +62,12 @@ struct fqsl_field {
     float* cur32[3];   // current density J (twist), float and double SoA
     double* cur64[3];
     size_t bytes;
+    // stretched grid: node coordinates per axis and their lookup tables
+    void* axes_mem;
+    size_t axes_bytes;
+    bool stretched;
+    StretchGrid<float> sg32;
+    StretchGrid<double> sg64;
 };
 
 __global__ void k_convert_to_f32(const double* in, float* out, long n) {
@@ -112,6 +120,7 @@ static void field_free(fqsl_field* f) {
         if (f->cur64[q]) (void)hipFree(f->cur64[q]);
     }
     if (f->cell32) (void)hipFree(f->cell32);
+    if (f->axes_mem) (void)hipFree(f->axes_mem);
     delete f;
 }
 
@@ -251,6 +260,93 @@ extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) 
     if (nz) *nz = f->n[2];
     return FQSL_OK;
 }
+
+// -------------------------------------------------------- stretched grids
+// Lookup tables of the three axes in one device allocation; fills the float
+// and double views.  Landing planes: 0 / n (the reference rules) or the box
+// faces (exact rule).
+static int build_stretch(const std::vector<double> (&a)[3], const int* n, int rule, void** mem, size_t* bytes,
+                         StretchGrid<float>* g32, StretchGrid<double>* g64) {
+    if (rule != kLookupBisect && rule != kLookupUniform && rule != kLookupExact)
+        return set_err(FQSL_ERR_INVALID, "lookup rule %d", rule);
+    AxisTables t[3];
+    for (int q = 0; q < 3; ++q) {
+        if ((int)a[q].size() != n[q]) return set_err(FQSL_ERR_INVALID, "axis %d has %zu nodes, field %d", q, a[q].size(), n[q]);
+        const char* why = build_axis_tables(a[q].data(), n[q], rule, t[q]);
+        if (why) return set_err(FQSL_ERR_INVALID, "axis %d: %s", q, why);
+    }
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t off64[3], off32[3], offr[3], offb[3], total = 0;
+    for (int q = 0; q < 3; ++q) { off64[q] = total; total = up(total + t[q].rec64.size() * sizeof(double4)); }
+    for (int q = 0; q < 3; ++q) { off32[q] = total; total = up(total + t[q].rec32.size() * sizeof(float4)); }
+    for (int q = 0; q < 3; ++q) { offr[q] = total; total = up(total + t[q].ref.size() * sizeof(int)); }
+    for (int q = 0; q < 3; ++q) { offb[q] = total; total = up(total + t[q].bins.size() * sizeof(int)); }
+    std::vector<char> host(total, 0);
+    for (int q = 0; q < 3; ++q) {
+        memcpy(host.data() + off64[q], t[q].rec64.data(), t[q].rec64.size() * sizeof(double4));
+        memcpy(host.data() + off32[q], t[q].rec32.data(), t[q].rec32.size() * sizeof(float4));
+        memcpy(host.data() + offr[q], t[q].ref.data(), t[q].ref.size() * sizeof(int));
+        memcpy(host.data() + offb[q], t[q].bins.data(), t[q].bins.size() * sizeof(int));
+    }
+    char* d = nullptr;
+    HIP_TRY(hipMalloc(&d, total));
+    hipError_t e = hipMemcpy(d, host.data(), total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return set_err(FQSL_ERR_HIP, "hipMemcpy(axis tables): %s", hipGetErrorString(e));
+    }
+    const void* rec64[3] = {d + off64[0], d + off64[1], d + off64[2]};
+    const void* rec32[3] = {d + off32[0], d + off32[1], d + off32[2]};
+    const int* ref[3] = {(const int*)(d + offr[0]), (const int*)(d + offr[1]), (const int*)(d + offr[2])};
+    const int* bins[3] = {(const int*)(d + offb[0]), (const int*)(d + offb[1]), (const int*)(d + offb[2])};
+    fill_grid<double>(t, rule, rec64, ref, bins, *g64);
+    fill_grid<float>(t, rule, rec32, ref, bins, *g32);
+    *mem = d;
+    *bytes = total;
+    return FQSL_OK;
+}
+
+// Host copy (double) of n values of an axis array in dtype.
+static int axis_to_host(const void* src, int n, int dtype, int on_device, std::vector<double>& out) {
+    out.resize(n);
+    const size_t esz = dtype == FQSL_DTYPE_F64 ? 8 : 4;
+    std::vector<char> raw(n * esz);
+    if (on_device) HIP_TRY(hipMemcpy(raw.data(), src, n * esz, hipMemcpyDeviceToHost));
+    else memcpy(raw.data(), src, n * esz);
+    for (int i = 0; i < n; ++i)
+        out[i] = dtype == FQSL_DTYPE_F64 ? ((const double*)raw.data())[i] : (double)((const float*)raw.data())[i];
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_field_set_axes(fqsl_field* f, const void* xa, const void* ya, const void* za, int dtype,
+                                   int on_device, int lookup) {
+    if (!f) return set_err(FQSL_ERR_INVALID, "field is NULL");
+    if (!xa && !ya && !za) {  // back to grid-index coordinates
+        if (f->axes_mem) { (void)hipFree(f->axes_mem); f->bytes -= f->axes_bytes; }
+        f->axes_mem = nullptr; f->axes_bytes = 0; f->stretched = false;
+        return FQSL_OK;
+    }
+    if (!xa || !ya || !za) return set_err(FQSL_ERR_INVALID, "axis pointer is NULL");
+    if (dtype != FQSL_DTYPE_F32 && dtype != FQSL_DTYPE_F64) return set_err(FQSL_ERR_INVALID, "dtype %d", dtype);
+    std::vector<double> a[3];
+    const void* src[3] = {xa, ya, za};
+    for (int q = 0; q < 3; ++q) {
+        int rc = axis_to_host(src[q], f->n[q], dtype, on_device, a[q]);
+        if (rc) return rc;
+    }
+    void* mem = nullptr;
+    size_t bytes = 0;
+    StretchGrid<float> g32;
+    StretchGrid<double> g64;
+    int rc = build_stretch(a, f->n, lookup, &mem, &bytes, &g32, &g64);
+    if (rc) return rc;
+    if (f->axes_mem) { (void)hipFree(f->axes_mem); f->bytes -= f->axes_bytes; }
+    f->axes_mem = mem; f->axes_bytes = bytes; f->bytes += bytes;
+    f->sg32 = g32; f->sg64 = g64; f->stretched = true;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_field_stretched(const fqsl_field* f) { return f && f->stretched ? 1 : 0; }
 
 // ------------------------------------------------------------- tuning knobs
 static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64};
@@ -760,6 +856,77 @@ __device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, c
     }
 }
 
+// The stretched tracer writes B_start / B_end / B_this by the stretched
+// Interp3dxyzn (raw, TraceBlineAdaptiveStretch.cu:463-496).
+template <typename R, class F>
+__device__ __forceinline__ void stretch_write(const PlainLine<R>& L, const F& f, const StretchGrid<R>& g,
+                                              const PlainParams& P, unsigned long long i, const PlainOut& o) {
+    const float xs[3] = {(float)L.xs[0], (float)L.xs[1], (float)L.xs[2]};
+    const float xe[3] = {(float)L.xe[0], (float)L.xe[1], (float)L.xe[2]};
+    if (o.sx) { o.sx[i] = xs[0]; o.sy[i] = xs[1]; o.sz[i] = xs[2]; }
+    if (o.ex) { o.ex[i] = xe[0]; o.ey[i] = xe[1]; o.ez[i] = xe[2]; }
+    if (o.fs) o.fs[i] = L.flag_s;
+    if (o.fe) o.fe[i] = L.flag_e;
+    if (o.len) o.len[i] = L.len_total;
+    if (o.evals) o.evals[i] = L.evals;
+    if (o.tw) o.tw[i] = L.tw;
+    double B[3];
+    if (o.bsx) {
+        const double p[3] = {xs[0], xs[1], xs[2]};
+        stretch_raw_at<R, F>(f, g, p, B);
+        o.bsx[i] = (float)B[0]; o.bsy[i] = (float)B[1]; o.bsz[i] = (float)B[2];
+    }
+    if (o.bex) {
+        const double p[3] = {xe[0], xe[1], xe[2]};
+        stretch_raw_at<R, F>(f, g, p, B);
+        o.bex[i] = (float)B[0]; o.bey[i] = (float)B[1]; o.bez[i] = (float)B[2];
+    }
+    const double p0[3] = {L.p0[0], L.p0[1], L.p0[2]};
+    stretch_raw_at<R, F>(f, g, p0, B);
+    const float b[3] = {(float)B[0], (float)B[1], (float)B[2]};
+    if (o.btx) { o.btx[i] = b[0]; o.bty[i] = b[1]; o.btz[i] = b[2]; }
+    if (o.bflag) {
+        const float bn = fabsf(b[0] * P.ncross[0] + b[1] * P.ncross[1] + b[2] * P.ncross[2]) * 100.f;
+        o.bflag[i] = bn < sqrtf(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]) ? 1 : 0;
+    }
+}
+
+// Geometry of the plain tracer: grid-index coordinates (the uniform tracers)
+// or a stretched grid; the overloads pick the line driver.
+struct UniformGrid {};
+
+template <typename R, class F>
+__device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const UniformGrid&, const PlainParams& P,
+                                          float x, float y, float z) {
+    plain_init<R>(L, f, P, x, y, z);
+}
+template <typename R, class F>
+__device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const StretchGrid<R>& g, const PlainParams& P,
+                                          float x, float y, float z) {
+    stretch_init<R, F>(L, f, g, P, x, y, z);
+}
+template <typename R, class F, class FJ>
+__device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const UniformGrid&, const PlainParams& P,
+                                             const FJ& fj) {
+    plain_attempt<R, F, FJ>(L, f, P, fj);
+}
+template <typename R, class F, class FJ>
+__device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const StretchGrid<R>& g,
+                                             const PlainParams& P, const FJ& fj) {
+    stretch_attempt<R, F, FJ>(L, f, g, P, fj);
+}
+template <typename R, class F>
+__device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, const UniformGrid&, const PlainParams& P,
+                                           unsigned long long i, const PlainOut& o, const DevScalars& ds) {
+    plain_write<R>(L, f, P, i, o, ds);
+}
+template <typename R, class F>
+__device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, const StretchGrid<R>& g,
+                                           const PlainParams& P, unsigned long long i, const PlainOut& o,
+                                           const DevScalars&) {
+    stretch_write<R>(L, f, g, P, i, o);
+}
+
 template <class F>
 __device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsigned long long& n,
                                                     const DevScalars& ds, const float* ncross) {
@@ -771,10 +938,10 @@ __device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsign
     P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];
 }
 
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, class G>
 __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Seeds s, unsigned long long n,
                                                           PlainOut out, DevScalars ds, const float* ncross,
-                                                          unsigned long long* queue, FJ fj) {
+                                                          unsigned long long* queue, FJ fj, G g) {
     apply_plain_scalars(f, P, n, ds, ncross);
     fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     PlainLine<R> L;
@@ -796,8 +963,8 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
                 if (!active) {
                     idx = base + (unsigned long long)__popcll(need & below);
                     if (idx < n) {
-                        plain_init<R>(L, f, P, s.x[idx], s.y[idx], s.z[idx]);
-                        if (L.done) plain_write<R>(L, f, P, idx, out, ds);
+                        line_init<R>(L, f, g, P, s.x[idx], s.y[idx], s.z[idx]);
+                        if (L.done) line_write<R>(L, f, g, P, idx, out, ds);
                         else active = true;
                     }
                 }
@@ -808,26 +975,27 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
             continue;
         }
         if (active) {
-            plain_attempt<R, F, FJ>(L, f, P, fj);
+            line_attempt<R, F, FJ>(L, f, g, P, fj);
             if (L.done) {
-                plain_write<R>(L, f, P, idx, out, ds);
+                line_write<R>(L, f, g, P, idx, out, ds);
                 active = false;
             }
         }
     }
 }
 
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, class G>
 __global__ void __launch_bounds__(256) k_plain_per_line(F f, PlainParams P, Seeds s, unsigned long long n,
-                                                        PlainOut out, DevScalars ds, const float* ncross, FJ fj) {
+                                                        PlainOut out, DevScalars ds, const float* ncross, FJ fj,
+                                                        G g) {
     apply_plain_scalars(f, P, n, ds, ncross);
     fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
         PlainLine<R> L;
-        plain_init<R>(L, f, P, s.x[i], s.y[i], s.z[i]);
-        while (!L.done) plain_attempt<R, F, FJ>(L, f, P, fj);
-        plain_write<R>(L, f, P, i, out, ds);
+        line_init<R>(L, f, g, P, s.x[i], s.y[i], s.z[i]);
+        while (!L.done) line_attempt<R, F, FJ>(L, f, g, P, fj);
+        line_write<R>(L, f, g, P, i, out, ds);
     }
 }
 
@@ -950,29 +1118,29 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
     return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
 }
 
-template <typename R, class F, class FJ = NoCurrent>
+template <typename R, class F, class FJ = NoCurrent, class G = UniformGrid>
 static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsigned long long n, const PlainOut& o,
                         const DevScalars& ds, const float* ncross, int schedule, hipStream_t st,
-                        unsigned long long n_hint, const FJ& fj = FJ{}) {
+                        unsigned long long n_hint, const FJ& fj = FJ{}, const G& g = G{}) {
     if (schedule == FQSL_SCHED_PERSISTENT) {
         unsigned long long* q = nullptr;
         int rc = queue_slot(&q);
         if (rc) return rc;
         int grid = 0;
-        rc = persistent_grid(k_plain_persistent<R, F, FJ>, &grid);
+        rc = persistent_grid(k_plain_persistent<R, F, FJ, G>, &grid);
         if (rc) return rc;
         k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
         HIP_TRY(hipGetLastError());
         rc = record_start(st);
         if (rc) return rc;
-        k_plain_persistent<R, F, FJ><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q, fj);
+        k_plain_persistent<R, F, FJ, G><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q, fj, g);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
         if (rc) return rc;
-        k_plain_per_line<R, F, FJ><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, ncross, fj);
+        k_plain_per_line<R, F, FJ, G><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, ncross, fj, g);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
@@ -1008,6 +1176,8 @@ extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params
     hipStream_t st = (hipStream_t)stream;
     const int* nn = field->n;
     if (o.tw && !field->cur32[0]) return set_err(FQSL_ERR_INVALID, "twist requested but the field has no current density");
+    if (field->stretched)
+        return set_err(FQSL_ERR_UNSUPPORTED, "the Scott tracer runs on grid-index coordinates; this field has stretched axes");
     if (p->precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
         FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
@@ -1376,6 +1546,7 @@ BatchLayout batch_layout(int nreq, int ngroups, unsigned long long lines, int pr
 
 int check_request(const fqsl_scott_request& q, int precision, int k) {
     if (!q.field) return set_err(FQSL_ERR_INVALID, "request %d: NULL field", k);
+    if (q.field->stretched) return set_err(FQSL_ERR_UNSUPPORTED, "request %d: field has stretched axes", k);
     if (q.method != FQSL_METHOD_RKF45 && q.method != FQSL_METHOD_RK4)
         return set_err(FQSL_ERR_INVALID, "request %d: method %d", k, q.method);
     if (!(q.step > 0) || !(q.step <= 100.0)) return set_err(FQSL_ERR_INVALID, "request %d: step %g outside (0, 100]", k, q.step);
@@ -1556,6 +1727,36 @@ extern "C" int fqsl_scott_batch_destroy(fqsl_scott_batch* b) {
     return FQSL_OK;
 }
 
+// The stretched tracer on a resident field (RKF45 only, as the reference).
+static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, const Seeds& s, unsigned long long n,
+                                 const PlainOut& o, int precision, int schedule, hipStream_t st) {
+    if (P.method != FQSL_METHOD_RKF45)
+        return set_err(FQSL_ERR_UNSUPPORTED, "a stretched grid is traced with RKF45 only (TraceBlineAdaptiveStretch.cu)");
+    const int* nn = field->n;
+    DevScalars ds{};
+    if (precision == FQSL_PREC_FP64) {
+        if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
+        FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) {
+            FieldSoA<double> fj{field->cur64[0], field->cur64[1], field->cur64[2], {nn[0], nn[1], nn[2]}};
+            return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, field->sg64);
+        }
+        return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, field->sg64);
+    }
+    const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
+    if (field->cell32) {
+        FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, field->sg32);
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, field->sg32);
+    }
+    if (field->soa32[0]) {
+        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, field->sg32);
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, field->sg32);
+    }
+    return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
 extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n, const float* x,
                                 const float* y, const float* z, const fqsl_plain_out* out, void* stream) {
     if (!field || !p || !out) return set_err(FQSL_ERR_INVALID, "NULL field/params/out");
@@ -1579,6 +1780,7 @@ extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params
     hipStream_t st = (hipStream_t)stream;
     const int* nn = field->n;
     if (o.tw && !field->cur32[0]) return set_err(FQSL_ERR_INVALID, "twist requested but the field has no current density");
+    if (field->stretched) return trace_plain_stretched(field, P, s, n, o, p->precision, p->schedule, st);
     if (p->precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
         FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
@@ -1685,6 +1887,94 @@ extern "C" int fqsl_TraceAllBline(int method, float* Bx, float* By, float* Bz, i
         return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, (hipStream_t)stream, n_hint, fj);
     }
     return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, (hipStream_t)stream, n_hint);
+}
+
+// Lookup tables of the stretched drop-in, per device, kept while the axes
+// and the rule of successive calls are unchanged.
+struct DropinAxes {
+    std::vector<double> a[3];
+    int rule = -1;
+    void* mem = nullptr;
+    StretchGrid<float> g;
+};
+static std::mutex g_dropin_mu;
+static DropinAxes g_dropin_axes[64];
+
+extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* BshapeN, float* x_arr, float* y_arr,
+                                         float* z_arr, bool* flag_uni, float* curB_x, float* curB_y, float* curB_z,
+                                         double* twist, bool* curB_flag, float* inp_x, float* inp_y, float* inp_z,
+                                         float* inp_cross_dir, float* start_x, float* start_y, float* start_z,
+                                         int* flag_start, float* end_x, float* end_y, float* end_z, int* flag_end,
+                                         float* B_this_x, float* B_this_y, float* B_this_z, int* B_flag,
+                                         float* B_start_x, float* B_start_y, float* B_start_z, float* B_end_x,
+                                         float* B_end_y, float* B_end_z, float* s_len, unsigned long long* N,
+                                         double* LineLen, float* tol_coef, int precision, int schedule,
+                                         void* stream) {
+    if (!Bx || !By || !Bz || !BshapeN || !x_arr || !y_arr || !z_arr || !inp_x || !inp_y || !inp_z ||
+        !inp_cross_dir || !s_len || !N || !tol_coef)
+        return set_err(FQSL_ERR_INVALID, "NULL required argument");
+    int rc = check_common(FQSL_METHOD_RKF45, precision, schedule);
+    if (rc) return rc;
+    if (precision != FQSL_PREC_FP32C)
+        return set_err(FQSL_ERR_UNSUPPORTED, "the in-place float field supports FQSL_PREC_FP32C only");
+    hipStream_t st = (hipStream_t)stream;
+    // the shape and the two switches are device scalars: read them (synchronises the stream)
+    int shape[3] = {0, 0, 0};
+    bool uni = false, twist_on = false;
+    HIP_TRY(hipMemcpyAsync(shape, BshapeN, sizeof(shape), hipMemcpyDeviceToHost, st));
+    if (flag_uni) HIP_TRY(hipMemcpyAsync(&uni, flag_uni, sizeof(bool), hipMemcpyDeviceToHost, st));
+    if (curB_flag) HIP_TRY(hipMemcpyAsync(&twist_on, curB_flag, sizeof(bool), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (shape[0] < 2 || shape[1] < 2 || shape[2] < 2)
+        return set_err(FQSL_ERR_INVALID, "BshapeN (%d,%d,%d): each axis needs >= 2 nodes", shape[0], shape[1], shape[2]);
+    if (twist_on && (!curB_x || !curB_y || !curB_z || !twist))
+        return set_err(FQSL_ERR_INVALID, "curB_flag is set: curB_x/y/z and twist must be device arrays");
+    std::vector<double> a[3];
+    const float* src[3] = {x_arr, y_arr, z_arr};
+    for (int q = 0; q < 3; ++q) {
+        rc = axis_to_host(src[q], shape[q], FQSL_DTYPE_F32, 1, a[q]);
+        if (rc) return rc;
+    }
+    const int rule = uni ? kLookupUniform : kLookupBisect;  // GetIndexUniStretch / GetIndexBiSearch
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_err(FQSL_ERR_UNSUPPORTED, "device index %d", dev);
+    StretchGrid<float> g;
+    {
+        std::lock_guard<std::mutex> lock(g_dropin_mu);
+        DropinAxes& c = g_dropin_axes[dev];
+        const bool same = c.mem && c.rule == rule && c.a[0] == a[0] && c.a[1] == a[1] && c.a[2] == a[2];
+        if (!same) {
+            void* mem = nullptr;
+            size_t bytes = 0;
+            StretchGrid<double> g64;
+            rc = build_stretch(a, shape, rule, &mem, &bytes, &c.g, &g64);
+            if (rc) return rc;
+            if (c.mem) {  // earlier launches may still read the old tables
+                HIP_TRY(hipDeviceSynchronize());
+                (void)hipFree(c.mem);
+            }
+            c.mem = mem;
+            c.rule = rule;
+            for (int q = 0; q < 3; ++q) c.a[q] = a[q];
+        }
+        g = c.g;
+    }
+    PlainParams P;
+    P.method = FQSL_METHOD_RKF45; P.s0 = 0.0; P.tol_coef = 1.0;
+    P.ncross[0] = P.ncross[1] = P.ncross[2] = 0.f;
+    Seeds s{inp_x, inp_y, inp_z};
+    PlainOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
+               B_flag, B_start_x, B_start_y, B_start_z, B_end_x, B_end_y, B_end_z, LineLen, nullptr,
+               twist_on ? twist : nullptr};
+    DevScalars ds{BshapeN, s_len, N, tol_coef, curB_flag, twist};
+    FieldSoA<float> f{Bx, By, Bz, {shape[0], shape[1], shape[2]}};
+    const unsigned long long n_hint = (unsigned long long)kPerLineMaxBlocks * 256;
+    if (twist_on) {
+        const FieldSoA<float> fj{curB_x, curB_y, curB_z, {shape[0], shape[1], shape[2]}};
+        return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, fj, g);
+    }
+    return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, NoCurrent{}, g);
 }
 
 // ---------------------------------------------- neighbour-difference assembly

@@ -49,10 +49,17 @@ class Field:
 
     ``current`` = (Jx, Jy, Jz) on the same grid (e.g. from
     ``FastQSL.curl_of_B_grid``) enables the twist output of the tracers.
+    ``axes`` = (xa, ya, za) makes the grid stretched (node (i, j, k) at
+    (xa[i], ya[j], za[k])): ``trace_plain`` then runs the stretched RKF45
+    tracer of TraceBlineAdaptiveStretch.cu in those coordinates, with the
+    cell rule ``lookup`` ("bisect": GetIndexBiSearch, the reference with
+    flag_uni false; "uniform": GetIndexUniStretch, flag_uni true; "exact":
+    the cell that holds the point, landing on the box faces).
     Accepts NumPy arrays or torch tensors (CPU or GPU, shape (nx, ny, nz)).
     """
 
-    def __init__(self, bx, by, bz, layouts=("soa64", "cell32"), dtype=None, current=None):
+    def __init__(self, bx, by, bz, layouts=("soa64", "cell32"), dtype=None, current=None, axes=None,
+                 lookup="bisect"):
         L = _lib.lib()
         mask = 0
         for name in layouts:
@@ -66,8 +73,12 @@ class Field:
         _lib.check(L.fqsl_field_create(ptrs[0], ptrs[1], ptrs[2], *self.shape, dt, on_dev, mask,
                                        ctypes.byref(self._h)))
         del keep
+        self.axes = None
+        self.lookup = None
         if current is not None:
             self.set_current(*current)
+        if axes is not None:
+            self.set_axes(*axes, lookup=lookup)
 
     def set_current(self, jx, jy, jz):
         """Attach the current density J (same grid as B) for the twist number."""
@@ -77,6 +88,37 @@ class Field:
         _lib.check(_lib.lib().fqsl_field_set_current(self.handle, ptrs[0], ptrs[1], ptrs[2], dt, on_dev))
         del keep
         self.has_current = True
+
+    def set_axes(self, xa, ya, za, lookup="bisect"):
+        """Node coordinates of a stretched grid (strictly increasing, one per
+        node); ``set_axes(None, None, None)`` returns to grid-index
+        coordinates."""
+        L = _lib.lib()
+        if xa is None and ya is None and za is None:
+            _lib.check(L.fqsl_field_set_axes(self.handle, None, None, None, _lib.DTYPE_F64, 0, 0))
+            self.axes = None
+            self.lookup = None
+            return
+        if lookup not in _lib.LOOKUPS:
+            raise ValueError(f"lookup must be one of {sorted(_lib.LOOKUPS)}")
+        arrs = []
+        for k, a in enumerate((xa, ya, za)):
+            torch = _maybe_torch(a)
+            a = a.detach().cpu().numpy() if torch is not None else np.asarray(a)
+            if a.ndim != 1 or a.shape[0] != self.shape[k]:
+                raise ValueError(f"axis {k} has shape {a.shape}; the field has {self.shape[k]} nodes")
+            arrs.append(np.ascontiguousarray(a, dtype=np.float64 if a.dtype == np.float64 else np.float32))
+        if len({a.dtype for a in arrs}) != 1:
+            arrs = [a.astype(np.float64) for a in arrs]
+        dt = _lib.DTYPE_F64 if arrs[0].dtype == np.float64 else _lib.DTYPE_F32
+        _lib.check(L.fqsl_field_set_axes(self.handle, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                                         arrs[2].ctypes.data, dt, 0, _lib.LOOKUPS[lookup]))
+        self.axes = tuple(a.copy() for a in arrs)
+        self.lookup = lookup
+
+    @property
+    def stretched(self) -> bool:
+        return bool(_lib.lib().fqsl_field_stretched(self.handle))
 
     @property
     def handle(self):

@@ -10,7 +10,8 @@
 
 The argument tuple has the reference kernel's parameter order
 (TraceBlineAdaptive.cu:429-437, TraceBline.cu:383-391,
-TraceBlineScott.cu:521-528).  Arrays may be torch CUDA tensors (used in place,
+TraceBlineScott.cu:521-528, TraceBlineAdaptiveStretch.cu:422-431 for
+``compileTraceBlineAdaptiveStretch()``).  Arrays may be torch CUDA tensors (used in place,
 as CuPy arrays are) or NumPy arrays (copied to the device and the outputs
 copied back into them); scalars are one-element arrays as in the reference.
 Memory is read as laid out, exactly like the reference: field arrays must
@@ -49,6 +50,13 @@ PLAIN_PARAMS = _COMMON_HEAD + [
     ("B_start_x", _F32, True), ("B_start_y", _F32, True), ("B_start_z", _F32, True),
     ("B_end_x", _F32, True), ("B_end_y", _F32, True), ("B_end_z", _F32, True),
     ("s_len", _F32, False), ("N", _U64, False), ("LineLen", _F64, True), ("tol_coef", _F32, False)]
+
+# TraceAllBline of TraceBlineAdaptiveStretch.cu:422-431: the axes and flag_uni
+# after BshapeN
+STRETCH_PARAMS = (_COMMON_HEAD[:4] +
+                  [("x_arr", _F32, False), ("y_arr", _F32, False), ("z_arr", _F32, False),
+                   ("flag_uni", _BOOL, False)] +
+                  _COMMON_HEAD[4:] + PLAIN_PARAMS[len(_COMMON_HEAD):])
 
 SCOTT_PARAMS = _COMMON_HEAD + [
     ("q_0", _F32, True), ("q_perp", _F32, True),
@@ -94,11 +102,15 @@ class _Binding:
 
 
 class _Kernel:
-    def __init__(self, name, params, method=None, accept_extra_tol=False):
+    def __init__(self, name, params, method=None, accept_extra_tol=False, stretch=False):
         self.name = name
         self.params = params
         self.method = method
         self.accept_extra_tol = accept_extra_tol
+        self.stretch = stretch
+        names = [p[0] for p in params]
+        self._flag_i = names.index("curB_flag")
+        self._twist_i = names.index("twist")
         # run-time selection of the tracer (not part of the reference call)
         self.precision = "fp32c"
         self.schedule = "persistent"
@@ -111,19 +123,21 @@ class _Kernel:
             args = args + (np.ones(1, np.float32),)  # TraceBline.cu declares no tol_coef
         if len(args) != n_expected:
             raise TypeError(f"{self.name} takes {n_expected} arguments ({len(args)} given)")
-        flag = args[8]
+        flag = args[self._flag_i]
         flag_val = bool(flag.reshape(-1)[0].item()) if hasattr(flag, "reshape") else bool(flag)
         if not flag_val:
             # twist off: the notebooks pass float32 dummies for curB and twist,
             # which the library never reads
-            twist_i = 7
+            twist_i = self._twist_i
             args = args[:twist_i] + (np.zeros(1, np.float64),) + args[twist_i + 1:]
         b = _Binding(self.params, args)
         st = stream if stream is not None else torch.cuda.current_stream()
         L = _lib.lib()
         prec = _lib.PRECISIONS[self.precision]
         sched = _lib.SCHEDULES[self.schedule]
-        if self.method is None:
+        if self.stretch:
+            rc = L.fqsl_TraceAllBlineStretch(*b.ptrs, prec, sched, ctypes.c_void_p(st.cuda_stream))
+        elif self.method is None:
             rc = L.fqsl_TraceAllBlineScott(*b.ptrs, prec, sched, ctypes.c_void_p(st.cuda_stream))
         else:
             rc = L.fqsl_TraceAllBline(self.method, *b.ptrs, prec, sched, ctypes.c_void_p(st.cuda_stream))
@@ -147,6 +161,18 @@ def compileTraceBlineRK4():
     reference kernel has no tol_coef parameter; a trailing one is accepted
     and ignored (demo/Q_RK4.ipynb passes it)."""
     return _Kernel("TraceAllBline", PLAIN_PARAMS, method=_lib.METHODS["rk4"], accept_extra_tol=True)
+
+
+def compileTraceBlineAdaptiveStretch():
+    """TraceAllBline of TraceBlineAdaptiveStretch.cu (RKF45 on a stretched
+    grid; fastqslpy/kernels.py lists the module but has no loader for it, and
+    demo/Q_RKF45CuPyStretch.ipynb cell [9] compiles the file directly).  Its
+    39 arguments put the axes x_arr, y_arr, z_arr (float32, one coordinate per
+    node) and flag_uni (bool: GetIndexUniStretch instead of GetIndexBiSearch)
+    after BshapeN; seeds and results are in the axes' coordinates.  The call
+    reads BshapeN, flag_uni, curB_flag and the axes to the host (one stream
+    synchronisation)."""
+    return _Kernel("TraceAllBline", STRETCH_PARAMS, method=_lib.METHODS["rkf45"], stretch=True)
 
 
 def compileTraceBlineScott():

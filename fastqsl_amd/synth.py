@@ -11,6 +11,8 @@ Workloads:
 * ``multipole_256``     40 random point sources (default_rng(seed)), 256^3 float32
 * ``rope_arcade``       Gold-Hoyle flux rope along x plus two bipoles, 512x256x256
 * ``multipole``         the random-multipole generator at any shape/seed
+* ``stretched_multipole`` the same sources on a stretched (non-uniform) grid,
+                        traced in the axes' coordinates
 
 Generation runs in torch (on the GPU when one is present) in float64 and is
 cast to the requested dtype at the end; the result is returned as NumPy.
@@ -53,11 +55,16 @@ def _default_device(device):
 
 
 def point_sources(shape, charges, positions, box=((-1, 1), (-1, 1), (0, 2)), dtype=np.float32,
-                  device=None, zchunk=16):
-    """B = sum_i q_i (r - r_i) / |r - r_i|^3 sampled at the grid nodes."""
+                  device=None, zchunk=16, axes=None):
+    """B = sum_i q_i (r - r_i) / |r - r_i|^3 sampled at the grid nodes (evenly
+    spaced over ``box``, or at the node coordinates ``axes`` = (xa, ya, za))."""
     torch = _torch()
     device = _default_device(device)
-    xs, ys, zs = _axes(shape, box, device, torch)
+    if axes is None:
+        xs, ys, zs = _axes(shape, box, device, torch)
+    else:
+        xs, ys, zs = (torch.as_tensor(np.asarray(a, dtype=np.float64), device=device) for a in axes)
+        shape = (xs.shape[0], ys.shape[0], zs.shape[0])
     nx, ny, nz = shape
     bx = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
     by = torch.zeros_like(bx)
@@ -140,6 +147,45 @@ def rope_arcade(shape=(512, 256, 256), dtype=np.float32, device=None, turns=1.5,
     add = [t.unsqueeze(2).expand(-1, -1, nx).to("cpu").numpy().transpose(2, 1, 0)
            for t in (bxr, byr, bzr)]
     return tuple((b + a).astype(dtype) for b, a in zip((bx, by, bz), add))
+
+
+def stretched_axis(n, lo, hi, kind="tanh", strength=2.0, dtype=np.float32):
+    """n strictly increasing node coordinates from lo to hi.  ``sinh``: nodes
+    cluster at the centre (spacing ratio about cosh(strength) from the ends to
+    the centre); ``tanh``: nodes cluster at both ends; ``geometric``: spacing
+    grows by a constant factor (1 + strength / n) from lo upward, as a
+    solar-atmosphere z axis does; ``uniform``: evenly spaced."""
+    t = np.linspace(0.0, 1.0, n)
+    if kind == "sinh":
+        u = 0.5 + 0.5 * np.sinh(strength * (2 * t - 1)) / np.sinh(strength)
+    elif kind == "tanh":
+        u = 0.5 + 0.5 * np.tanh(strength * (2 * t - 1)) / np.tanh(strength)
+    elif kind == "geometric":
+        r = 1.0 + strength / n
+        w = r ** np.arange(n - 1)
+        u = np.r_[0.0, np.cumsum(w)] / np.sum(w)
+    elif kind == "uniform":
+        u = t
+    else:
+        raise ValueError(f"unknown axis kind {kind!r}")
+    a = (lo + (hi - lo) * u).astype(dtype)
+    a[0], a[-1] = dtype(lo), dtype(hi)
+    if np.any(np.diff(a.astype(np.float64)) <= 0):
+        raise ValueError("axis is not strictly increasing at this precision")
+    return a
+
+
+def stretched_multipole(shape=(256, 256, 256), seed=0, count=40, dtype=np.float32, device=None,
+                        box=((-1.0, 1.0), (-1.0, 1.0), (0.0, 2.0)), kinds=("sinh", "sinh", "geometric"),
+                        strength=(1.5, 1.5, 3.0)):
+    """The random multipole of ``multipole`` sampled on a stretched grid: x
+    and y clustered at the centre, z spacing growing with height.  Returns
+    ((bx, by, bz), (xa, ya, za)) with float32 axes in the box's units."""
+    axes = tuple(stretched_axis(n, lo, hi, kind, st)
+                 for n, (lo, hi), kind, st in zip(shape, box, kinds, strength))
+    q, pos = multipole_sources(seed, count)
+    field = point_sources(shape, q, pos, dtype=dtype, device=device, axes=axes)
+    return field, axes
 
 
 def hyperbolic(shape, k=0.02, b0=1.0, dtype=np.float64, center=None):

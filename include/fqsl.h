@@ -80,6 +80,27 @@ int fqsl_field_shape(const fqsl_field *field, int *nx, int *ny, int *nz);
 int fqsl_field_set_current(fqsl_field *field, const void *jx, const void *jy, const void *jz, int dtype,
                            int on_device);
 
+/* Give the field stretched (non-uniform) axes: node (i, j, k) at
+ * (xa[i], ya[j], za[k]), each axis strictly increasing with the field's
+ * node count.  Seeds and results of fqsl_trace_plain are then in these
+ * coordinates, and it runs the stretched RKF45 tracer of
+ * TraceBlineAdaptiveStretch.cu:323-508 (the reference passes x_arr / y_arr /
+ * z_arr and flag_uni per launch, :422-431).  lookup picks the interpolation
+ * cell: FQSL_LOOKUP_BISECT (GetIndexBiSearch, :42-65, the reference with
+ * flag_uni false), FQSL_LOOKUP_UNIFORM (GetIndexUniStretch, :68-75, flag_uni
+ * true) -- both land face exits on the plane 0 / n_dim as the reference does
+ * (:385-386) -- or FQSL_LOOKUP_EXACT (the cell holding the point; landing on
+ * the box faces).  All three axis pointers NULL returns the field to
+ * grid-index coordinates.  Scott tracing and batches reject a stretched
+ * field.  Synchronous. */
+#define FQSL_LOOKUP_BISECT 0
+#define FQSL_LOOKUP_UNIFORM 1
+#define FQSL_LOOKUP_EXACT 2
+int fqsl_field_set_axes(fqsl_field *field, const void *xa, const void *ya, const void *za, int dtype,
+                        int on_device, int lookup);
+/* 1 if the field has stretched axes, else 0. */
+int fqsl_field_stretched(const fqsl_field *field);
+
 /* ----------------------------------------------------- current density */
 /* curl B on the node grid of axes ax/ay/az (device double arrays of nx, ny,
  * nz coordinates), FastQSL.curl_of_B_grid (fastqslpy/FastQSL.py:120-131):
@@ -267,6 +288,23 @@ int fqsl_TraceAllBline(int method, float *Bx, float *By, float *Bz, int *BshapeN
                        float *B_start_z, float *B_end_x, float *B_end_y, float *B_end_z,
                        float *s_len, unsigned long long *N, double *LineLen, float *tol_coef,
                        int precision, int schedule, void *stream);
+
+/* TraceAllBline of TraceBlineAdaptiveStretch.cu:422-508 with its 39
+ * arguments in the reference's order (device pointers; BshapeN, flag_uni,
+ * curB_flag, s_len, N and tol_coef are one-element device arrays).  Reads
+ * BshapeN, flag_uni, curB_flag and the three axes to the host (one stream
+ * synchronisation), builds the axes' lookup tables (kept for the next call
+ * while axes and flag_uni are unchanged), then traces the in-place float
+ * field.  precision must be FQSL_PREC_FP32C. */
+int fqsl_TraceAllBlineStretch(float *Bx, float *By, float *Bz, int *BshapeN, float *x_arr, float *y_arr,
+                              float *z_arr, bool *flag_uni, float *curB_x, float *curB_y, float *curB_z,
+                              double *twist, bool *curB_flag, float *inp_x, float *inp_y, float *inp_z,
+                              float *inp_cross_dir, float *start_x, float *start_y, float *start_z,
+                              int *flag_start, float *end_x, float *end_y, float *end_z, int *flag_end,
+                              float *B_this_x, float *B_this_y, float *B_this_z, int *B_flag,
+                              float *B_start_x, float *B_start_y, float *B_start_z, float *B_end_x,
+                              float *B_end_y, float *B_end_z, float *s_len, unsigned long long *N,
+                              double *LineLen, float *tol_coef, int precision, int schedule, void *stream);
 
 /* ---------------------------------------------------------- tuning knobs */
 /* Process-wide launch parameters (fastqsl_amd._lib.tune_set / tune_get).

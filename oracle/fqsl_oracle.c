@@ -25,11 +25,11 @@
  *                                             TraceBlineAdaptive.cu:333-426,
  *                                             TraceBline.cu:300-378
  *   orc_trace_plain   TraceAllBline           TraceBlineAdaptive.cu:429-510
- *   lookup_bisect     GetIndexBiSearch        TraceBlineAdaptiveStretch.cu:43-64
- *   lookup_uniform    GetIndexUniStretch      TraceBlineAdaptiveStretch.cu:67-74
+ *   lookup_bisect     GetIndexBiSearch        TraceBlineAdaptiveStretch.cu:42-65
+ *   lookup_uniform    GetIndexUniStretch      TraceBlineAdaptiveStretch.cu:68-75
  *   stretch_cell      Interp3dxyzn clamp and weights
- *                                             TraceBlineAdaptiveStretch.cu:80-124
- *   trace_stretch_dir TraceBlineAdap          TraceBlineAdaptiveStretch.cu:292-419
+ *                                             TraceBlineAdaptiveStretch.cu:82-124
+ *   trace_stretch_dir TraceBlineAdap          TraceBlineAdaptiveStretch.cu:323-419
  *   orc_trace_stretch TraceAllBline           TraceBlineAdaptiveStretch.cu:422-508
  *
  * Where the reference reads uninitialised memory or leaves outputs undefined,
@@ -121,7 +121,7 @@ static void axis_cell(double p, int n, int *idx, double *r) {
     *idx = ii; *r = rr;
 }
 
-/* GetIndexBiSearch (TraceBlineAdaptiveStretch.cu:43-64): bisection that
+/* GetIndexBiSearch (TraceBlineAdaptiveStretch.cu:42-65): bisection that
  * stops once the bracket spans at most three cells and then decides between
  * the bracket's middle node and the one below it only, so a point in the
  * bracket's top cell is given the cell below (and extrapolated from it). */
@@ -135,7 +135,7 @@ static int lookup_bisect(double p, int n, const double *a) {
     return r > n - 2 ? n - 2 : r;
 }
 
-/* GetIndexUniStretch (TraceBlineAdaptiveStretch.cu:67-74): the cell index of
+/* GetIndexUniStretch (TraceBlineAdaptiveStretch.cu:68-75): the cell index of
  * an evenly spaced axis, with the span divided by n (not n - 1). */
 static int lookup_uniform(double p, int n, const double *a) {
     double r = floor((p - a[0]) / ((a[n - 1] - a[0]) / (double)n));
@@ -152,7 +152,7 @@ static int lookup_exact(double p, int n, const double *a) {
     return c;
 }
 
-/* Interp3dxyzn of the stretched tracer (TraceBlineAdaptiveStretch.cu:80-194):
+/* Interp3dxyzn of the stretched tracer (TraceBlineAdaptiveStretch.cu:82-194):
  * the point is first pulled into [a0 + 1e-6, a_{n-1} - 1e-6] (NaN goes to the
  * low bound), then the rule picks the cell c and the weight is
  * (p - a[c]) / (a[c+1] - a[c]), outside [0, 1] when the cell does not hold p. */
@@ -723,14 +723,14 @@ static void trace_plain_dir(const field_t *f, const field_t *cur, const double *
 #define STRETCH_LEN_RATIO 2.0   /* MAX_STEP_RATIO, TraceBlineAdaptiveStretch.cu:11 */
 
 /* TraceBlineAdap of the stretched tracer (TraceBlineAdaptiveStretch.cu:
- * 292-419), one direction, positions in the axes' coordinates: the RKF45
+ * 323-419), one direction, positions in the axes' coordinates: the RKF45
  * step and controller of the uniform tracer, the box [a0, a_{n-1}] per axis,
  * len_lim = 2 (sum of the three spans), no step-count limit, and a face
  * landing on the plane p_mid = 0 (min face) or n_dim (max face) as the
- * reference writes it (:386) -- or on the face itself (a0 / a_{n-1}) when
+ * reference writes it (:385-386) -- or on the face itself (a0 / a_{n-1}) when
  * the axes use the exact lookup.  With cur, the twist of each step at its
  * start point and, unlike the uniform RKF45 tracer, the landing term
- * J.b/|b|^2 with b the normalised B at p1 (:407-410). */
+ * J.b/|b|^2 with b the normalised B at p1 (:403-406). */
 static void trace_stretch_dir(const field_t *f, const field_t *cur, const double *p0, double s,
                               double tol_coef, const double *ncross, double dir,
                               double *pout, int *flag_out, double *len_out, long *evals_out,
@@ -746,7 +746,7 @@ static void trace_stretch_dir(const field_t *f, const field_t *cur, const double
     memcpy(p1, p0, sizeof(p1));
     memcpy(pout, p0, sizeof(p1));
     interp_vec(f, p0, b0, 1);
-    {                                                  /* :311-317 */
+    {                                                  /* :345-352 */
         double bn = fabs(dot3(b0, ncross));
         tol = (bn <= 0.05) ? TOL_BASE / 8e3 : TOL_BASE * pow(bn, 1.5);
         tol = tol * tol_coef;
@@ -945,7 +945,7 @@ int orc_trace_plain(const double *bx, const double *by, const double *bz, int nx
 }
 
 /* Cell index and weight along one stretched axis at n points
- * (Interp3dxyzn's lookup, TraceBlineAdaptiveStretch.cu:80-124). */
+ * (Interp3dxyzn's lookup, TraceBlineAdaptiveStretch.cu:82-124). */
 void orc_stretch_cell(const double *a, int n, int rule, long npts, const double *p, int *cell, double *w) {
     axes_t ax = {{a, a, a}, rule};
     long i;

@@ -17,3 +17,10 @@ def oracle():
     from oracle import oracle as O
     O.build()
     return O
+
+
+@pytest.fixture(scope="session")
+def hostcheck():
+    from tests.hostcheck import hc
+    hc.build()
+    return hc

@@ -17,8 +17,8 @@ _lib = None
 
 def build():
     src = os.path.join(HERE, "hostcheck.cpp")
-    hdr = os.path.join(ROOT, "fastqsl_amd", "csrc", "fqsl_tracer.h")
-    if os.path.exists(LIB) and os.path.getmtime(LIB) > max(os.path.getmtime(src), os.path.getmtime(hdr)):
+    hdrs = [os.path.join(ROOT, "fastqsl_amd", "csrc", h) for h in ("fqsl_tracer.h", "fqsl_stretch.h")]
+    if os.path.exists(LIB) and os.path.getmtime(LIB) > max(os.path.getmtime(x) for x in [src] + hdrs):
         return LIB
     subprocess.run(["hipcc", "-x", "hip", "--offload-host-only", "-O2", "-fPIC", "-shared", "-std=c++17",
                     "-I" + os.path.join(ROOT, "fastqsl_amd", "csrc"), src, "-o", LIB], check=True)
@@ -94,3 +94,43 @@ def pow_minus_tenth(e2):
     L.hc_pow_minus_tenth.argtypes = [ctypes.c_long, _dp, _dp]
     L.hc_pow_minus_tenth(e.size, P(e), P(out))
     return out
+
+
+RULES = {"bisect": 0, "uniform": 1, "exact": 2}
+
+
+def stretch_cell(axis, pts, rule="bisect", precision=1):
+    a = np.ascontiguousarray(np.asarray(axis, np.float64))
+    p = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1))
+    cell = np.empty(p.size, np.int32)
+    w = np.empty(p.size)
+    L = lib()
+    L.hc_stretch_cell.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp, _ip, _dp]
+    rc = L.hc_stretch_cell(P(a), a.size, RULES[rule], precision, p.size, P(p), P(cell, _ip), P(w))
+    assert rc == 0
+    return cell, w
+
+
+def trace_stretch(f, axes, seeds, ncross=(0, 0, 1), rule="bisect", precision=1, s0=0.01, tol_coef=1.0,
+                  current=None):
+    (bx, by, bz), sh = _field(f)
+    xa, ya, za = [np.ascontiguousarray(np.asarray(a, np.float64)) for a in axes]
+    j = _field(current)[0] if current is not None else (None, None, None)
+    s = np.ascontiguousarray(np.asarray(seeds, np.float32).reshape(-1, 3))
+    nc = np.ascontiguousarray(np.asarray(ncross, np.float32))
+    n = s.shape[0]
+    o = dict(start=np.empty((n, 3)), end=np.empty((n, 3)), flag_start=np.empty(n, np.int32),
+             flag_end=np.empty(n, np.int32), length=np.empty(n), evals=np.empty(n, np.int64),
+             twist=np.zeros(n))
+    L = lib()
+    L.hc_trace_stretch.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _fp,
+                                   ctypes.c_long, _fp, _dp, _dp, _ip, _ip, _dp, _lp, _dp, _dp, _dp, _dp]
+    op = lambda a: None if a is None else P(a)
+    rc = L.hc_trace_stretch(P(bx), P(by), P(bz), *sh, P(xa), P(ya), P(za), RULES[rule], precision, s0,
+                            tol_coef, P(nc, _fp), n, P(s, _fp), P(o["start"]), P(o["end"]),
+                            P(o["flag_start"], _ip), P(o["flag_end"], _ip), P(o["length"]),
+                            P(o["evals"], _lp), op(j[0]), op(j[1]), op(j[2]),
+                            P(o["twist"]) if current is not None else None)
+    assert rc == 0
+    return o

@@ -1,4 +1,4 @@
-// Test-only CPU build of fastqsl_amd/csrc/fqsl_tracer.h: runs the per-line
+// Test-only CPU build of fastqsl_amd/csrc/fqsl_tracer.h and fqsl_stretch.h: runs the per-line
 // tracer code of the HIP kernels on the host so that its logic (the exact
 // difference-quotient stencil, the compensated position, the driver) can be
 // compared with the FP64 oracle without a GPU.  Never part of the product.
@@ -6,6 +6,7 @@
 
 #include <vector>
 
+#include "fqsl_stretch.h"
 #include "fqsl_tracer.h"
 
 using namespace fqsl;
@@ -131,5 +132,94 @@ void hc_trace_plain(const double* bx, const double* by, const double* bz, int nx
         }
     }
 }
+
+// Cell and weight along one stretched axis by the device lookup (bin table,
+// cell records, rule map) at npts points; precision 1 = double tables, 0 =
+// float tables.
+int hc_stretch_cell(const double* a, int n, int rule, int precision, long npts, const double* p, int* cell,
+                    double* w) {
+    AxisTables t[3];
+    for (int q = 0; q < 3; ++q)
+        if (build_axis_tables(a, n, rule, t[q])) return -1;
+    const void* r64[3] = {t[0].rec64.data(), t[1].rec64.data(), t[2].rec64.data()};
+    const void* r32[3] = {t[0].rec32.data(), t[1].rec32.data(), t[2].rec32.data()};
+    const int* ref[3] = {t[0].ref.data(), t[1].ref.data(), t[2].ref.data()};
+    const int* bins[3] = {t[0].bins.data(), t[1].bins.data(), t[2].bins.data()};
+    StretchGrid<double> g64;
+    StretchGrid<float> g32;
+    fill_grid<double>(t, rule, r64, ref, bins, g64);
+    fill_grid<float>(t, rule, r32, ref, bins, g32);
+    for (long i = 0; i < npts; ++i) {
+        if (precision == 1) {
+            double wd;
+            stretch_locate<double>(g64.ax[0], rule, p[i], 0.0, cell[i], wd);
+            w[i] = wd;
+        } else {
+            const float hi = (float)p[i], lo = (float)(p[i] - (double)hi);
+            float wf;
+            stretch_locate<float>(g32.ax[0], rule, hi, lo, cell[i], wf);
+            w[i] = wf;
+        }
+    }
+    return 0;
 }
 
+// The stretched tracer of the kernels on the host; precision 1 = FP64 over
+// the double SoA field, 0 = FP32c over the coefficient cells.
+int hc_trace_stretch(const double* bx, const double* by, const double* bz, int nx, int ny, int nz,
+                     const double* xa, const double* ya, const double* za, int rule, int precision, double s0,
+                     double tol_coef, const float* ncross, long n, const float* seeds, double* start, double* end,
+                     int* fs, int* fe, double* len, long* evals, const double* jx, const double* jy,
+                     const double* jz, double* twist) {
+    const int nn3[3] = {nx, ny, nz};
+    const double* ax[3] = {xa, ya, za};
+    AxisTables t[3];
+    for (int q = 0; q < 3; ++q)
+        if (build_axis_tables(ax[q], nn3[q], rule, t[q])) return -1;
+    const void* r64[3] = {t[0].rec64.data(), t[1].rec64.data(), t[2].rec64.data()};
+    const void* r32[3] = {t[0].rec32.data(), t[1].rec32.data(), t[2].rec32.data()};
+    const int* ref[3] = {t[0].ref.data(), t[1].ref.data(), t[2].ref.data()};
+    const int* bins[3] = {t[0].bins.data(), t[1].bins.data(), t[2].bins.data()};
+    StretchGrid<double> g64;
+    StretchGrid<float> g32;
+    fill_grid<double>(t, rule, r64, ref, bins, g64);
+    fill_grid<float>(t, rule, r32, ref, bins, g32);
+    PlainParams P;
+    P.method = kRKF45; P.s0 = s0; P.tol_coef = tol_coef;
+    P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];
+    const long nn = (long)nx * ny * nz;
+    std::vector<float> fx(bx, bx + nn), fy(by, by + nn), fz(bz, bz + nn);
+    std::vector<float4> cells = build_cells(fx.data(), fy.data(), fz.data(), nx, ny, nz);
+    FieldSoA<double> fd{bx, by, bz, {nx, ny, nz}};
+    FieldCell32 fc{cells.data(), {nx, ny, nz}};
+    std::vector<float> cx, cy, cz;
+    if (jx) { cx.assign(jx, jx + nn); cy.assign(jy, jy + nn); cz.assign(jz, jz + nn); }
+    FieldSoA<double> jd{jx, jy, jz, {nx, ny, nz}};
+    FieldSoA<float> jf{cx.data(), cy.data(), cz.data(), {nx, ny, nz}};
+    for (long i = 0; i < n; ++i) {
+        const float* s = seeds + 3 * i;
+        if (precision == 1) {
+            PlainLine<double> L;
+            stretch_init<double>(L, fd, g64, P, s[0], s[1], s[2]);
+            while (!L.done) {
+                if (jx) stretch_attempt<double, FieldSoA<double>, FieldSoA<double>>(L, fd, g64, P, jd);
+                else stretch_attempt<double>(L, fd, g64, P);
+            }
+            for (int a = 0; a < 3; ++a) { start[3 * i + a] = L.xs[a]; end[3 * i + a] = L.xe[a]; }
+            fs[i] = L.flag_s; fe[i] = L.flag_e; len[i] = L.len_total; evals[i] = L.evals;
+            if (twist) twist[i] = L.tw;
+        } else {
+            PlainLine<float> L;
+            stretch_init<float>(L, fc, g32, P, s[0], s[1], s[2]);
+            while (!L.done) {
+                if (jx) stretch_attempt<float, FieldCell32, FieldSoA<float>>(L, fc, g32, P, jf);
+                else stretch_attempt<float>(L, fc, g32, P);
+            }
+            for (int a = 0; a < 3; ++a) { start[3 * i + a] = L.xs[a]; end[3 * i + a] = L.xe[a]; }
+            fs[i] = L.flag_s; fe[i] = L.flag_e; len[i] = L.len_total; evals[i] = L.evals;
+            if (twist) twist[i] = L.tw;
+        }
+    }
+    return 0;
+}
+}

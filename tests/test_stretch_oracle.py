@@ -1,0 +1,307 @@
+"""Stretched-grid tracer (TraceBlineAdaptiveStretch.cu) in the oracle and in
+the CPU build of the kernels' header.
+
+What pins it:
+
+* Known answers of the two reference cell rules, derived by hand from
+  GetIndexBiSearch (:42-65) and GetIndexUniStretch (:68-75): on the 8-node
+  axis 0..7 the bisection ends with the bracket [4, 7] for a point in cell 6
+  and answers 5; on 16 nodes cell 10 is answered 9 and every other cell
+  correctly; the uniform formula divides the span by n, so on 0..3 the point
+  1.9 gets cell 2 with weight -0.1.
+* The interpolant in a remapped cell is that cell's trilinear polynomial
+  extrapolated (checked against NumPy from the eight corners).
+* A linear field is reproduced exactly by every cell rule, so its field lines
+  are known in closed form on any axes: x - xc = (x0 - xc) exp(k (z - z0) / B0)
+  and the arc length is a one-dimensional integral.
+* The landing plane: 0 / n_dim under the reference rules (:385-386), the box
+  faces under the exact rule -- a straight line's footpoint is where that
+  plane cuts it.
+* On the axes 0..n-1 with a linear field the stretched tracer is the uniform
+  RKF45 tracer (same tolerance rule, controller and landing).
+* The kernels' table lookup (bins, cell records, rule map) against the
+  oracle's lookup at every cell, and the kernels' stretched driver (host
+  build) against the oracle's.
+"""
+import functools
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from fastqsl_amd import synth
+
+
+def test_bisect_known_answers(oracle):
+    a = np.arange(8.0)
+    cell, w = oracle.stretch_cell(a, [0.5, 1.5, 2.5, 3.5, 4.5, 5.5, 6.5], "bisect")
+    np.testing.assert_array_equal(cell, [0, 1, 2, 3, 4, 5, 5])
+    np.testing.assert_allclose(w, [0.5] * 6 + [1.5])
+    a = np.arange(16.0)
+    p = np.arange(15) + 0.5
+    cell, w = oracle.stretch_cell(a, p, "bisect")
+    expect = np.arange(15)
+    expect[10] = 9
+    np.testing.assert_array_equal(cell, expect)
+    # 7 nodes: [0, 6] splits at 3 into [0, 3] (middle node 1) and [3, 6]
+    # (middle node 4), so cells 2 and 5 are answered 1 and 4
+    c7, _ = oracle.stretch_cell(np.arange(7.0), np.arange(6) + 0.5, "bisect")
+    np.testing.assert_array_equal(c7, [0, 1, 1, 3, 4, 4])
+    cell, w = oracle.stretch_cell(a, p, "bisect")
+    np.testing.assert_array_equal(cell, expect)
+    assert w[10] == pytest.approx(1.5)
+    # the clamp: outside and NaN go to a0 + 1e-6 / a_{n-1} - 1e-6
+    cell, w = oracle.stretch_cell(np.arange(8.0), [-3.0, np.nan, 9.0], "bisect")
+    np.testing.assert_array_equal(cell, [0, 0, 5])
+    np.testing.assert_allclose(w, [1e-6, 1e-6, 2.0 - 1e-6])
+
+
+def test_uniform_formula_known_answers(oracle):
+    cell, w = oracle.stretch_cell(np.arange(4.0), [1.9, 0.5, 2.9, 0.7], "uniform")
+    np.testing.assert_array_equal(cell, [2, 0, 2, 0])
+    np.testing.assert_allclose(w, [-0.1, 0.5, 0.9, 0.7])
+
+
+def test_exact_rule_is_the_holding_cell(oracle):
+    rng = np.random.default_rng(1)
+    for n in (2, 3, 5, 17, 100):
+        a = np.cumsum(rng.uniform(0.1, 2.0, n))
+        p = rng.uniform(a[0] + 1e-5, a[-1] - 1e-5, 500)
+        cell, w = oracle.stretch_cell(a, p, "exact")
+        ref = np.clip(np.searchsorted(a, p, side="left") - 1, 0, n - 2)
+        np.testing.assert_array_equal(cell, ref)
+        assert np.all((w >= 0) & (w <= 1))
+
+
+def test_interpolant_extrapolates_in_a_remapped_cell(oracle):
+    rng = np.random.default_rng(2)
+    shape = (16, 9, 7)
+    axes = [np.cumsum(rng.uniform(0.5, 1.5, n)) for n in shape]
+    f = tuple(rng.normal(size=shape) for _ in range(3))
+    # x = mid of cell 10 of the 16-node axis (answered 9); y in cell 3 of 9
+    # nodes and z in cell 1 of 7, which the bisection answers correctly (on 7
+    # nodes it remaps cells 2 and 5)
+    x = 0.5 * (axes[0][10] + axes[0][11])
+    y = 0.3 * axes[1][3] + 0.7 * axes[1][4]
+    z = 0.6 * axes[2][1] + 0.4 * axes[2][2]
+    got = oracle.interp_stretch(f, axes, [[x, y, z]], rule="bisect")[0]
+    i, j, k = 9, 3, 1
+    rx = (x - axes[0][i]) / (axes[0][i + 1] - axes[0][i])
+    ry = (y - axes[1][j]) / (axes[1][j + 1] - axes[1][j])
+    rz = (z - axes[2][k]) / (axes[2][k + 1] - axes[2][k])
+    assert rx > 1
+    want = []
+    for comp in f:
+        c = comp[i:i + 2, j:j + 2, k:k + 2]
+        wx, wy, wz = np.array([1 - rx, rx]), np.array([1 - ry, ry]), np.array([1 - rz, rz])
+        want.append(np.einsum("abc,a,b,c->", c, wx, wy, wz))
+    np.testing.assert_allclose(got, want, rtol=1e-13)
+    exact = oracle.interp_stretch(f, axes, [[x, y, z]], rule="exact")[0]
+    assert np.max(np.abs(exact - got)) > 1e-3  # the two rules differ here
+
+
+def _linear_field(axes, k, b0, centre):
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    return (k * (X - centre[0]), -k * (Y - centre[1]), np.full(X.shape, b0))
+
+
+def _axes_to_n(shape, kinds=("sinh", "sinh", "geometric")):
+    """Axes spanning [0, n_dim]: there the reference's landing plane is the face."""
+    return [synth.stretched_axis(n, 0.0, float(n), kind, 1.5, dtype=np.float64)
+            for n, kind in zip(shape, kinds)]
+
+
+@pytest.mark.parametrize("rule", ["bisect", "uniform", "exact"])
+def test_linear_field_closed_form(oracle, rule):
+    shape = (24, 22, 20)
+    axes = _axes_to_n(shape)
+    k, b0 = 0.05, 1.0
+    centre = (12.0, 11.0)
+    f = _linear_field(axes, k, b0, centre)
+    rng = np.random.default_rng(5)
+    z0 = 4.0
+    seeds = np.column_stack([centre[0] + rng.uniform(-2, 2, 40), centre[1] + rng.uniform(-2, 2, 40),
+                             np.full(40, z0)])
+    r = oracle.trace_stretch(f, axes, seeds, s0=0.05, tol_coef=1e-4, rule=rule)
+    zt = axes[2][-1]
+    assert np.all(r["flag_end"] == 6) and np.all(r["flag_start"] == 5)
+    g_up, g_dn = math.exp(k * (zt - z0) / b0), math.exp(-k * z0 / b0)
+    for (side, zz, g) in (("end", zt, g_up), ("start", 0.0, g_dn)):
+        p = r[side]
+        np.testing.assert_allclose(p[:, 2], zz, atol=1e-9)
+        np.testing.assert_allclose(p[:, 0] - centre[0], (seeds[:, 0] - centre[0]) * g, rtol=2e-6, atol=2e-6)
+        np.testing.assert_allclose(p[:, 1] - centre[1], (seeds[:, 1] - centre[1]) / g, rtol=2e-6, atol=2e-6)
+
+    def arc(x0, y0):
+        def ds(z):
+            e = math.exp(k * (z - z0) / b0)
+            bx, by = k * (x0 - centre[0]) * e, -k * (y0 - centre[1]) / e
+            return math.sqrt(bx * bx + by * by + b0 * b0) / b0
+        return quad(ds, 0.0, zt, epsabs=1e-12)[0]
+
+    want = np.array([arc(x, y) for x, y, _ in seeds])
+    np.testing.assert_allclose(r["length"], want, rtol=2e-4)
+
+
+@pytest.mark.parametrize("rule,plane_lo,plane_hi", [("bisect", 0.0, 40.0), ("uniform", 0.0, 40.0),
+                                                    ("exact", -5.0, 5.0)])
+def test_landing_plane(oracle, rule, plane_lo, plane_hi):
+    """A uniform field along -x leaving through the x faces of a box [-5, 5]
+    x ...: the footpoint lies on the straight line through the seed, on the
+    plane x = 0 / n_x (the reference's p_mid, :385-386) or on the face
+    (exact).  (The step grows to the 100-unit clamp in a uniform field, so a
+    tilted field would leave through an edge instead: class 42.)"""
+    shape = (40, 12, 10)
+    axes = [synth.stretched_axis(shape[0], -5.0, 5.0, "sinh", 1.0, dtype=np.float64),
+            np.linspace(0.0, 3.0, shape[1]), np.linspace(0.0, 2.0, shape[2])]
+    B = np.array([-1.0, 0.0, 0.0])
+    f = tuple(np.full(shape, c) for c in B)
+    seeds = np.array([[1.3, 1.5, 0.5], [-2.2, 0.7, 1.1], [3.9, 2.2, 0.9]])
+    r = oracle.trace_stretch(f, axes, seeds, ncross=(1.0, 0.0, 0.0), s0=0.1, rule=rule)
+    np.testing.assert_array_equal(r["flag_end"], 1)
+    np.testing.assert_array_equal(r["flag_start"], 2)
+    for side, plane in (("end", plane_lo), ("start", plane_hi)):
+        p = r[side]
+        np.testing.assert_allclose(p[:, 0], plane, atol=1e-9)
+        np.testing.assert_allclose(p[:, 1], seeds[:, 1], atol=1e-9)
+        np.testing.assert_allclose(p[:, 2], seeds[:, 2], atol=1e-9)
+
+
+def test_index_axes_reproduce_the_uniform_tracer(oracle):
+    shape = (30, 28, 26)
+    f = synth.hyperbolic(shape, k=0.03, center=(14.5, 13.5))
+    axes = [np.arange(n, dtype=np.float64) for n in shape]
+    rng = np.random.default_rng(9)
+    seeds = np.column_stack([rng.uniform(6, 22, 60), rng.uniform(6, 21, 60), rng.uniform(1, 20, 60)])
+    u = oracle.trace_plain(f, seeds, method="rkf45", s0=0.3, tol_coef=1.0)
+    # class-42 footpoints are the outside point of a step whose stages left
+    # the box, where the two clamp rules (face vs face -+ 1e-6) differ
+    face = ((u["flag_start"] >= 1) & (u["flag_start"] <= 6) & (u["flag_end"] >= 1) & (u["flag_end"] <= 6))
+    assert face.sum() > 30
+    for rule in ("bisect", "uniform"):
+        s = oracle.trace_stretch(f, axes, seeds, s0=0.3, tol_coef=1.0, rule=rule)
+        for key in ("flag_start", "flag_end", "evals"):
+            np.testing.assert_array_equal(s[key], u[key])
+        for key in ("start", "end", "length"):
+            np.testing.assert_allclose(s[key][face], u[key][face], rtol=1e-8, atol=1e-7)
+
+
+def test_twist_in_a_unit_force_free_field(oracle):
+    """B = (sin az, cos az, 0) has |B| = 1 and curl B = a B: each step adds
+    a dl / 4pi and the landing term J.b/|b|^2 adds a |B_interp| x arc / 4pi,
+    so the twist is a L / 4pi up to the interpolant's |B| deficit."""
+    shape = (20, 20, 24)
+    axes = [synth.stretched_axis(n, 0.0, 10.0, kind, 1.0, dtype=np.float64)
+            for n, kind in zip(shape, ("sinh", "tanh", "geometric"))]
+    a = 0.1  # |B_interp| >= 1 - (a dz)^2 / 8 > 0.9995 on this grid
+    Z = np.broadcast_to(axes[2][None, None, :], shape)
+    f = (np.sin(a * Z), np.cos(a * Z), np.zeros(shape))
+    cur = tuple(a * c for c in f)
+    rng = np.random.default_rng(4)
+    seeds = np.column_stack([rng.uniform(2, 8, 30), rng.uniform(2, 8, 30), rng.uniform(1, 9, 30)])
+    r = oracle.trace_stretch(f, axes, seeds, ncross=(0.0, 1.0, 0.0), s0=0.05, tol_coef=1e-2, rule="exact",
+                             current=cur)
+    # the lines are straight, so the step grows to its 100-unit clamp and some
+    # lines leave through an edge (class 42, no landing term)
+    face = (r["flag_start"] >= 1) & (r["flag_start"] <= 4)
+    assert face.sum() >= 5
+    np.testing.assert_allclose(r["twist"], a * r["length"] / (4 * math.pi), rtol=5e-4)
+    r0 = oracle.trace_stretch(f, axes, seeds, ncross=(0.0, 1.0, 0.0), s0=0.05, tol_coef=1e-2, rule="exact")
+    np.testing.assert_array_equal(r0["end"], r["end"])
+    assert np.all(r0["twist"] == 0)
+
+
+# ------------------------------------------------------- host build of the kernels
+@pytest.mark.parametrize("rule", ["bisect", "uniform", "exact"])
+def test_table_lookup_matches_oracle(oracle, hostcheck, rule):
+    rng = np.random.default_rng(11)
+    for n in (2, 3, 4, 5, 8, 16, 33, 258, 342):
+        for kind in ("sinh", "tanh", "geometric", "random"):
+            if kind == "random":
+                a = np.cumsum(rng.uniform(0.05, 3.0, n)) - 7.0
+            else:
+                a = synth.stretched_axis(n, -3.0, 40.0, kind, 2.0, dtype=np.float64)
+            mids = 0.5 * (a[:-1] + a[1:])
+            p = np.r_[mids, rng.uniform(a[0] - 1, a[-1] + 1, 400), a, np.nan]
+            c_o, w_o = oracle.stretch_cell(a, p, rule)
+            c_h, w_h = hostcheck.stretch_cell(a, p, rule, precision=1)
+            np.testing.assert_array_equal(c_h, c_o, err_msg=f"n={n} {kind}")
+            np.testing.assert_allclose(w_h, w_o, rtol=1e-12, atol=1e-12)
+            # float tables: the same cells away from nodes, weights to float accuracy
+            a32 = a.astype(np.float32).astype(np.float64)
+            if np.any(np.diff(a32) <= 0):
+                continue
+            mids32 = 0.5 * (a32[:-1] + a32[1:])
+            if rule == "uniform":  # its breakpoints lie at a0 + k span / n, not at nodes
+                u = (mids32 - a32[0]) * n / (a32[-1] - a32[0])
+                mids32 = mids32[np.abs(u - np.round(u)) > 1e-4]
+            c_o, w_o = oracle.stretch_cell(a32, mids32, rule)
+            c_f, w_f = hostcheck.stretch_cell(a32, mids32, rule, precision=0)
+            np.testing.assert_array_equal(c_f, c_o)
+            np.testing.assert_allclose(w_f, w_o, rtol=1e-6, atol=3e-6)
+
+
+def _agree(o, x, tol):
+    same = (o["flag_start"] == x["flag_start"]) & (o["flag_end"] == x["flag_end"])
+    d = np.maximum(np.max(np.abs(o["end"] - x["end"]), axis=1), np.max(np.abs(o["start"] - x["start"]), axis=1))
+    return same.mean(), np.mean(same & (d < tol))
+
+
+@functools.lru_cache(maxsize=None)
+def _case(rule):
+    """The random multipole on a stretched grid; for the uniform rule on an
+    evenly spaced one, the grids the reference selects it for (flag_uni is
+    set when the spacing varies by under 1 %, demo/Q_RKF45CuPyStretch.ipynb
+    cell [4]) -- on a stretched grid its cells jump so often that half of
+    the lines stall at the attempt cap."""
+    kinds = ("uniform",) * 3 if rule == "uniform" else ("sinh", "sinh", "geometric")
+    f, axes = synth.stretched_multipole((34, 30, 28), seed=6, dtype=np.float64, device="cpu", kinds=kinds)
+    f = tuple(np.asarray(b, np.float64) for b in f)
+    axes = [np.asarray(a, np.float64) for a in axes]
+    rng = np.random.default_rng(3)
+    seeds = np.column_stack([rng.uniform(-0.8, 0.8, 300), rng.uniform(-0.8, 0.8, 300),
+                             rng.uniform(0.01, 0.5, 300)]).astype(np.float32)
+    return f, axes, seeds
+
+
+@pytest.mark.parametrize("rule", ["bisect", "uniform", "exact"])
+def test_host_build_fp64_matches_oracle(oracle, hostcheck, rule):
+    """FP64 through the kernels' tables and driver.  The first step from
+    s0 = 0.01 has an error estimate at the FP64 rounding level (~1e-19), so
+    the next step size -- and with it the step sequence -- depends on the
+    order of the arithmetic; footpoints then agree to the tolerance level."""
+    f, axes, seeds = _case(rule)
+    o = oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1e-2, rule=rule)
+    h = hostcheck.trace_stretch(f, axes, seeds, rule=rule, precision=1, s0=0.01, tol_coef=1e-2)
+    face = ((o["flag_start"] >= 1) & (o["flag_start"] <= 6) & (o["flag_end"] >= 1) & (o["flag_end"] <= 6))
+    assert face.sum() > 100
+    cls, _ = _agree(o, h, 0.0)
+    d = np.maximum(np.max(np.abs(o["end"] - h["end"]), axis=1), np.max(np.abs(o["start"] - h["start"]), axis=1))
+    assert cls > 0.99 and np.mean(d[face] < 1e-4) > 0.98, (cls, np.mean(d[face] < 1e-4))
+
+
+def test_host_build_fp32c_exact_rule(oracle, hostcheck):
+    """FP32c with a continuous interpolant and a tolerance well above the
+    FP32 noise of the error estimate: the same footpoints to 1e-4."""
+    f, axes, seeds = _case("exact")
+    o = oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1e-4, rule="exact")
+    h = hostcheck.trace_stretch(f, axes, seeds, rule="exact", precision=0, s0=0.01, tol_coef=1e-4)
+    cls, close = _agree(o, h, 1e-4)
+    assert cls > 0.99 and close > 0.97, (cls, close)
+
+
+@pytest.mark.parametrize("rule", ["bisect", "uniform"])
+def test_host_build_fp32c_reference_rules(oracle, hostcheck, rule):
+    """The reference rules' interpolant jumps at some nodes, and a step
+    across a jump moves its endpoint ~10x the change in step size; the FP32
+    stages put ~1 % noise on the RKF45 error estimate at this tolerance, so
+    the bar is the oracle's own spread when its tolerance moves by 1 %."""
+    f, axes, seeds = _case(rule)
+    o = oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1e-3, rule=rule)
+    h = hostcheck.trace_stretch(f, axes, seeds, rule=rule, precision=0, s0=0.01, tol_coef=1e-3)
+    spread = [_agree(o, oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1e-3 * (1 + e), rule=rule), 1e-2)
+              for e in (1e-2, -1e-2)]
+    cls, close = _agree(o, h, 1e-2)
+    assert cls >= min(c for c, _ in spread) - 0.03, (cls, spread)
+    assert close >= min(c for _, c in spread) - 0.15, (close, spread)
