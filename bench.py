@@ -41,11 +41,12 @@ def parse():
                     help="serve: Poisson arrival rates (requests/s) to sweep")
     ap.add_argument("--duration", type=float, default=2.0,
                     help="serve: seconds of arrivals per rate")
-    ap.add_argument("--workload", default="plane", choices=["plane", "volume", "batch", "serve"],
+    ap.add_argument("--workload", default="plane", choices=["plane", "volume", "batch", "serve", "stretch"],
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
                          "of the 512x256x256 rope-arcade field; batch: 256 uneven plane requests "
                          "over 3 resident fields in one graph-captured run; serve: Poisson "
-                         "arrivals of 128^2..2048^2 maps over 16 resident snapshots")
+                         "arrivals of 128^2..2048^2 maps over 16 resident snapshots; stretch: "
+                         "1024^2 plain RKF45 lines on a stretched 256^3 grid")
     ap.add_argument("--pixels", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch knobs instead of autotuning them")
@@ -359,6 +360,118 @@ def bench_serve(args, fq, torch, log):
         f.close()
 
 
+def bench_stretch(args, fq, torch, log):
+    """Plain RKF45 tracing on a stretched grid (TraceBlineAdaptiveStretch.cu):
+    the random multipole (default_rng(0)) scaled by 10 and sampled on a 256^3
+    float32 grid with x and y clustered at the centre (sinh, strength 1.5;
+    cells 0.055-0.13) and z spacing growing with height (geometric, strength
+    3; cells 0.013-0.24), box [-10,10]^2 x [0,20]; 1024^2 seeds on z = 1e-3
+    over [-9.5, 9.5]^2, s_len 0.01, tol_coef 1e-2, the reference's bisection
+    cell rule (flag_uni false).  FP32c persistent against FP64 one
+    thread per line on the same GPU, and the FP64 oracle on the host."""
+    dev = torch.device("cuda", 0)
+    steps = args.steps or 5
+    warm = args.warmup if args.warmup is not None else 1
+    t0 = time.perf_counter()
+    (bx, by, bz), axes = fq.synth.stretched_multipole((256, 256, 256), seed=0, device="cuda", scale=10.0)
+    field = fq.Field(bx, by, bz, layouts=("soa64", "cell32"), axes=axes, lookup="bisect")
+    n = args.pixels
+    g = np.linspace(-9.5, 9.5, n, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.full(X.size, 1e-3, np.float32)])
+    sx, sy, sz = (torch.as_tensor(np.ascontiguousarray(seeds[:, i]), device=dev) for i in range(3))
+    nlines = seeds.shape[0]
+    log(f"stretched field resident ({field.nbytes / 1e9:.2f} GB) in {time.perf_counter() - t0:.1f} s; "
+        f"{nlines} lines per step")
+    kw = dict(method="rkf45", step=0.01, tol_coef=1e-2)
+    outs = {}
+
+    def run(precision, schedule):
+        key = (precision, schedule)
+        outs[key] = fq.trace_plain(field, sx, sy, sz, precision=precision, schedule=schedule,
+                                   out=outs.get(key), **kw)
+        return outs[key]
+
+    for _ in range(warm):
+        run(args.precision, args.schedule)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        run(args.precision, args.schedule)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    kms = []
+    for _ in range(steps):
+        run(args.precision, args.schedule)
+        kms.append(fq.last_kernel_ms())
+    out = outs[(args.precision, args.schedule)]
+    evals = int(out["evals"].to(torch.int64).sum().item())
+    value = nlines * steps / wall
+    k = float(np.mean(kms))
+    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    achieved = evals * bpe / (k * 1e-3)
+    log(f"stretch {args.precision}/{args.schedule}: {value:.4g} lines/s, kernel {k:.2f} ms, "
+        f"{evals / nlines:.1f} evals/line")
+    extra = {}
+    if not args.no_fp64_compare:
+        run("fp64", "per_line")
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        run("fp64", "per_line")
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t2
+        extra["fp64_per_line"] = {"lines_per_s": nlines / dt, "ms_per_step": dt * 1e3}
+        extra["speedup_vs_fp64_per_line"] = value / (nlines / dt)
+        log(f"stretch fp64/per_line: {nlines / dt:.4g} lines/s ({dt * 1e3:.1f} ms)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        rng = np.random.default_rng(1)
+        idx = np.sort(rng.choice(nlines, size=min(args.cpu_sample >> 2, nlines), replace=False))
+        f64 = tuple(np.asarray(b, np.float64) for b in (bx, by, bz))
+        ax64 = [np.asarray(a, np.float64) for a in axes]
+        nt = O.num_threads()
+        t3 = time.perf_counter()
+        ref = O.trace_stretch(f64, ax64, seeds[idx], s0=0.01, tol_coef=1e-2, rule="bisect", nthreads=nt)
+        dt = time.perf_counter() - t3
+        cpu = {"value": len(idx) / dt, "unit": "lines/s", "cores": nt, "kind": "port",
+               "sample": f"{len(idx)} seeds drawn uniformly (default_rng(1)) from the same {nlines} "
+                         "seeds, FP64 C oracle (stretched tracer), OpenMP dynamic schedule",
+               "cpu_model": cpu_model(), "oracle_evals_per_line": float(ref["evals"].mean())}
+        g_ = {key: out[key][idx].cpu().numpy() for key in ("flag_start", "flag_end", "end_x", "end_y",
+                                                           "end_z")}
+        cls = (g_["flag_start"] == ref["flag_start"]) & (g_["flag_end"] == ref["flag_end"])
+        end = np.column_stack([g_["end_x"], g_["end_y"], g_["end_z"]]).astype(np.float64)
+        e = np.abs(end - ref["end"]).max(axis=1)[cls]
+        extra["error_vs_oracle"] = {"lines": int(len(idx)), "class_match": float(cls.mean()),
+                                    "endpoint_err_median": float(np.median(e)),
+                                    "frac_endpoint_err_le_1e-4": float(np.mean(e <= 1e-4))}
+        log(f"cpu baseline: {len(idx)} lines in {dt:.2f} s on {nt} threads")
+    flags = torch.stack([out["flag_start"], out["flag_end"]]).cpu().numpy()
+    classes, counts = np.unique(flags, return_counts=True)
+    rec = {
+        "metric": "field lines/s (plain RKF45, stretched grid)",
+        "value": value, "unit": "lines/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32c" else "f64", "data": "synthetic",
+        "config": {"workload": "random multipole (default_rng(0), scaled x10) on a stretched 256^3 "
+                               "float32 grid over [-10,10]^2x[0,20] (sinh x/y, geometric z), plain "
+                               "RKF45 tol_coef 1e-2 s_len 0.01, bisection cell rule, z=1e-3 plane "
+                               "%dx%d" % (n, n),
+                   "lines_per_step": nlines, "precision": args.precision, "schedule": args.schedule,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_plain_persistent (stretched)", "kernel_ms": k,
+                     "evals_per_launch": evals, "bytes_per_eval": bpe},
+        "cpu_baseline": cpu,
+        "classes": {str(int(c)): int(v) for c, v in zip(classes, counts)},
+        **extra,
+    }
+    print(json.dumps(rec), flush=True)
+    field.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -376,6 +489,8 @@ def main():
 
     if args.workload == "serve":
         return bench_serve(args, fq, torch, log)
+    if args.workload == "stretch":
+        return bench_stretch(args, fq, torch, log)
     if args.workload == "batch":
         if args.steps is None:
             args.steps = 3

@@ -177,14 +177,16 @@ def stretched_axis(n, lo, hi, kind="tanh", strength=2.0, dtype=np.float32):
 
 def stretched_multipole(shape=(256, 256, 256), seed=0, count=40, dtype=np.float32, device=None,
                         box=((-1.0, 1.0), (-1.0, 1.0), (0.0, 2.0)), kinds=("sinh", "sinh", "geometric"),
-                        strength=(1.5, 1.5, 3.0)):
+                        strength=(1.5, 1.5, 3.0), scale=1.0):
     """The random multipole of ``multipole`` sampled on a stretched grid: x
-    and y clustered at the centre, z spacing growing with height.  Returns
-    ((bx, by, bz), (xa, ya, za)) with float32 axes in the box's units."""
-    axes = tuple(stretched_axis(n, lo, hi, kind, st)
+    and y clustered at the centre, z spacing growing with height.  ``scale``
+    multiplies the box and the source positions (the field lines scale with
+    them).  Returns ((bx, by, bz), (xa, ya, za)) with float32 axes in the
+    box's units."""
+    axes = tuple(stretched_axis(n, lo * scale, hi * scale, kind, st)
                  for n, (lo, hi), kind, st in zip(shape, box, kinds, strength))
     q, pos = multipole_sources(seed, count)
-    field = point_sources(shape, q, pos, dtype=dtype, device=device, axes=axes)
+    field = point_sources(shape, q, pos * scale, dtype=dtype, device=device, axes=axes)
     return field, axes
 
 
