@@ -82,6 +82,10 @@ PROTOTYPES = {
     "fqsl_pseudo_seeds": (i32, [u64, vp, vp, vp, vp, vp, vp, vp, ctypes.c_float, u64,
                                 vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64), vp]),
     "fqsl_qcut_scatter": (i32, [u64] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),
+    "fqsl_slog10q": (i32, [u64, vp, vp, vp, vp, vp]),
+    "fqsl_bad_pixels": (i32, [u64, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, ctypes.POINTER(u64), vp]),
+    "fqsl_scatter_f32": (i32, [u64, vp, vp, vp, vp]),
+    "fqsl_curl_index": (i32, [vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "fqsl_TraceAllBlineScott": (i32, [vp] * 30 + [i32, i32, vp]),
     "fqsl_TraceAllBline": (i32, [i32] + [vp] * 35 + [i32, i32, vp]),
     "fqsl_TraceAllBlineStretch": (i32, [vp] * 39 + [i32, i32, vp]),

@@ -1315,6 +1315,81 @@ extern "C" int fqsl_copy_boundary(double* a, int nx, int ny, int nz, void* strea
     return FQSL_OK;
 }
 
+// curl B in grid-index units as demo/Q_RK4.ipynb cell [11] (and
+// demo/npzinputRKF45CuPyTwist.ipynb cell [9]) computes it for the twist: on a
+// zero array, central differences (c[+1] - c[-1]) / 2 on the index ranges the
+// notebook slices (curl_x on [:, 1:-1, 1:-1], curl_y on [1:-1, :, 1:-1],
+// curl_z on [1:-1, 1:-1, :]), then at z = 0 on [1:-1, 1:-1] the one-sided
+// derivative (-3 c[0] + 4 c[1] - c[2]) / 2 along z for curl_x and curl_y.
+// In B's type and in the notebook's operation order, without contraction,
+// so the result is the same bits as its array arithmetic.
+#pragma clang fp contract(off)
+template <typename T>
+__global__ void __launch_bounds__(256) k_curl_index(const T* __restrict__ bx, const T* __restrict__ by,
+                                                    const T* __restrict__ bz, int nx, int ny, int nz,
+                                                    T* __restrict__ cx, T* __restrict__ cy, T* __restrict__ cz) {
+    const long nxy = (long)nx * ny, nn = nxy * nz;
+    const T two = (T)2, three = (T)3, four = (T)4;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nn; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / nxy);
+        const bool ii = i >= 1 && i <= nx - 2, jj = j >= 1 && j <= ny - 2, kk = k >= 1 && k <= nz - 2;
+        T vx = (T)0, vy = (T)0, vz = (T)0;
+        if (jj && kk) vx = (bz[t + nx] - bz[t - nx]) / two - (by[t + nxy] - by[t - nxy]) / two;
+        if (ii && kk) vy = (bx[t + nxy] - bx[t - nxy]) / two - (bz[t + 1] - bz[t - 1]) / two;
+        if (ii && jj) vz = (by[t + 1] - by[t - 1]) / two - (bx[t + nx] - bx[t - nx]) / two;
+        if (k == 0 && ii && jj) {
+            vx = (bz[t + nx] - bz[t - nx]) / two - ((-three * by[t] + four * by[t + nxy]) - by[t + 2 * nxy]) / two;
+            vy = ((-three * bx[t] + four * bx[t + nxy]) - bx[t + 2 * nxy]) / two - (bz[t + 1] - bz[t - 1]) / two;
+        }
+        cx[t] = vx;
+        cy[t] = vy;
+        cz[t] = vz;
+    }
+}
+
+// slog10 Q = log10(Q) * sign(Bn) [* (flag <= 6)] (demo/Q_RKF45CuPy.ipynb
+// cells [13]-[14], demo/Q_RKF45MixCuPyDiffScott.ipynb cell [8]), with
+// NumPy's sign (0 at 0, NaN at NaN) and its left-to-right products.
+__global__ void __launch_bounds__(256) k_slog10q(unsigned long long n, const float* __restrict__ q,
+                                                 const float* __restrict__ bn, const int32_t* __restrict__ flag,
+                                                 float* __restrict__ out) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const float b = bn[i];
+        const float sg = b > 0.f ? 1.f : (b < 0.f ? -1.f : (b == 0.f ? 0.f : b));
+        float r = log10f(q[i]) * sg;
+        if (flag) r = r * (flag[i] <= 6 ? 1.f : 0.f);
+        out[i] = r;
+    }
+}
+#pragma clang fp contract(on)
+
+extern "C" int fqsl_curl_index(const void* bx, const void* by, const void* bz, int nx, int ny, int nz, int dtype,
+                               void* cx, void* cy, void* cz, void* stream) {
+    int rc = check_grid3(nx, ny, nz, dtype);
+    if (rc) return rc;
+    if (!bx || !by || !bz || !cx || !cy || !cz) return set_err(FQSL_ERR_INVALID, "NULL array");
+    hipStream_t st = (hipStream_t)stream;
+    const int g = grid_for((long)nx * ny * nz, 256);
+    if (dtype == FQSL_DTYPE_F32)
+        k_curl_index<float><<<g, 256, 0, st>>>((const float*)bx, (const float*)by, (const float*)bz, nx, ny, nz,
+                                               (float*)cx, (float*)cy, (float*)cz);
+    else
+        k_curl_index<double><<<g, 256, 0, st>>>((const double*)bx, (const double*)by, (const double*)bz, nx, ny, nz,
+                                                (double*)cx, (double*)cy, (double*)cz);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_slog10q(uint64_t n, const float* q, const float* bn, const int32_t* flag, float* out,
+                            void* stream) {
+    if (n == 0) return FQSL_OK;
+    if (!q || !bn || !out) return set_err(FQSL_ERR_INVALID, "NULL array");
+    k_slog10q<<<grid_for((long)((n + 255) / 256 * 256), 256), 256, 0, (hipStream_t)stream>>>(n, q, bn, flag, out);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+
 // ----------------------------------------------------------------- batches
 // A batch of Scott requests, each with its own field, method, step,
 // tolerance, seeds and outputs.  Requests are grouped by (field, method); the
@@ -2215,3 +2290,100 @@ extern "C" int fqsl_qcut_scatter(uint64_t m, const float* start_x, const float* 
     HIP_TRY(hipGetLastError());
     return FQSL_OK;
 }
+
+// Pixels of a neighbour-difference map that the Scott method re-traces
+// (demo/Q_RKF45MixCuPyDiffScott.ipynb cell [9]): B_flag + isnan(Q) > 0.5.
+__device__ __forceinline__ bool bad_pixel(const int32_t* __restrict__ bflag, const float* __restrict__ q,
+                                          unsigned long long i) {
+    const float v = q[i];
+    return bflag[i] + (v != v ? 1 : 0) > 0;
+}
+
+__global__ void k_bad_count(unsigned long long n, const int32_t* __restrict__ bflag, const float* __restrict__ q,
+                            unsigned* __restrict__ counts) {
+    __shared__ unsigned s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const unsigned long long i = blockIdx.x * (unsigned long long)kCompactBlock + threadIdx.x;
+    const bool sel = i < n && bad_pixel(bflag, q, i);
+    const unsigned long long m = __ballot(sel);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&s, (unsigned)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = s;
+}
+
+__global__ void k_bad_write(unsigned long long n, const int32_t* __restrict__ bflag, const float* __restrict__ q,
+                            const float* __restrict__ xs, const float* __restrict__ ys, const float* __restrict__ zs,
+                            const unsigned* __restrict__ offs, unsigned long long capacity, float* __restrict__ ox,
+                            float* __restrict__ oy, float* __restrict__ oz, uint64_t* __restrict__ pix) {
+    __shared__ unsigned wave_base[kCompactBlock / 64];
+    const unsigned long long i = blockIdx.x * (unsigned long long)kCompactBlock + threadIdx.x;
+    const bool sel = i < n && bad_pixel(bflag, q, i);
+    const unsigned long long m = __ballot(sel);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wave_base[w] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned run = 0;
+        for (int k = 0; k < kCompactBlock / 64; ++k) { const unsigned v = wave_base[k]; wave_base[k] = run; run += v; }
+    }
+    __syncthreads();
+    if (!sel) return;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long j = (unsigned long long)offs[blockIdx.x] + wave_base[w] + __popcll(m & below);
+    if (j >= capacity) return;
+    ox[j] = xs[i];
+    oy[j] = ys[i];
+    oz[j] = zs[i];
+    pix[j] = i;
+}
+
+__global__ void k_scatter_f32(unsigned long long m, const uint64_t* __restrict__ pix, const float* __restrict__ v,
+                              float* __restrict__ target) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < m;
+         i += (unsigned long long)gridDim.x * blockDim.x)
+        target[pix[i]] = v[i];
+}
+
+extern "C" int fqsl_bad_pixels(uint64_t n, const int32_t* b_flag, const float* q, const float* x, const float* y,
+                               const float* z, uint64_t capacity, float* seed_x, float* seed_y, float* seed_z,
+                               uint64_t* pixel, uint64_t* count_host, void* stream) {
+    if (!count_host) return set_err(FQSL_ERR_INVALID, "count_host is NULL");
+    *count_host = 0;
+    if (n == 0) return FQSL_OK;
+    if (!b_flag || !q || !x || !y || !z) return set_err(FQSL_ERR_INVALID, "NULL input array");
+    if (capacity && (!seed_x || !seed_y || !seed_z || !pixel)) return set_err(FQSL_ERR_INVALID, "NULL output array");
+    if (n > 0xffffffffull) return set_err(FQSL_ERR_INVALID, "bad-pixel compaction supports < 2^32 pixels");
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned nblocks = (unsigned)((n + kCompactBlock - 1) / kCompactBlock);
+    unsigned* counts = nullptr;
+    unsigned long long* total = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&counts, nblocks * sizeof(unsigned) + 16, st));
+    HIP_TRY(hipMallocAsync((void**)&total, sizeof(unsigned long long), st));
+    k_bad_count<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, q, counts);
+    k_pseudo_scan<<<1, 1024, 0, st>>>(nblocks, counts, total);
+    k_bad_write<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, q, x, y, z, counts, capacity, seed_x, seed_y, seed_z,
+                                                   pixel);
+    hipError_t e = hipGetLastError();
+    unsigned long long cnt = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, total, sizeof(cnt), hipMemcpyDeviceToHost, st);
+    (void)hipFreeAsync(counts, st);
+    (void)hipFreeAsync(total, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "bad-pixel compaction: %s", hipGetErrorString(e));
+    *count_host = cnt;
+    if (capacity == 0) return FQSL_OK;
+    if (cnt > capacity)
+        return set_err(FQSL_ERR_INVALID, "%llu bad pixels exceed capacity %llu (outputs hold the first %llu)",
+                       (unsigned long long)cnt, (unsigned long long)capacity, (unsigned long long)capacity);
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_scatter_f32(uint64_t m, const uint64_t* pixel, const float* values, float* target, void* stream) {
+    if (m == 0) return FQSL_OK;
+    if (!pixel || !values || !target) return set_err(FQSL_ERR_INVALID, "NULL array");
+    k_scatter_f32<<<grid_for((long)m, 256), 256, 0, (hipStream_t)stream>>>(m, pixel, values, target);
+    HIP_TRY(hipGetLastError());
+    return FQSL_OK;
+}
+

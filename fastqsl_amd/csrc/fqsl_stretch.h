@@ -14,7 +14,9 @@
 //                   of the cell that holds the point.
 //   kLookupUniform  GetIndexUniStretch (:68-75): floor((p - a0) / (span / n)),
 //                   span / n rather than span / (n - 1), clamped to [0, n-2];
-//                   r may fall outside [0, 1].
+//                   r may fall outside [0, 1].  Evaluated as the reference
+//                   does (a division by the rounded width) in the path's
+//                   precision.
 //   kLookupExact    the cell that holds the point (not in the reference; with
 //                   it the landing plane is the box face, see below).
 // The interpolant is the cell's trilinear polynomial evaluated at r, so a cell
@@ -54,7 +56,7 @@ struct StretchAxis {
     T lo, hi;                           // a[0], a[n-1]
     T clo, chi;                         // a[0] + 1e-6, a[n-1] - 1e-6
     T bin_scale;                        // nbins / (hi - lo)
-    T uni_scale;                        // n / (hi - lo)
+    T uni_width;                        // (hi - lo) / n in T, GetIndexUniStretch's cell width
     int n, nbins;
 };
 
@@ -75,8 +77,8 @@ FQSL_HD void stretch_locate(const StretchAxis<T>& A, int rule, T phi, T plo, int
     T p = phi + plo;
     if (!(p > A.lo)) { p = A.clo; phi = A.clo; plo = (T)0; }
     if (!(p < A.hi)) { p = A.chi; phi = A.chi; plo = (T)0; }
-    if (rule == kLookupUniform) {
-        int k = to_int(flr((p - A.lo) * A.uni_scale));
+    if (rule == kLookupUniform) {  // the reference's division, so ties at a breakpoint fall the same way
+        int k = to_int(flr((p - A.lo) / A.uni_width));
         k = k < 0 ? 0 : (k > A.n - 2 ? A.n - 2 : k);
         const auto r = A.rec[k];  // built with m = c for this rule
         c = k;
@@ -413,7 +415,7 @@ inline void fill_grid(const AxisTables (&t)[3], int rule, const void* const (&re
         A.clo = (T)(lo + 1e-6);
         A.chi = (T)(hi - 1e-6);
         A.bin_scale = (T)(t[q].nbins / (hi - lo));
-        A.uni_scale = (T)(t[q].n / (hi - lo));
+        A.uni_width = ((T)hi - (T)lo) / (T)t[q].n;
         A.n = t[q].n;
         A.nbins = t[q].nbins;
         g.lo[q] = lo;

@@ -5,6 +5,9 @@
 * ``neighbour_qmap`` -- position-only traces, QCalcPlane, pseudo lines for
   pixels where B is nearly tangent to the plane, Qcut and merge
   (demo/Q_RKF45CuPy.ipynb cells [6]-[11]).
+* ``mixed_qmap``     -- ``neighbour_qmap`` with the pixels that have B_flag set
+  or Q = NaN traced again by the Scott method
+  (demo/Q_RKF45MixCuPyDiffScott.ipynb cells [7]-[9]).
 
 Seed grids use the notebooks' conventions (``linspace`` x ``meshgrid``, rows
 = the second in-plane axis), see ``synth.plane_seeds_z`` / ``plane_seeds_abc``.
@@ -14,7 +17,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import FastQSL
+from . import FastQSL, postproc
 from .field import Field
 from .trace import trace_plain, trace_scott
 
@@ -106,3 +109,24 @@ def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), m
                      rc["end_x"], rc["end_y"], rc["end_z"], rc["flag_end"],
                      c["bz0_start"], c["bz0_end"], ps, Q_all=q_all, pixel=c["pixel"])
     return dict(q=q_all[1:-1, 1:-1], q_all=q_all, q_inner=q_inner, traces=g, n_pseudo=m)
+
+
+def mixed_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), method="rkf45", step=0.125,
+               tol_coef=1.0, scott_tol=None, scott_use="q_perp", precision="fp32c", schedule="persistent",
+               **neighbour_kw):
+    """The mixed method: ``neighbour_qmap``, then every pixel with B_flag set
+    or Q = NaN takes the Scott method's q_perp (the notebook's choice;
+    ``scott_use="q_0"`` for the other one) from a Scott trace of its seed
+    with the same step and a per-step tolerance ``scott_tol`` (default
+    1e-3 * tol_coef, the reference kernel's mapping of the coefficient).
+    Returns the ``neighbour_qmap`` dict with ``q_all`` / ``q`` updated and
+    ``n_scott`` = the number of pixels replaced."""
+    res = neighbour_qmap(field, seeds, shape, stride, normal=normal, method=method, step=step,
+                         tol_coef=tol_coef, precision=precision, schedule=schedule, **neighbour_kw)
+    x, y, z = _grid(seeds, shape)
+    tol = 1e-3 * tol_coef if scott_tol is None else scott_tol
+    res["n_scott"] = postproc.scott_fixup(field, res["q_all"], res["traces"]["b_flag"], x, y, z, method=method,
+                                          step=step, tol=tol, precision=precision, schedule=schedule,
+                                          use=scott_use)
+    res["q"] = res["q_all"][1:-1, 1:-1]
+    return res

@@ -256,6 +256,33 @@ int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, co
                       const float *bz0_end, float stride, const uint64_t *pixel, float *q_all,
                       void *stream);
 
+/* ------------------------------------------------- around the Q map */
+/* slog10 Q = log10(q) * sign(bn) [* (flag <= 6) when flag is not NULL]
+ * (demo/Q_RKF45CuPy.ipynb cells [13]-[14], demo/Q_RKF45MixCuPyDiffScott.ipynb
+ * cell [8]); n device floats, NumPy's sign and operation order. */
+int fqsl_slog10q(uint64_t n, const float *q, const float *bn, const int32_t *flag, float *out, void *stream);
+
+/* The pixels the Scott method re-traces in the mixed method
+ * (demo/Q_RKF45MixCuPyDiffScott.ipynb cell [9]): b_flag + isnan(q) > 0.5,
+ * in pixel order.  Writes up to capacity seeds (x, y, z of those pixels) and
+ * pixel indices; count_host receives the number of such pixels (capacity 0:
+ * count only).  Synchronises the stream. */
+int fqsl_bad_pixels(uint64_t n, const int32_t *b_flag, const float *q, const float *x, const float *y,
+                    const float *z, uint64_t capacity, float *seed_x, float *seed_y, float *seed_z,
+                    uint64_t *pixel, uint64_t *count_host, void *stream);
+
+/* target[pixel[i]] = values[i] for i < m (the fix-up's
+ * Qall_1d[idx_bad] = q_perp_bad, same cell). */
+int fqsl_scatter_f32(uint64_t m, const uint64_t *pixel, const float *values, float *target, void *stream);
+
+/* curl B in grid-index units as the twist notebooks compute it
+ * (demo/Q_RK4.ipynb cell [11], demo/npzinputRKF45CuPyTwist.ipynb cell [9]):
+ * central differences on the notebook's index ranges, a one-sided
+ * second-order z derivative at z = 0 for curl_x and curl_y, zero elsewhere;
+ * arrays x fastest in dtype, curl in the same dtype. */
+int fqsl_curl_index(const void *bx, const void *by, const void *bz, int nx, int ny, int nz, int dtype, void *cx,
+                    void *cy, void *cz, void *stream);
+
 /* ------------------------------------------- reference-shaped entry points */
 /* Same parameters, order and meaning as the CUDA kernels the fastqslpy host
  * code launches (all device pointers; scalars as one-element device arrays),

@@ -11,6 +11,8 @@ contraction matches it bit for bit:
 * ``pseudo_seeds``   FastQSL.CookPseudoLine       fastqslpy/FastQSL.py:4-57
 * ``qcut``           Qcut assembly                demo/Q_RKF45CuPy.ipynb cell [10]
 * ``merge``          Q_all / Q_final merge        demo/Q_RKF45CuPy.ipynb cell [11]
+* ``slog10q``        signed log map               demo/Q_RKF45CuPy.ipynb cells [13]-[14]
+* ``bad_pixels``     the mixed method's selection demo/Q_RKF45MixCuPyDiffScott.ipynb cell [9]
 """
 from __future__ import annotations
 
@@ -133,3 +135,17 @@ def merge(Q_inner, qcut_vals, b_flag):
     Q_all[1:-1, 1:-1] = Q_inner
     Q_all[bf == 1] = qcut_vals
     return Q_all[1:-1, 1:-1]
+
+
+def slog10q(q, bn, flag=None):
+    """np.log10(Q) * np.sign(Bn) [* (flag <= 6)], float32, left to right."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.log10(np.asarray(q, f32)) * np.sign(np.asarray(bn, f32))
+        if flag is not None:
+            r = r * (np.asarray(flag) <= 6)
+    return r
+
+
+def bad_pixels(b_flag, q):
+    """Flat indices where B_flag + isnan(Q) > 0.5, ascending."""
+    return np.where((np.asarray(b_flag).reshape(-1) + np.isnan(np.asarray(q).reshape(-1))) > 0.5)[0]

@@ -8,7 +8,8 @@ the oracle's outputs.  They pin the oracle against drift (tests/test_golden.py
 regenerates them on the CPU) and give the GPU tests fixed expected values.
 No reference-held vector with inputs exists for this path, so these are the
 oracle's own -- themselves pinned by the closed-form tests of
-tests/test_oracle.py and tests/test_qplane.py.
+tests/test_oracle.py and tests/test_qplane.py.  ``Q.fits`` is the reference
+repository's own output file (no inputs), copied for the FITS format test.
 """
 import json
 import os
@@ -110,7 +111,22 @@ def make_qplane():
                         q=q, **dict(zip(names, args)))
 
 
+REFERENCE_QFITS = "/root/reference/Q.fits"
+
+
+def copy_qfits():
+    """Q.fits of the reference repository (q_0 and a BZ extension written by
+    astropy in demo/Q_RKF45CuPyScott.ipynb cell [14]): a data file, kept here
+    to check the FITS writer byte for byte against astropy's layout."""
+    import shutil
+    if os.path.exists(REFERENCE_QFITS):
+        shutil.copyfile(REFERENCE_QFITS, os.path.join(HERE, "Q.fits"))
+        return True
+    return False
+
+
 if __name__ == "__main__":
+    print("Q.fits copied" if copy_qfits() else "Q.fits: reference file not present, kept as is")
     oracle.build()
     for name in CASES:
         o = make(name)
