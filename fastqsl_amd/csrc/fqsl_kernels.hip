@@ -66,8 +66,8 @@ struct fqsl_field {
     void* axes_mem;
     size_t axes_bytes;
     bool stretched;
-    StretchGrid<float> sg32;
-    StretchGrid<double> sg64;
+    const StretchGrid<float>* sg32;   // device copies, beside the tables
+    const StretchGrid<double>* sg64;
 };
 
 __global__ void k_convert_to_f32(const double* in, float* out, long n) {
@@ -266,7 +266,7 @@ extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) 
 // and double views.  Landing planes: 0 / n (the reference rules) or the box
 // faces (exact rule).
 static int build_stretch(const std::vector<double> (&a)[3], const int* n, int rule, void** mem, size_t* bytes,
-                         StretchGrid<float>* g32, StretchGrid<double>* g64) {
+                         const StretchGrid<float>** g32, const StretchGrid<double>** g64) {
     if (rule != kLookupBisect && rule != kLookupUniform && rule != kLookupExact)
         return set_err(FQSL_ERR_INVALID, "lookup rule %d", rule);
     AxisTables t[3];
@@ -276,13 +276,21 @@ static int build_stretch(const std::vector<double> (&a)[3], const int* n, int ru
         if (why) return set_err(FQSL_ERR_INVALID, "axis %d: %s", q, why);
     }
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    size_t off64[3], off32[3], offr[3], offb[3], total = 0;
+    size_t off64[3], off32[3], offr[3], offb[3], offb64[3], offb32[3], total = 0;
+    for (int q = 0; q < 3; ++q) { offb64[q] = total; total = up(total + t[q].brec64.size() * sizeof(BinRec<double>)); }
+    for (int q = 0; q < 3; ++q) { offb32[q] = total; total = up(total + t[q].brec32.size() * sizeof(BinRec<float>)); }
     for (int q = 0; q < 3; ++q) { off64[q] = total; total = up(total + t[q].rec64.size() * sizeof(double4)); }
     for (int q = 0; q < 3; ++q) { off32[q] = total; total = up(total + t[q].rec32.size() * sizeof(float4)); }
     for (int q = 0; q < 3; ++q) { offr[q] = total; total = up(total + t[q].ref.size() * sizeof(int)); }
     for (int q = 0; q < 3; ++q) { offb[q] = total; total = up(total + t[q].bins.size() * sizeof(int)); }
+    const size_t offg32 = total;
+    total = up(total + sizeof(StretchGrid<float>));
+    const size_t offg64 = total;
+    total = up(total + sizeof(StretchGrid<double>));
     std::vector<char> host(total, 0);
     for (int q = 0; q < 3; ++q) {
+        memcpy(host.data() + offb64[q], t[q].brec64.data(), t[q].brec64.size() * sizeof(BinRec<double>));
+        memcpy(host.data() + offb32[q], t[q].brec32.data(), t[q].brec32.size() * sizeof(BinRec<float>));
         memcpy(host.data() + off64[q], t[q].rec64.data(), t[q].rec64.size() * sizeof(double4));
         memcpy(host.data() + off32[q], t[q].rec32.data(), t[q].rec32.size() * sizeof(float4));
         memcpy(host.data() + offr[q], t[q].ref.data(), t[q].ref.size() * sizeof(int));
@@ -290,17 +298,25 @@ static int build_stretch(const std::vector<double> (&a)[3], const int* n, int ru
     }
     char* d = nullptr;
     HIP_TRY(hipMalloc(&d, total));
+    const void* rec64[3] = {d + off64[0], d + off64[1], d + off64[2]};
+    const void* rec32[3] = {d + off32[0], d + off32[1], d + off32[2]};
+    const int* ref[3] = {(const int*)(d + offr[0]), (const int*)(d + offr[1]), (const int*)(d + offr[2])};
+    const int* bins[3] = {(const int*)(d + offb[0]), (const int*)(d + offb[1]), (const int*)(d + offb[2])};
+    const void* br64[3] = {d + offb64[0], d + offb64[1], d + offb64[2]};
+    const void* br32[3] = {d + offb32[0], d + offb32[1], d + offb32[2]};
+    StretchGrid<double> h64;
+    StretchGrid<float> h32;
+    fill_grid<double>(t, rule, br64, rec64, ref, bins, h64);
+    fill_grid<float>(t, rule, br32, rec32, ref, bins, h32);
+    memcpy(host.data() + offg32, &h32, sizeof(h32));
+    memcpy(host.data() + offg64, &h64, sizeof(h64));
     hipError_t e = hipMemcpy(d, host.data(), total, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(d);
         return set_err(FQSL_ERR_HIP, "hipMemcpy(axis tables): %s", hipGetErrorString(e));
     }
-    const void* rec64[3] = {d + off64[0], d + off64[1], d + off64[2]};
-    const void* rec32[3] = {d + off32[0], d + off32[1], d + off32[2]};
-    const int* ref[3] = {(const int*)(d + offr[0]), (const int*)(d + offr[1]), (const int*)(d + offr[2])};
-    const int* bins[3] = {(const int*)(d + offb[0]), (const int*)(d + offb[1]), (const int*)(d + offb[2])};
-    fill_grid<double>(t, rule, rec64, ref, bins, *g64);
-    fill_grid<float>(t, rule, rec32, ref, bins, *g32);
+    *g32 = (const StretchGrid<float>*)(d + offg32);
+    *g64 = (const StretchGrid<double>*)(d + offg64);
     *mem = d;
     *bytes = total;
     return FQSL_OK;
@@ -336,8 +352,8 @@ extern "C" int fqsl_field_set_axes(fqsl_field* f, const void* xa, const void* ya
     }
     void* mem = nullptr;
     size_t bytes = 0;
-    StretchGrid<float> g32;
-    StretchGrid<double> g64;
+    const StretchGrid<float>* g32 = nullptr;
+    const StretchGrid<double>* g64 = nullptr;
     int rc = build_stretch(a, f->n, lookup, &mem, &bytes, &g32, &g64);
     if (rc) return rc;
     if (f->axes_mem) { (void)hipFree(f->axes_mem); f->bytes -= f->axes_bytes; }
@@ -900,10 +916,15 @@ __device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const Uni
                                           float x, float y, float z) {
     plain_init<R>(L, f, P, x, y, z);
 }
+template <typename R>
+__device__ __forceinline__ const StretchGrid<R>& grid_of(const StretchRef<R>& g) {
+    return *g.g;
+}
+
 template <typename R, class F>
-__device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const StretchGrid<R>& g, const PlainParams& P,
+__device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const StretchRef<R>& g, const PlainParams& P,
                                           float x, float y, float z) {
-    stretch_init<R, F>(L, f, g, P, x, y, z);
+    stretch_init<R, F>(L, f, grid_of(g), P, x, y, z);
 }
 template <typename R, class F, class FJ>
 __device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const UniformGrid&, const PlainParams& P,
@@ -911,9 +932,9 @@ __device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const 
     plain_attempt<R, F, FJ>(L, f, P, fj);
 }
 template <typename R, class F, class FJ>
-__device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const StretchGrid<R>& g,
+__device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const StretchRef<R>& g,
                                              const PlainParams& P, const FJ& fj) {
-    stretch_attempt<R, F, FJ>(L, f, g, P, fj);
+    stretch_attempt<R, F, FJ>(L, f, grid_of(g), P, fj);
 }
 template <typename R, class F>
 __device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, const UniformGrid&, const PlainParams& P,
@@ -921,10 +942,10 @@ __device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, co
     plain_write<R>(L, f, P, i, o, ds);
 }
 template <typename R, class F>
-__device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, const StretchGrid<R>& g,
+__device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, const StretchRef<R>& g,
                                            const PlainParams& P, unsigned long long i, const PlainOut& o,
                                            const DevScalars&) {
-    stretch_write<R>(L, f, g, P, i, o);
+    stretch_write<R>(L, f, grid_of(g), P, i, o);
 }
 
 template <class F>
@@ -1811,23 +1832,25 @@ static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, 
     DevScalars ds{};
     if (precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
+        const StretchRef<double> g{field->sg64};
         FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
         if (o.tw) {
             FieldSoA<double> fj{field->cur64[0], field->cur64[1], field->cur64[2], {nn[0], nn[1], nn[2]}};
-            return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, field->sg64);
+            return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, g);
         }
-        return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, field->sg64);
+        return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
     }
+    const StretchRef<float> g{field->sg32};
     const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
     if (field->cell32) {
         FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, field->sg32);
-        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, field->sg32);
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, g);
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
     }
     if (field->soa32[0]) {
         FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, field->sg32);
-        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, field->sg32);
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, g);
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
     }
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
 }
@@ -1970,7 +1993,7 @@ struct DropinAxes {
     std::vector<double> a[3];
     int rule = -1;
     void* mem = nullptr;
-    StretchGrid<float> g;
+    const StretchGrid<float>* g = nullptr;
 };
 static std::mutex g_dropin_mu;
 static DropinAxes g_dropin_axes[64];
@@ -2014,7 +2037,7 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return set_err(FQSL_ERR_UNSUPPORTED, "device index %d", dev);
-    StretchGrid<float> g;
+    StretchRef<float> g{nullptr};
     {
         std::lock_guard<std::mutex> lock(g_dropin_mu);
         DropinAxes& c = g_dropin_axes[dev];
@@ -2022,18 +2045,20 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
         if (!same) {
             void* mem = nullptr;
             size_t bytes = 0;
-            StretchGrid<double> g64;
-            rc = build_stretch(a, shape, rule, &mem, &bytes, &c.g, &g64);
+            const StretchGrid<double>* g64 = nullptr;
+            const StretchGrid<float>* g32 = nullptr;
+            rc = build_stretch(a, shape, rule, &mem, &bytes, &g32, &g64);
             if (rc) return rc;
             if (c.mem) {  // earlier launches may still read the old tables
                 HIP_TRY(hipDeviceSynchronize());
                 (void)hipFree(c.mem);
             }
             c.mem = mem;
+            c.g = g32;
             c.rule = rule;
             for (int q = 0; q < 3; ++q) c.a[q] = a[q];
         }
-        g = c.g;
+        g.g = c.g;
     }
     PlainParams P;
     P.method = FQSL_METHOD_RKF45; P.s0 = 0.0; P.tol_coef = 1.0;

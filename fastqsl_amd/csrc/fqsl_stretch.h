@@ -26,11 +26,20 @@
 // (:385-386), which is the face only when the axis spans [0, n]; the exact
 // rule lands on a0 / a_{n-1}.
 //
-// On the device a point's holding cell comes from a uniform bin table (the
-// bin gives the lowest cell a point in it can lie in; one or two compares
-// finish the search), and one 16-byte record of that cell carries its bounds
-// together with the low node and the inverse width of the cell the rule
-// interpolates in:
+// On the device every rule is one table load: the axis is cut into uniform
+// bins narrower than the spacing of the rule's breakpoints (the interior
+// nodes for the bisection and exact rules, the points where
+// floor((p - a0) / (span / n)) steps for the uniform one), so a bin holds at
+// most one breakpoint, and each bin's record carries it (the split) and, for
+// each side, the cell the rule interpolates in with that cell's low node and
+// inverse width:
+//   brec[b] = {split, a[m_lo], 1/(a[m_lo+1]-a[m_lo]), a[m_hi], 1/(...), m_lo, m_hi}
+// (split = +inf for a bin without a breakpoint; a point above the split takes
+// m_hi).  For the uniform rule the split is the largest value, in the path's
+// precision, whose formula cell is still the lower one, so the reference's
+// rounding decides ties.  Where the bin count had to be capped and a bin
+// holds several breakpoints, the record says so and a search over per-cell
+// records (or the formula) takes over:
 //   rec[c] = {a[c], a[c+1], a[m], 1 / (a[m+1] - a[m])},  m = ref[c].
 // Built on the host (build_axis_tables) once per field.
 #pragma once
@@ -38,6 +47,8 @@
 #include "fqsl_tracer.h"
 
 #include <climits>
+#include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace fqsl {
@@ -48,9 +59,54 @@ template <typename T> struct Vec4;
 template <> struct Vec4<float> { using type = float4; };
 template <> struct Vec4<double> { using type = double4; };
 
+// One bin of an axis: 32 bytes in float, 48 in double, read with two or
+// three 16-byte loads issued together.  m_lo < 0 marks a bin that holds more
+// than one breakpoint (the search fallback).
+template <typename T>
+struct alignas(16) BinRec {
+    T split, a_lo, iw_lo, a_hi, iw_hi;
+    int m_lo, m_hi;
+};
+
+FQSL_HD int bits_of(float f) {
+    int i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+
+// The bin records live in global memory; say so to the device compiler so
+// the loads are global (not flat) loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FQSL_GLOBAL __attribute__((address_space(1)))
+#else
+#define FQSL_GLOBAL
+#endif
+
+FQSL_HD BinRec<float> load_binrec(const BinRec<float>* p) {
+    const FQSL_GLOBAL float4* q = (const FQSL_GLOBAL float4*)p;
+    const float4 v0 = q[0], v1 = q[1];
+    BinRec<float> r;
+    r.split = v0.x; r.a_lo = v0.y; r.iw_lo = v0.z; r.a_hi = v0.w;
+    r.iw_hi = v1.x; r.m_lo = bits_of(v1.y); r.m_hi = bits_of(v1.z);
+    return r;
+}
+
+FQSL_HD BinRec<double> load_binrec(const BinRec<double>* p) {
+    const FQSL_GLOBAL double2* q = (const FQSL_GLOBAL double2*)p;
+    const double2 v0 = q[0], v1 = q[1], v2 = q[2];
+    BinRec<double> r;
+    r.split = v0.x; r.a_lo = v0.y; r.iw_lo = v1.x; r.a_hi = v1.y; r.iw_hi = v2.x;
+    long long m;
+    memcpy(&m, &v2.y, 8);
+    r.m_lo = (int)(m & 0xffffffffll);
+    r.m_hi = (int)(m >> 32);
+    return r;
+}
+
 template <typename T>
 struct StretchAxis {
-    const typename Vec4<T>::type* rec;  // per holding cell, see above
+    const BinRec<T>* brec;              // per uniform bin, see above
+    const typename Vec4<T>::type* rec;  // per holding cell, see above (multi-breakpoint bins)
     const int* ref;                     // per holding cell: the cell the rule interpolates in
     const int* bins;                    // per uniform bin: lowest cell a point of the bin can lie in
     T lo, hi;                           // a[0], a[n-1]
@@ -69,6 +125,42 @@ struct StretchGrid {
     double len_lim;                 // 2 x (sum of the three spans), :330-332
 };
 
+// What the kernels take: the grid lives in device memory beside its tables
+// (as a kernel argument its ~80 dwords would crowd the scalar registers).
+template <typename T>
+struct StretchRef {
+    const StretchGrid<T>* g;
+};
+
+template <typename T>
+struct CellWeight {
+    int c;
+    T w;
+};
+
+// A bin that holds several breakpoints (only where the bin count had to be
+// capped): the uniform formula, or a search for the holding cell from the
+// bin's lowest possible one followed by the rule's cell.
+template <typename T>
+FQSL_COLD CellWeight<T> stretch_search(const StretchAxis<T>* A, int b, int rule, T p, T phi, T plo) {
+    CellWeight<T> o;
+    if (rule == kLookupUniform) {  // the reference's division, so ties fall its way
+        int k = to_int(flr((p - A->lo) / A->uni_width));
+        k = k < 0 ? 0 : (k > A->n - 2 ? A->n - 2 : k);
+        const auto r = A->rec[k];
+        o.c = k;
+        o.w = ((phi - r.x) + plo) * r.w;
+        return o;
+    }
+    int k = A->bins[b];
+    auto r = A->rec[k];
+    while (k < A->n - 2 && p > r.y) { ++k; r = A->rec[k]; }
+    while (k > 0 && !(p > r.x)) { --k; r = A->rec[k]; }
+    o.c = (rule == kLookupExact) ? k : A->ref[k];
+    o.w = ((phi - r.z) + plo) * r.w;
+    return o;
+}
+
 // Cell and weight of the stage coordinate phi + plo along one axis (phi the
 // leading part, plo a small remainder: the weight is formed as
 // ((phi - a[m]) + plo) / width so it keeps the remainder's resolution).
@@ -77,22 +169,16 @@ FQSL_HD void stretch_locate(const StretchAxis<T>& A, int rule, T phi, T plo, int
     T p = phi + plo;
     if (!(p > A.lo)) { p = A.clo; phi = A.clo; plo = (T)0; }
     if (!(p < A.hi)) { p = A.chi; phi = A.chi; plo = (T)0; }
-    if (rule == kLookupUniform) {  // the reference's division, so ties at a breakpoint fall the same way
-        int k = to_int(flr((p - A.lo) / A.uni_width));
-        k = k < 0 ? 0 : (k > A.n - 2 ? A.n - 2 : k);
-        const auto r = A.rec[k];  // built with m = c for this rule
-        c = k;
-        w = ((phi - r.x) + plo) * r.w;
-        return;
-    }
     int b = to_int((p - A.lo) * A.bin_scale);
     b = b < 0 ? 0 : (b >= A.nbins ? A.nbins - 1 : b);
-    int k = A.bins[b];
-    auto r = A.rec[k];
-    while (k < A.n - 2 && p > r.y) { ++k; r = A.rec[k]; }
-    while (k > 0 && !(p > r.x)) { --k; r = A.rec[k]; }
-    c = (rule == kLookupExact) ? k : A.ref[k];
-    w = ((phi - r.z) + plo) * r.w;
+    const BinRec<T> br = load_binrec(A.brec + b);
+    const bool up = p > br.split;  // on a node: the cell below it
+    c = up ? br.m_hi : br.m_lo;
+    w = ((phi - (up ? br.a_hi : br.a_lo)) + plo) * (up ? br.iw_hi : br.iw_lo);
+    if (br.m_lo >= 0) return;
+    const CellWeight<T> s = stretch_search<T>(&A, b, rule, p, phi, plo);
+    c = s.c;
+    w = s.w;
 }
 
 // Leading part and remainder of a stage point x + delta.
@@ -345,33 +431,113 @@ inline int reference_bisect(double p, int n, const double* a) {
 struct AxisTables {
     std::vector<float4> rec32;
     std::vector<double4> rec64;
+    std::vector<BinRec<float>> brec32;
+    std::vector<BinRec<double>> brec64;
     std::vector<int> ref, bins;
     double lo = 0, hi = 0;
-    int n = 0, nbins = 0;
+    int n = 0, nbins = 0, multi = 0;  // multi: bins holding more than one interior node
 };
 
 constexpr int kMaxStretchBins = 1 << 16;
+
+// The uniform rule's cell of p evaluated in T as the reference does:
+// floor((p - a0) / w) with w = (a_{n-1} - a0) / n in T, clamped to [0, n-2].
+template <typename T>
+inline int uniform_cell(T p, T a0, T w, int n) {
+    const T r = std::floor((p - a0) / w);
+    if (!(r > (T)0)) return 0;
+    return r > (T)(n - 2) ? n - 2 : (int)r;
+}
+
+// Finite T values in their numeric order as integers (+0 and -0 both map to
+// 0), and back.
+template <typename T>
+inline long long ordinal_of(T x) {
+    if (sizeof(T) == 4) {
+        int i;
+        memcpy(&i, &x, 4);
+        return i >= 0 ? (long long)i : -(long long)(i & 0x7fffffff);
+    }
+    long long i;
+    memcpy(&i, &x, 8);
+    return i >= 0 ? i : -(i & 0x7fffffffffffffffLL);
+}
+
+template <typename T>
+inline T from_ordinal(long long o) {
+    T x;
+    if (sizeof(T) == 4) {
+        const unsigned u = o >= 0 ? (unsigned)o : ((unsigned)(-o) | 0x80000000u);
+        memcpy(&x, &u, 4);
+    } else {
+        const unsigned long long u = o >= 0 ? (unsigned long long)o : ((unsigned long long)(-o) | (1ULL << 63));
+        memcpy(&x, &u, 8);
+    }
+    return x;
+}
+
+// The largest T value whose uniform-rule cell is below j (the split of
+// breakpoint j): a point above it takes cell j.  The cell is monotone in p,
+// so this bisects over the ordered values between a bracket around the
+// guess (at most 64 halvings, wherever the split lies, zero included).
+template <typename T>
+inline T uniform_split(int j, T a0, T w, int n, double guess) {
+    const double step = 2.0 * (double)w;
+    double dl = guess - step, dh = guess + step;
+    while (uniform_cell<T>((T)dl, a0, w, n) >= j) dl -= (guess - dl);
+    while (uniform_cell<T>((T)dh, a0, w, n) < j) dh += (dh - guess);
+    long long lo = ordinal_of<T>((T)dl), hi = ordinal_of<T>((T)dh);
+    // the gap between ordinals of opposite signs can exceed LLONG_MAX
+    for (unsigned long long gap; (gap = (unsigned long long)hi - (unsigned long long)lo) > 1;) {
+        const long long mid = lo + (long long)(gap / 2);
+        if (uniform_cell<T>(from_ordinal<T>(mid), a0, w, n) >= j) hi = mid;
+        else lo = mid;
+    }
+    return from_ordinal<T>(lo);
+}
+
+// Bin records of one precision from the breakpoints split[j - 1] (between
+// cells j - 1 and j, ascending) and the rule's cell map.
+template <typename T>
+inline int fill_bins(const std::vector<T>& split, const std::vector<int>& ref, const double* a, int n, double lo,
+                     double span, int nbins, std::vector<BinRec<T>>& out) {
+    out.resize(nbins);
+    const int ns = (int)split.size(), nc = n - 1;
+    int below = 0, multi = 0;
+    for (int b = 0; b < nbins; ++b) {
+        const double e = lo + span * ((double)b / (double)nbins);
+        const double e1 = (b + 1 == nbins) ? lo + 2.0 * span : lo + span * ((double)(b + 1) / (double)nbins);
+        while (below < ns && (double)split[below] < e) ++below;
+        int inside = 0;
+        while (below + inside < ns && (double)split[below + inside] < e1) ++inside;
+        const int lo_c = below, hi_c = below + 1 < nc ? below + 1 : nc - 1;
+        const int m_lo = ref[lo_c], m_hi = ref[hi_c];
+        BinRec<T> r;
+        r.split = inside == 1 ? split[below] : (T)HUGE_VAL;
+        r.a_lo = (T)a[m_lo];
+        r.iw_lo = (T)(1.0 / (a[m_lo + 1] - a[m_lo]));
+        r.a_hi = (T)a[m_hi];
+        r.iw_hi = (T)(1.0 / (a[m_hi + 1] - a[m_hi]));
+        r.m_lo = inside > 1 ? -1 : m_lo;  // several breakpoints: search
+        r.m_hi = m_hi;
+        out[b] = r;
+        multi += inside > 1 ? 1 : 0;
+    }
+    return multi;
+}
 
 // Tables of one strictly increasing axis of n >= 2 finite nodes; returns an
 // error message or nullptr.
 inline const char* build_axis_tables(const double* a, int n, int rule, AxisTables& t) {
     if (n < 2) return "an axis needs at least 2 nodes";
-    double dmin = 0.0;
     for (int i = 0; i < n; ++i) {
         if (!(a[i] == a[i]) || a[i] - a[i] != 0.0) return "axis coordinates must be finite";
-        if (i > 0) {
-            const double d = a[i] - a[i - 1];
-            if (!(d > 0.0)) return "axis coordinates must be strictly increasing";
-            dmin = (i == 1 || d < dmin) ? d : dmin;
-        }
+        if (i > 0 && !(a[i] - a[i - 1] > 0.0)) return "axis coordinates must be strictly increasing";
     }
     t.n = n;
     t.lo = a[0];
     t.hi = a[n - 1];
     const double span = t.hi - t.lo;
-    double want = 2.0 * span / dmin + 1.0;
-    if (want < n) want = n;
-    t.nbins = want > kMaxStretchBins ? kMaxStretchBins : (int)want;
     const int nc = n - 1;
     t.ref.resize(nc);
     t.rec32.resize(nc);
@@ -384,6 +550,31 @@ inline const char* build_axis_tables(const double* a, int n, int rule, AxisTable
         t.rec64[c] = make_double4(a[c], a[c + 1], a[m], iw);
         t.rec32[c] = make_float4((float)a[c], (float)a[c + 1], (float)a[m], (float)iw);
     }
+    // breakpoints: interior nodes, or where the uniform formula steps
+    std::vector<double> s64(n - 2);
+    std::vector<float> s32(n - 2);
+    const double w64 = (a[n - 1] - a[0]) / (double)n;
+    const float a0f = (float)a[0], w32 = ((float)a[n - 1] - a0f) / (float)n;
+    for (int j = 1; j <= n - 2; ++j) {
+        if (rule == kLookupUniform) {
+            const double guess = a[0] + j * w64;
+            s64[j - 1] = uniform_split<double>(j, a[0], w64, n, guess);
+            s32[j - 1] = uniform_split<float>(j, a0f, w32, n, guess);
+        } else {
+            s64[j - 1] = a[j];
+            s32[j - 1] = (float)a[j];
+        }
+    }
+    // bins narrower than the closest two breakpoints
+    double dmin = span;
+    for (int j = 1; j < n - 2; ++j) {
+        const double d64 = s64[j] - s64[j - 1], d32 = (double)s32[j] - (double)s32[j - 1];
+        if (d64 > 0.0 && d64 < dmin) dmin = d64;
+        if (d32 > 0.0 && d32 < dmin) dmin = d32;
+    }
+    double want = 1.25 * span / dmin + 2.0;
+    if (want < n) want = n;
+    t.nbins = want > kMaxStretchBins ? kMaxStretchBins : (int)want;
     // bins[b]: the cell of the bin's lower edge e (largest c with a[c] < e, or 0)
     t.bins.resize(t.nbins);
     int c = 0;
@@ -392,21 +583,25 @@ inline const char* build_axis_tables(const double* a, int n, int rule, AxisTable
         while (c < nc - 1 && a[c + 1] < e) ++c;
         t.bins[b] = c;
     }
+    const int m64 = fill_bins<double>(s64, t.ref, a, n, t.lo, span, t.nbins, t.brec64);
+    const int m32 = fill_bins<float>(s32, t.ref, a, n, t.lo, span, t.nbins, t.brec32);
+    t.multi = m64 > m32 ? m64 : m32;
     return nullptr;
 }
 
-// The grid view over tables placed at rec[q] (float4 or double4 per cell of
-// axis q), ref[q] and bins[q]: clamp bounds, bin and cell scales, the box,
+// The grid view over tables placed at brec[q] (BinRec<T> per bin of axis q),
+// rec[q] (float4 or double4 per cell), ref[q] and bins[q]: clamp bounds, bin and cell scales, the box,
 // the landing planes (0 / n for the reference rules, the faces for the exact
 // rule) and len_lim.
 template <typename T>
-inline void fill_grid(const AxisTables (&t)[3], int rule, const void* const (&rec)[3], const int* const (&ref)[3],
-                      const int* const (&bins)[3], StretchGrid<T>& g) {
+inline void fill_grid(const AxisTables (&t)[3], int rule, const void* const (&brec)[3], const void* const (&rec)[3],
+                      const int* const (&ref)[3], const int* const (&bins)[3], StretchGrid<T>& g) {
     double span = 0.0;
     for (int q = 0; q < 3; ++q) {
         const double lo = t[q].lo, hi = t[q].hi;
         span += hi - lo;
         StretchAxis<T>& A = g.ax[q];
+        A.brec = (const BinRec<T>*)brec[q];
         A.rec = (const typename Vec4<T>::type*)rec[q];
         A.ref = ref[q];
         A.bins = bins[q];

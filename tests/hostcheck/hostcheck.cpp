@@ -147,8 +147,10 @@ int hc_stretch_cell(const double* a, int n, int rule, int precision, long npts, 
     const int* bins[3] = {t[0].bins.data(), t[1].bins.data(), t[2].bins.data()};
     StretchGrid<double> g64;
     StretchGrid<float> g32;
-    fill_grid<double>(t, rule, r64, ref, bins, g64);
-    fill_grid<float>(t, rule, r32, ref, bins, g32);
+    const void* b64[3] = {t[0].brec64.data(), t[1].brec64.data(), t[2].brec64.data()};
+    const void* b32[3] = {t[0].brec32.data(), t[1].brec32.data(), t[2].brec32.data()};
+    fill_grid<double>(t, rule, b64, r64, ref, bins, g64);
+    fill_grid<float>(t, rule, b32, r32, ref, bins, g32);
     for (long i = 0; i < npts; ++i) {
         if (precision == 1) {
             double wd;
@@ -182,8 +184,10 @@ int hc_trace_stretch(const double* bx, const double* by, const double* bz, int n
     const int* bins[3] = {t[0].bins.data(), t[1].bins.data(), t[2].bins.data()};
     StretchGrid<double> g64;
     StretchGrid<float> g32;
-    fill_grid<double>(t, rule, r64, ref, bins, g64);
-    fill_grid<float>(t, rule, r32, ref, bins, g32);
+    const void* b64[3] = {t[0].brec64.data(), t[1].brec64.data(), t[2].brec64.data()};
+    const void* b32[3] = {t[0].brec32.data(), t[1].brec32.data(), t[2].brec32.data()};
+    fill_grid<double>(t, rule, b64, r64, ref, bins, g64);
+    fill_grid<float>(t, rule, b32, r32, ref, bins, g32);
     PlainParams P;
     P.method = kRKF45; P.s0 = s0; P.tol_coef = tol_coef;
     P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];

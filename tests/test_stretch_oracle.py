@@ -217,9 +217,16 @@ def test_twist_in_a_unit_force_free_field(oracle):
 def test_table_lookup_matches_oracle(oracle, hostcheck, rule):
     rng = np.random.default_rng(11)
     for n in (2, 3, 4, 5, 8, 16, 33, 258, 342):
-        for kind in ("sinh", "tanh", "geometric", "random"):
+        for kind in ("sinh", "tanh", "geometric", "random", "cluster"):
             if kind == "random":
                 a = np.cumsum(rng.uniform(0.05, 3.0, n)) - 7.0
+            elif kind == "cluster":
+                # a few nodes 1e-6 apart: the bin count is capped and bins
+                # hold several nodes (the lookup's search fallback)
+                a = np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n - 1)])
+                if n >= 8:
+                    a[n // 2 + 1:n // 2 + 4] = a[n // 2] + 1e-6 * np.arange(1, 4)
+                    a[n // 2 + 4:] += 1e-6 * 4
             else:
                 a = synth.stretched_axis(n, -3.0, 40.0, kind, 2.0, dtype=np.float64)
             mids = 0.5 * (a[:-1] + a[1:])
@@ -233,6 +240,8 @@ def test_table_lookup_matches_oracle(oracle, hostcheck, rule):
             if np.any(np.diff(a32) <= 0):
                 continue
             mids32 = 0.5 * (a32[:-1] + a32[1:])
+            # cells a few float32 ulps wide have no float32 midpoint
+            mids32 = mids32[np.diff(a32) > 1e-5 * np.maximum(np.abs(a32[1:]), 1.0)]
             if rule == "uniform":  # its breakpoints lie at a0 + k span / n, not at nodes
                 u = (mids32 - a32[0]) * n / (a32[-1] - a32[0])
                 mids32 = mids32[np.abs(u - np.round(u)) > 1e-4]
@@ -307,3 +316,44 @@ def test_host_build_fp32c_reference_rules(oracle, hostcheck, rule):
     cls, close = _agree(o, h, 1e-2)
     assert cls >= min(c for c, _ in spread) - 0.03, (cls, spread)
     assert close >= min(c for _, c in spread) - 0.05, (close, spread)
+
+
+def _formula_steps(a0, w, n, dtype):
+    """Points around each place where floor((p - a0) / w) steps: the
+    estimate a0 + j w and its neighbours at the resolution of p - a0 (the
+    subtraction rounds to ulp(max(|p|, |a0|))), so the sweep crosses the
+    true step even where it sits at p = 0."""
+    pts = []
+    for j in range(1, n - 1):
+        t = dtype(a0 + dtype(j) * w)
+        res = np.spacing(max(abs(t), abs(a0), dtype(j) * w)) / 2
+        pts.append(t + res * np.arange(-8, 9, dtype=dtype))
+        u = t
+        for _ in range(4):
+            u = np.nextafter(u, dtype(-np.inf))
+        for _ in range(9):
+            pts.append(np.array([u], dtype))
+            u = np.nextafter(u, dtype(np.inf))
+    return np.concatenate(pts)
+
+
+@pytest.mark.parametrize("n,lo,hi", [(60, -1.0, 1.0), (56, -1.0, 1.0), (17, 0.0, 16.0), (258, -5.3, 7.1),
+                                     (48, 0.0, 2.0)])
+def test_uniform_rule_ties_follow_the_formula(oracle, hostcheck, n, lo, hi):
+    """At the uniform rule's breakpoints the tables must pick the cell the
+    reference's arithmetic picks, floor((p - a0) / ((a_n-1 - a0) / n))
+    (GetIndexUniStretch, :68-75): in float64 the oracle's cell, in float32
+    the formula in float32, on every point of a sweep across each step
+    (the even-n axes on [-1, 1] put a step at p = 0)."""
+    a = np.linspace(lo, hi, n).astype(np.float32).astype(np.float64)
+    inside = lambda p, t: p[(p > t(a[0])) & (p < t(a[-1]))]
+    p64 = inside(_formula_steps(a[0], (a[-1] - a[0]) / n, n, np.float64), np.float64)
+    want, _ = oracle.stretch_cell(a, p64, "uniform")
+    got, _ = hostcheck.stretch_cell(a, p64, "uniform", precision=1)
+    np.testing.assert_array_equal(got, want)
+    a0 = np.float32(a[0])
+    w = (np.float32(a[-1]) - a0) / np.float32(n)
+    p32 = inside(_formula_steps(a0, w, n, np.float32), np.float32)
+    want = np.clip(np.floor((p32 - a0) / w), 0, n - 2).astype(np.int32)
+    got, _ = hostcheck.stretch_cell(a, p32.astype(np.float64), "uniform", precision=0)
+    np.testing.assert_array_equal(got, want)
