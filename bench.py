@@ -53,7 +53,9 @@ def parse():
     ap.add_argument("--per-line-fp32", action="store_true",
                     help="also time the FP32 one-thread-per-line kernel (always on for volume)")
     ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
-    ap.add_argument("--schedule", default="persistent", choices=["persistent", "per_line"])
+    ap.add_argument("--schedule", default=None, choices=["persistent", "per_line"],
+                    help="persistent (default) or per_line; the stretch workload defaults to per_line, "
+                         "which is the faster schedule for its short lines")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
@@ -367,8 +369,10 @@ def bench_stretch(args, fq, torch, log):
     cells 0.055-0.13) and z spacing growing with height (geometric, strength
     3; cells 0.013-0.24), box [-10,10]^2 x [0,20]; 1024^2 seeds on z = 1e-3
     over [-9.5, 9.5]^2, s_len 0.01, tol_coef 1e-2, the reference's bisection
-    cell rule (flag_uni false).  FP32c persistent against FP64 one
-    thread per line on the same GPU, and the FP64 oracle on the host."""
+    cell rule (flag_uni false).  FP32c (one thread per line by default:
+    these lines are short, ~200 evaluations, and adjacent seeds in one
+    wavefront share cache lines) against FP64 one thread per line on the
+    same GPU, and the FP64 oracle on the host."""
     dev = torch.device("cuda", 0)
     steps = args.steps or 5
     warm = args.warmup if args.warmup is not None else 1
@@ -427,7 +431,7 @@ def bench_stretch(args, fq, torch, log):
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         rng = np.random.default_rng(1)
-        idx = np.sort(rng.choice(nlines, size=min(args.cpu_sample >> 2, nlines), replace=False))
+        idx = np.sort(rng.choice(nlines, size=min(args.cpu_sample << 2, nlines), replace=False))
         f64 = tuple(np.asarray(b, np.float64) for b in (bx, by, bz))
         ax64 = [np.asarray(a, np.float64) for a in axes]
         nt = O.num_threads()
@@ -462,7 +466,7 @@ def bench_stretch(args, fq, torch, log):
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_plain_persistent (stretched)", "kernel_ms": k,
+                     "kernel": "k_plain_%s (stretched)" % args.schedule, "kernel_ms": k,
                      "evals_per_launch": evals, "bytes_per_eval": bpe},
         "cpu_baseline": cpu,
         "classes": {str(int(c)): int(v) for c, v in zip(classes, counts)},
@@ -476,7 +480,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(lines are independent, so a multi-GPU run shards seeds; see DESIGN.md)",
+              "(lines are independent, so a multi-GPU run would shard the seeds over one process per GPU)",
               file=sys.stderr)
         sys.exit(2)
     import torch
@@ -487,6 +491,8 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
 
+    if args.schedule is None:
+        args.schedule = "per_line" if args.workload == "stretch" else "persistent"
     if args.workload == "serve":
         return bench_serve(args, fq, torch, log)
     if args.workload == "stretch":

@@ -125,7 +125,7 @@ def check(rc: int) -> None:
 
 
 TUNE_KNOBS = {"scott_waves": 0, "grid_pct": 1, "queues": 2, "refill_min": 3, "order": 4,
-              "order_buckets": 5}
+              "order_buckets": 5, "plain_refill": 6}
 
 
 def tune_set(knob: str, value: int) -> None:

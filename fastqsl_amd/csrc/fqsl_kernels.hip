@@ -365,7 +365,7 @@ extern "C" int fqsl_field_set_axes(fqsl_field* f, const void* xa, const void* ya
 extern "C" int fqsl_field_stretched(const fqsl_field* f) { return f && f->stretched ? 1 : 0; }
 
 // ------------------------------------------------------------- tuning knobs
-static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64};
+static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 32};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
@@ -373,7 +373,7 @@ extern "C" int fqsl_tune_set(int knob, int value) {
         return set_err(FQSL_ERR_INVALID, "scott waves/SIMD target %d not in {0, 2, 3}", value);
     if (knob == FQSL_TUNE_QUEUES && value != 1 && value != 8)
         return set_err(FQSL_ERR_INVALID, "queue count %d not in {1, 8}", value);
-    if (knob == FQSL_TUNE_REFILL_MIN && (value < 1 || value > 64))
+    if ((knob == FQSL_TUNE_REFILL_MIN || knob == FQSL_TUNE_PLAIN_REFILL) && (value < 1 || value > 64))
         return set_err(FQSL_ERR_INVALID, "refill group %d not in 1..64", value);
     if (knob == FQSL_TUNE_ORDER && (value < 0 || value > 2))
         return set_err(FQSL_ERR_INVALID, "order knob must be 0, 1 or 2 (got %d)", value);
@@ -875,7 +875,7 @@ __device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, c
 // The stretched tracer writes B_start / B_end / B_this by the stretched
 // Interp3dxyzn (raw, TraceBlineAdaptiveStretch.cu:463-496).
 template <typename R, class F>
-__device__ __forceinline__ void stretch_write(const PlainLine<R>& L, const F& f, const StretchGrid<R>& g,
+__device__ __forceinline__ void stretch_write(const PlainLine<R>& L, const F& f, const FQSL_CONST StretchGrid<R>& g,
                                               const PlainParams& P, unsigned long long i, const PlainOut& o) {
     const float xs[3] = {(float)L.xs[0], (float)L.xs[1], (float)L.xs[2]};
     const float xe[3] = {(float)L.xe[0], (float)L.xe[1], (float)L.xe[2]};
@@ -917,8 +917,8 @@ __device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const Uni
     plain_init<R>(L, f, P, x, y, z);
 }
 template <typename R>
-__device__ __forceinline__ const StretchGrid<R>& grid_of(const StretchRef<R>& g) {
-    return *g.g;
+__device__ __forceinline__ const FQSL_CONST StretchGrid<R>& grid_of(const StretchRef<R>& g) {
+    return *(const FQSL_CONST StretchGrid<R>*)g.g;
 }
 
 template <typename R, class F>
@@ -959,10 +959,14 @@ __device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsign
     P.ncross[0] = ncross[0]; P.ncross[1] = ncross[1]; P.ncross[2] = ncross[2];
 }
 
+// Persistent plain tracing: each lane owns one line (forward, then backward)
+// and writes its outputs when it ends; idle lanes take the next queue
+// entries once at least `refill_min` of the wavefront's lanes are idle (or
+// all are), so refills and their setup run together.
 template <typename R, class F, class FJ, class G>
 __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Seeds s, unsigned long long n,
                                                           PlainOut out, DevScalars ds, const float* ncross,
-                                                          unsigned long long* queue, FJ fj, G g) {
+                                                          unsigned long long* queue, int refill_min, FJ fj, G g) {
     apply_plain_scalars(f, P, n, ds, ncross);
     fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     PlainLine<R> L;
@@ -974,9 +978,9 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
     for (;;) {
         if (!drained) {
             const unsigned long long need = __ballot(!active);
-            if (need) {
+            const int cnt = __popcll(need);
+            if (cnt >= refill_min || need == ~0ull) {
                 const int leader = __ffsll((long long)need) - 1;
-                const int cnt = __popcll(need);
                 unsigned long long base = 0;
                 if (lane == leader) base = atomicAdd(queue, (unsigned long long)cnt);
                 base = bcast64(base, leader);
@@ -1154,7 +1158,8 @@ static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsign
         HIP_TRY(hipGetLastError());
         rc = record_start(st);
         if (rc) return rc;
-        k_plain_persistent<R, F, FJ, G><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q, fj, g);
+        k_plain_persistent<R, F, FJ, G><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q,
+                                                             g_tune[FQSL_TUNE_PLAIN_REFILL], fj, g);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }

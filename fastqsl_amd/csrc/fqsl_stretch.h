@@ -74,14 +74,6 @@ FQSL_HD int bits_of(float f) {
     return i;
 }
 
-// The bin records live in global memory; say so to the device compiler so
-// the loads are global (not flat) loads.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define FQSL_GLOBAL __attribute__((address_space(1)))
-#else
-#define FQSL_GLOBAL
-#endif
-
 FQSL_HD BinRec<float> load_binrec(const BinRec<float>* p) {
     const FQSL_GLOBAL float4* q = (const FQSL_GLOBAL float4*)p;
     const float4 v0 = q[0], v1 = q[1];
@@ -142,7 +134,7 @@ struct CellWeight {
 // capped): the uniform formula, or a search for the holding cell from the
 // bin's lowest possible one followed by the rule's cell.
 template <typename T>
-FQSL_COLD CellWeight<T> stretch_search(const StretchAxis<T>* A, int b, int rule, T p, T phi, T plo) {
+FQSL_COLD CellWeight<T> stretch_search(const FQSL_CONST StretchAxis<T>* A, int b, int rule, T p, T phi, T plo) {
     CellWeight<T> o;
     if (rule == kLookupUniform) {  // the reference's division, so ties fall its way
         int k = to_int(flr((p - A->lo) / A->uni_width));
@@ -165,7 +157,7 @@ FQSL_COLD CellWeight<T> stretch_search(const StretchAxis<T>* A, int b, int rule,
 // leading part, plo a small remainder: the weight is formed as
 // ((phi - a[m]) + plo) / width so it keeps the remainder's resolution).
 template <typename T>
-FQSL_HD void stretch_locate(const StretchAxis<T>& A, int rule, T phi, T plo, int& c, T& w) {
+FQSL_HD void stretch_locate(const FQSL_CONST StretchAxis<T>& A, int rule, T phi, T plo, int& c, T& w) {
     T p = phi + plo;
     if (!(p > A.lo)) { p = A.clo; phi = A.clo; plo = (T)0; }
     if (!(p < A.hi)) { p = A.chi; phi = A.chi; plo = (T)0; }
@@ -198,7 +190,7 @@ FQSL_HD void point_parts(const double* x, R* phi, R* plo) {
 
 // Raw B (Interp3dxyzn(..., false), :82-194) at a split stage point.
 template <typename R, class F>
-FQSL_HD void stretch_raw(const F& f, const StretchGrid<R>& g, const R* phi, const R* plo, R B[3]) {
+FQSL_HD void stretch_raw(const F& f, const FQSL_CONST StretchGrid<R>& g, const R* phi, const R* plo, R B[3]) {
     int c[3];
     R w[3];
 #pragma unroll
@@ -209,7 +201,7 @@ FQSL_HD void stretch_raw(const F& f, const StretchGrid<R>& g, const R* phi, cons
 }
 
 template <typename R, class F>
-FQSL_HD void stretch_unit(const F& f, const StretchGrid<R>& g, const R* phi, const R* plo, R b[3]) {
+FQSL_HD void stretch_unit(const F& f, const FQSL_CONST StretchGrid<R>& g, const R* phi, const R* plo, R b[3]) {
     R B[3];
     stretch_raw<R, F>(f, g, phi, plo, B);
     const R inb = rsq(dot3(B, B));
@@ -218,14 +210,14 @@ FQSL_HD void stretch_unit(const F& f, const StretchGrid<R>& g, const R* phi, con
 }
 
 template <typename R, class F>
-FQSL_HD void stretch_unit_at(const F& f, const StretchGrid<R>& g, const double* x, R b[3]) {
+FQSL_HD void stretch_unit_at(const F& f, const FQSL_CONST StretchGrid<R>& g, const double* x, R b[3]) {
     R phi[3], plo[3];
     point_parts<R>(x, phi, plo);
     stretch_unit<R, F>(f, g, phi, plo, b);
 }
 
 template <typename R, class F>
-FQSL_HD void stretch_raw_at(const F& f, const StretchGrid<R>& g, const double* x, double B[3]) {
+FQSL_HD void stretch_raw_at(const F& f, const FQSL_CONST StretchGrid<R>& g, const double* x, double B[3]) {
     R phi[3], plo[3], Bv[3];
     point_parts<R>(x, phi, plo);
     stretch_raw<R, F>(f, g, phi, plo, Bv);
@@ -255,7 +247,7 @@ FQSL_HD int check_flag_box(const double* lo, const double* hi, const double* p) 
 // J.b/|b|^2 (b the normalised B at p1, :403-406) that the uniform RKF45
 // tracer has commented out.
 template <typename R, class F>
-FQSL_HD void stretch_init(PlainLine<R>& L, const F& f, const StretchGrid<R>& g, const PlainParams& P, float px,
+FQSL_HD void stretch_init(PlainLine<R>& L, const F& f, const FQSL_CONST StretchGrid<R>& g, const PlainParams& P, float px,
                           float py, float pz) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
     L.len_lim = g.len_lim;
@@ -276,7 +268,7 @@ FQSL_HD void stretch_init(PlainLine<R>& L, const F& f, const StretchGrid<R>& g, 
 
 // RK4_boundary (:274-296) on the stretched interpolant.
 template <typename R, class F>
-FQSL_HD void stretch_rk4_boundary(const F& f, const StretchGrid<R>& g, const double* p, double s, int dim,
+FQSL_HD void stretch_rk4_boundary(const F& f, const FQSL_CONST StretchGrid<R>& g, const double* p, double s, int dim,
                                     double out[3]) {
     double k[4][3], t[3];
     const double cf[4] = {0.0, 0.5, 0.5, 1.0};
@@ -315,7 +307,7 @@ FQSL_HD void rkf45_offset(int st, const R k[6][3], R t[3]) {
 }
 
 template <typename R, class F, class FJ = NoCurrent>
-FQSL_HD void stretch_attempt(PlainLine<R>& L, const F& f, const StretchGrid<R>& g, const PlainParams& P,
+FQSL_HD void stretch_attempt(PlainLine<R>& L, const F& f, const FQSL_CONST StretchGrid<R>& g, const PlainParams& P,
                              const FJ& fj = FJ{}) {
     const R sc = (R)(L.s * L.dir);
     L.attempts += 1;

@@ -49,6 +49,18 @@
 #define FQSL_HD __host__ __device__ __forceinline__
 #define FQSL_COLD __host__ __device__ __attribute__((noinline))
 
+// Address spaces for the device compiler: the field and lookup tables live
+// in global memory (global, not flat, loads), and the small per-grid
+// parameter block is read-only for a kernel's lifetime (constant: uniform
+// reads become scalar loads).  Host builds see plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FQSL_GLOBAL __attribute__((address_space(1)))
+#define FQSL_CONST __attribute__((address_space(4)))
+#else
+#define FQSL_GLOBAL
+#define FQSL_CONST
+#endif
+
 namespace fqsl {
 
 // ---------------------------------------------------------------- constants
@@ -129,9 +141,9 @@ struct FieldSoA {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const long o = base + (q & 1) + ((q >> 1) & 1) * nx + ((q >> 2) & 1) * nxy;
-            c[0][q] = (R)bx[o];
-            c[1][q] = (R)by[o];
-            c[2][q] = (R)bz[o];
+            c[0][q] = (R)((const FQSL_GLOBAL T*)bx)[o];
+            c[1][q] = (R)((const FQSL_GLOBAL T*)by)[o];
+            c[2][q] = (R)((const FQSL_GLOBAL T*)bz)[o];
         }
     }
 };
@@ -165,7 +177,7 @@ struct FieldCell32 {
     template <typename R>
     FQSL_HD void load(int i, int j, int k, R c[3][8]) const {
         const long cx = n[0] - 1, cxy = (long)(n[0] - 1) * (n[1] - 1);
-        const float4* p = cells + 8 * ((long)k * cxy + (long)j * cx + i);
+        const FQSL_GLOBAL float4* p = (const FQSL_GLOBAL float4*)(cells + 8 * ((long)k * cxy + (long)j * cx + i));
 #pragma unroll
         for (int comp = 0; comp < 3; ++comp) {
             const float4 lo = p[2 * comp], hi = p[2 * comp + 1];

@@ -74,7 +74,8 @@ def save_cache(data: dict, path=None) -> None:
 
 
 def current() -> dict:
-    return {k: _lib.tune_get(k) for k in _lib.TUNE_KNOBS}
+    """The Scott tracer's knobs (the ones the autotuner searches)."""
+    return {k: _lib.tune_get(k) for k in SPACE}
 
 
 def apply(cfg: dict) -> None:
