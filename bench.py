@@ -64,15 +64,18 @@ def parse():
     return ap.parse_args()
 
 
-PROFILE = os.path.join(ROOT, "profiles", "scott_rkf45_multipole256_1024sq.md")
+PROFILES = {"plane": "profiles/scott_rkf45_multipole256_1024sq.md",
+            "volume": "profiles/scott_rkf45_rope512x256x256_volume.md",
+            "stretch": "profiles/plain_rkf45_stretched_multipole256_1024sq.md"}
+TRAFFIC_HOW = "rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated"
 
 
-def measured_traffic():
-    """HBM bytes per launch of the stepping kernel as measured with rocprofv3
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and recorded in the profile
-    summary for this workload; None if absent."""
+def measured_traffic(workload):
+    """HBM bytes per launch of the workload's dominant kernel as measured
+    with rocprofv3 (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and recorded in
+    the profile summary for the workload; None if absent."""
     try:
-        with open(PROFILE) as fh:
+        with open(os.path.join(ROOT, PROFILES[workload])) as fh:
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch"):
                     return float(line.split("=")[1])
@@ -362,6 +365,17 @@ def bench_serve(args, fq, torch, log):
         f.close()
 
 
+def calibration_copy(torch, dev, log):
+    """A 1 GiB device-to-device copy of known bytes, for scaling the HBM
+    byte counters of a profiled run."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("calibration copy: 1073741824 B read + 1073741824 B written")
+    del src, dst
+
+
 def bench_stretch(args, fq, torch, log):
     """Plain RKF45 tracing on a stretched grid (TraceBlineAdaptiveStretch.cu):
     the random multipole (default_rng(0)) scaled by 10 and sampled on a 256^3
@@ -389,6 +403,8 @@ def bench_stretch(args, fq, torch, log):
         f"{nlines} lines per step")
     kw = dict(method="rkf45", step=0.01, tol_coef=1e-2)
     outs = {}
+    if args.calibrate:
+        calibration_copy(torch, dev, log)
 
     def run(precision, schedule):
         key = (precision, schedule)
@@ -465,7 +481,10 @@ def bench_stretch(args, fq, torch, log):
                    "lines_per_step": nlines, "precision": args.precision, "schedule": args.schedule,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": (measured_traffic("stretch") if (args.precision, args.schedule, n) ==
+                                 ("fp32c", "per_line", 1024) else None),
+                     "traffic_source": "%s (%s)" % (PROFILES["stretch"], TRAFFIC_HOW),
                      "kernel": "k_plain_%s (stretched)" % args.schedule, "kernel_ms": k,
                      "evals_per_launch": evals, "bytes_per_eval": bpe},
         "cpu_baseline": cpu,
@@ -559,12 +578,7 @@ def main():
         log(f"autotuned in {tuning['seconds']:.1f} s: {cfg}")
 
     if args.calibrate:
-        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: 1073741824 B read + 1073741824 B written")
-        del src, dst
+        calibration_copy(torch, dev, log)
 
     for _ in range(args.warmup):
         run(args.precision, args.schedule)
@@ -660,10 +674,10 @@ def main():
                    "schedule": args.schedule, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": (measured_traffic() if args.precision == "fp32c" and not volume
-                                 and args.pixels == 1024 else None),
-                     "traffic_source": "profiles/scott_rkf45_multipole256_1024sq.md (rocprofv3 "
-                                       "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
+                     "traffic": (measured_traffic(args.workload) if args.precision == "fp32c"
+                                 and args.schedule == "persistent" and (volume or args.pixels == 1024)
+                                 else None),
+                     "traffic_source": "%s (%s)" % (PROFILES[args.workload], TRAFFIC_HOW),
                      "kernel": ("stepping kernel k_scott_persistent_w2" if args.schedule == "persistent"
                                 else "k_scott_per_line"),
                      "kernel_ms": kms, "trace_ms": float(np.mean(trace_ms)),
