@@ -41,12 +41,14 @@ def parse():
                     help="serve: Poisson arrival rates (requests/s) to sweep")
     ap.add_argument("--duration", type=float, default=2.0,
                     help="serve: seconds of arrivals per rate")
-    ap.add_argument("--workload", default="plane", choices=["plane", "volume", "batch", "serve", "stretch"],
+    ap.add_argument("--workload", default="plane",
+                    choices=["plane", "volume", "batch", "serve", "stretch", "neighbour"],
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
                          "of the 512x256x256 rope-arcade field; batch: 256 uneven plane requests "
                          "over 3 resident fields in one graph-captured run; serve: Poisson "
                          "arrivals of 128^2..2048^2 maps over 16 resident snapshots; stretch: "
-                         "1024^2 plain RKF45 lines on a stretched 256^3 grid")
+                         "1024^2 plain RKF45 lines on a stretched 256^3 grid; neighbour: the "
+                         "1024^2 neighbour-difference Q map (plain RKF45, QCalcPlane, pseudo lines)")
     ap.add_argument("--pixels", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch knobs instead of autotuning them")
@@ -365,6 +367,123 @@ def bench_serve(args, fq, torch, log):
         f.close()
 
 
+def bench_neighbour(args, fq, torch, log):
+    """The neighbour-difference Q map of demo/Q_RKF45CuPy.ipynb cells [6]-[11]
+    on the random multipole (default_rng(0)) 256^3 float32: plain RKF45
+    (TraceBlineAdaptive.cu) with s_len 1/8 and tol_coef 0.5 from 1024^2 seeds
+    on z = 1e-4 (a quarter-cell stride), QCalcPlane (FastQSL.py:65-117), four
+    pseudo lines per B_flag pixel traced with tol_coef 0.1 (CookPseudoLine,
+    FastQSL.py:4-57), Qcut and the merge.  One step = one full map; inputs
+    resident.  The CPU leg runs the same pipeline (FP64 C oracle lines, the
+    NumPy float32 restatements of the map steps) on the central block of the
+    seed grid."""
+    from fastqsl_amd import qmap
+    dev = torch.device("cuda", 0)
+    steps = args.steps or 10
+    warm = args.warmup if args.warmup is not None else 2
+    if args.calibrate:
+        calibration_copy(torch, dev, log)
+    bx, by, bz = fq.synth.multipole_256(seed=0, device="cuda")
+    field = fq.Field(bx, by, bz, layouts=("soa64", "cell32"))
+    n = args.pixels
+    g = np.linspace(0, 255, n, dtype=np.float32)
+    stride = 255.0 / (n - 1)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.full(X.size, 1e-4, np.float32)])
+    st = torch.as_tensor(seeds, device=dev)
+    shape = (n, n)
+    kw = dict(method="rkf45", step=0.125, tol_coef=0.5, pseudo_tol_scale=0.2,
+              precision=args.precision, schedule=args.schedule)
+
+    def qmap_once():
+        return qmap.neighbour_qmap(field, st, shape, stride, **kw)
+
+    for _ in range(warm):
+        res = qmap_once()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        res = qmap_once()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    value = n * n * steps / wall
+    npseudo = int(res["n_pseudo"])
+    # the dominant kernel: the main trace, timed alone with HIP events on its stream
+    x, y, z = (st[:, i].contiguous() for i in range(3))
+    tr, kms = None, []
+    for _ in range(max(3, steps // 2)):
+        tr = fq.trace_plain(field, x, y, z, method="rkf45", step=0.125, tol_coef=0.5,
+                            precision=args.precision, schedule=args.schedule, out=tr)
+        kms.append(fq.last_kernel_ms())
+    evals = int(tr["evals"].to(torch.int64).sum().item())
+    k = float(np.mean(kms))
+    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    achieved = evals * bpe / (k * 1e-3)
+    log(f"neighbour {args.precision}/{args.schedule}: {value:.4g} pixels/s ({wall / steps * 1e3:.2f} ms "
+        f"per map, {npseudo} pseudo pixels); main trace {k:.2f} ms, {evals / (n * n):.1f} evals/line")
+    extra = {}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        from oracle import qplane
+        side = min(n, int(round(np.sqrt(args.cpu_sample << 2))))
+        r0 = (n - side) // 2
+        blk = seeds.reshape(n, n, 3)[r0:r0 + side, r0:r0 + side].reshape(-1, 3)
+        f64 = tuple(np.asarray(b, np.float64) for b in (bx, by, bz))
+        nt = O.num_threads()
+        t3 = time.perf_counter()
+        o = O.trace_plain(f64, blk, method="rkf45", s0=0.125, tol_coef=0.5, nthreads=nt)
+        gr = lambda a: np.asarray(a, np.float32).reshape(side, side)
+        ge = lambda a, i: gr(a[:, i])
+        qin = qplane.qcalc_plane(ge(o["end"], 0), ge(o["end"], 1), ge(o["end"], 2), gr(o["flag_end"]),
+                                 ge(o["start"], 0), ge(o["start"], 1), ge(o["start"], 2), gr(o["flag_start"]),
+                                 ge(o["b_start"], 0), ge(o["b_start"], 1), ge(o["b_start"], 2),
+                                 ge(o["b_end"], 0), ge(o["b_end"], 1), ge(o["b_end"], 2),
+                                 ge(o["b_this"], 0), ge(o["b_this"], 1), ge(o["b_this"], 2),
+                                 gr(o["b_flag"]), stride)
+        ps, b0s, b0e = qplane.pseudo_seeds(o["start"][:, 0], o["start"][:, 1], o["start"][:, 2],
+                                           o["flag_start"], o["b_start"][:, 2], o["b_end"][:, 2],
+                                           o["b_flag"], stride)
+        qc = np.zeros(0, np.float32)
+        if len(ps):
+            op = O.trace_plain(f64, ps, method="rkf45", s0=0.125, tol_coef=0.1, nthreads=nt)
+            qc = qplane.qcut(op["start"], op["flag_start"], op["end"], op["flag_end"], b0s, b0e, stride)
+        qref = qplane.merge(qin, qc, gr(o["b_flag"]))
+        dt = time.perf_counter() - t3
+        cpu = {"value": side * side / dt, "unit": "pixels/s", "cores": nt, "kind": "port",
+               "sample": f"the central {side}x{side} block of the same seed grid as its own map "
+                         f"({side * side} lines + {len(ps)} pseudo lines), FP64 C oracle lines "
+                         "(OpenMP dynamic schedule) and the NumPy float32 map steps",
+               "cpu_model": cpu_model()}
+        qg = res["q_all"][r0 + 1:r0 + side - 1, r0 + 1:r0 + side - 1].cpu().numpy()
+        fin = np.isfinite(qg) & np.isfinite(qref) & (qg > 0) & (qref > 0)
+        dl = np.abs(np.log10(qg[fin]) - np.log10(qref[fin]))
+        extra["error_vs_oracle"] = {
+            "pixels": int(qref.size), "nan_match": float(np.mean(np.isnan(qg) == np.isnan(qref))),
+            "dlog10q_median": float(np.median(dl)), "frac_dlog10q_le_1e-2": float(np.mean(dl <= 1e-2)),
+            "note": "plain RKF45 step sequences fork on round-off; the map differences amplify endpoint "
+                    "differences near separatrices"}
+        log(f"cpu baseline: {side}x{side} block in {dt:.2f} s on {nt} threads")
+    rec = {
+        "metric": "Q-map pixels/s (neighbour difference, plain RKF45)",
+        "value": value, "unit": "pixels/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32c" else "f64", "data": "synthetic",
+        "config": {"workload": "random multipole (40 sources, default_rng(0)) 256^3 float32, plain RKF45 "
+                               "s_len 1/8 tol_coef 0.5, z=1e-4 plane %dx%d, QCalcPlane + pseudo lines "
+                               "(tol_coef 0.1) + Qcut" % (n, n),
+                   "pixels_per_step": n * n, "pseudo_pixels": npseudo, "precision": args.precision,
+                   "schedule": args.schedule, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_plain_%s (main trace)" % args.schedule, "kernel_ms": k,
+                     "evals_per_launch": evals, "bytes_per_eval": bpe},
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def calibration_copy(torch, dev, log):
     """A 1 GiB device-to-device copy of known bytes, for scaling the HBM
     byte counters of a profiled run."""
@@ -516,6 +635,8 @@ def main():
         return bench_serve(args, fq, torch, log)
     if args.workload == "stretch":
         return bench_stretch(args, fq, torch, log)
+    if args.workload == "neighbour":
+        return bench_neighbour(args, fq, torch, log)
     if args.workload == "batch":
         if args.steps is None:
             args.steps = 3
