@@ -1,6 +1,13 @@
 """Parity metrics shared by the GPU tests (GPU result vs FP64 oracle)."""
 import numpy as np
 
+# Seeds the reference's drivers meet at the edges of a 32^3 box: below the
+# surface, on faces and edges, on the strict y = 0 plane, outside the box,
+# NaN, at the centre (a null of the b0 = 0 hyperbolic field), just above z = 0.
+EDGE_SEEDS = np.array([[10.0, 10.0, -0.5], [0.0, 5.0, 0.0], [31.0, 12.0, 3.0], [5.0, 0.0, 2.0],
+                       [15.5, 15.5, 4.0], [np.nan, 3.0, 3.0], [40.0, 3.0, 3.0], [31.0, 31.0, 31.0],
+                       [0.0, 0.0, 0.0], [12.0, 12.0, 31.0], [3.0, 4.0, 1e-7]], np.float32)
+
 
 def log_q_diff(q_gpu, q_ref):
     """|log10 q_gpu - log10 q_ref| per line; 0 where both are NaN or both 0,

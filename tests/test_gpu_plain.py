@@ -280,3 +280,48 @@ def test_dropin_plain_kernels_match_direct_api(fq):
                              ("bf", "b_flag"), ("bsz", "b_start_z"), ("bez", "b_end_z"), ("btx", "b_this_x")):
             assert np.array_equal(outs[mine], ref[theirs], equal_nan=True), (which, mine)
         assert np.array_equal(line_len, ref["line_len"]), which
+
+
+@pytest.mark.parametrize("b0", [1.0, 0.0])
+@pytest.mark.parametrize("method", ["rk4", "rkf45"])
+def test_plain_edge_seeds(fq, oracle, b0, method):
+    """Seeds at the box's edges (below the surface, on faces and edges, on
+    the strict y = 0 plane, outside, NaN, at a null): the exit classes of
+    the oracle in both precisions and schedules (checkFlag,
+    TraceBlineAdaptive.cu:276-298; drivers :333-426, TraceBline.cu:300-378);
+    FP64 RK4 footpoints to 1e-6 cells with the oracle's evaluation counts."""
+    seeds = parity.EDGE_SEEDS
+    field = synth.hyperbolic((32, 32, 32), k=0.03, b0=b0)
+    ref = oracle.trace_plain(field, seeds, method=method, s0=0.25, tol_coef=1.0)
+    print(f"edge seeds b0 {b0} {method}", flush=True)
+    with fq.Field(*field) as f:
+        for prec in ("fp64", "fp32c"):
+            for sched in ("persistent", "per_line"):
+                r = _np(fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method=method, step=0.25,
+                                       tol_coef=1.0, precision=prec, schedule=sched))
+                np.testing.assert_array_equal(r["flag_start"], ref["flag_start"])
+                np.testing.assert_array_equal(r["flag_end"], ref["flag_end"])
+                np.testing.assert_array_equal(np.isnan(_xyz(r, "end")), np.isnan(ref["end"]))
+                if prec == "fp64" and method == "rk4":
+                    np.testing.assert_allclose(_xyz(r, "end"), ref["end"], rtol=0, atol=1e-6)
+                    np.testing.assert_allclose(_xyz(r, "start"), ref["start"], rtol=0, atol=1e-6)
+                    np.testing.assert_array_equal(r["evals"], ref["evals"])
+
+
+def test_empty_inputs(fq):
+    """No seeds: every tracer returns empty outputs without a launch."""
+    import torch
+    e = torch.empty(0, dtype=torch.float32, device="cuda")
+    field = synth.hyperbolic((16, 16, 16), k=0.03, b0=1.0)
+    with fq.Field(*field) as f:
+        for prec in ("fp64", "fp32c"):
+            for sched in ("persistent", "per_line"):
+                r = fq.trace_plain(f, e, e, e, precision=prec, schedule=sched)
+                s = fq.trace_scott(f, e, e, e, precision=prec, schedule=sched)
+                assert all(r[k].numel() == 0 for k in TRACE_KEYS)
+                assert s["q_0"].numel() == 0 and s["flag_end"].numel() == 0
+        ax = np.linspace(0.0, 15.0, 16)
+        f.set_axes(ax, ax, ax)
+        r = fq.trace_plain(f, e, e, e, step=0.01)
+        assert r["end_x"].numel() == 0
+    torch.cuda.synchronize()

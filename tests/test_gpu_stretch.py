@@ -270,3 +270,27 @@ def test_stretched_field_errors(fq):
         assert not field.stretched
         r = fq.trace_scott(field, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rk4")
         assert r["flag_end"].numel() == 1
+
+
+@pytest.mark.parametrize("b0", [1.0, 0.0])
+@pytest.mark.parametrize("rule", ["bisect", "uniform", "exact"])
+def test_edge_seeds(fq, oracle, rule, b0):
+    """Seeds at the edges of a stretched box over [0, 31]^3 (below the
+    surface, on faces and edges, the strict y = 0 plane, outside, NaN): the
+    oracle's exit classes (checkFlag, TraceBlineAdaptiveStretch.cu:298-321)
+    in FP64 on every seed and in FP32c away from the null of the b0 = 0
+    field (where B rounds to zero in float, class 42)."""
+    from tests import parity
+    seeds = parity.EDGE_SEEDS
+    ax = synth.stretched_axis(32, 0.0, 31.0, "sinh", 1.5, dtype=np.float64)
+    axes = [ax, ax, synth.stretched_axis(32, 0.0, 31.0, "geometric", 3.0, dtype=np.float64)]
+    f = synth.hyperbolic((32, 32, 32), k=0.03, b0=b0)
+    ref = oracle.trace_stretch(f, axes, seeds, s0=0.01, tol_coef=1.0, rule=rule)
+    keep = np.ones(len(seeds), bool)
+    print(f"stretched edge seeds {rule} b0 {b0}", flush=True)
+    with fq.Field(*f, axes=axes, lookup=rule) as field:
+        for precision in ("fp64", "fp32c"):
+            r = _run(fq, field, seeds, precision, step=0.01, tol_coef=1.0)
+            m = keep if (precision == "fp64" or b0 != 0.0) else keep & (np.arange(len(seeds)) != 4)
+            np.testing.assert_array_equal(r["flag_start"][m], ref["flag_start"][m])
+            np.testing.assert_array_equal(r["flag_end"][m], ref["flag_end"][m])

@@ -135,3 +135,25 @@ def test_twist_matches_the_oracle(oracle, precision, rtol):
     scale = np.maximum(np.abs(ref["twist"]), 1e-3)
     assert np.quantile(np.abs(r["twist"] - ref["twist"])[ok] / scale[ok], 0.99) < rtol
     assert np.mean(np.abs(r["twist"]) > 1e-6) > 0.5
+
+
+
+@pytest.mark.parametrize("b0", [1.0, 0.0])
+@pytest.mark.parametrize("method", ["rk4", "rkf45"])
+def test_plain_tracer_edge_seeds(oracle, b0, method):
+    """Exit classes of the plain driver at the edges (TraceBlineAdaptive.cu
+    :333-426, TraceBline.cu:300-378, checkFlag :276-298): identical to the
+    oracle in both precisions; in FP64 the same evaluation counts and
+    footpoints to round-off."""
+    f = synth.hyperbolic((32, 32, 32), k=0.03, b0=b0)
+    ref = oracle.trace_plain(f, parity.EDGE_SEEDS, method=method, s0=0.25, tol_coef=1.0)
+    for precision in (1, 0):
+        r = hc.trace_plain(f, parity.EDGE_SEEDS, precision=precision, method={"rk4": 1, "rkf45": 0}[method],
+                           s0=0.25, tol_coef=1.0)
+        np.testing.assert_array_equal(r["flag_start"], ref["flag_start"])
+        np.testing.assert_array_equal(r["flag_end"], ref["flag_end"])
+        np.testing.assert_array_equal(np.isnan(r["end"]), np.isnan(ref["end"]))
+        if precision == 1:
+            np.testing.assert_allclose(r["end"], ref["end"], rtol=0, atol=1e-7)
+            np.testing.assert_allclose(r["start"], ref["start"], rtol=0, atol=1e-7)
+            np.testing.assert_array_equal(r["evals"], ref["evals"])
