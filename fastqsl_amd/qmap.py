@@ -8,6 +8,9 @@
 * ``mixed_qmap``     -- ``neighbour_qmap`` with the pixels that have B_flag set
   or Q = NaN traced again by the Scott method
   (demo/Q_RKF45MixCuPyDiffScott.ipynb cells [7]-[9]).
+* ``neighbour_qmaps`` -- several ``neighbour_qmap`` jobs over HIP streams, so
+  one map's pseudo-line trace (a few long lines that leave most of the GPU
+  idle) runs beside the next map's main trace.
 
 Seed grids use the notebooks' conventions (``linspace`` x ``meshgrid``, rows
 = the second in-plane axis), see ``synth.plane_seeds_z`` / ``plane_seeds_abc``.
@@ -109,6 +112,57 @@ def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), m
                      rc["end_x"], rc["end_y"], rc["end_z"], rc["flag_end"],
                      c["bz0_start"], c["bz0_end"], ps, Q_all=q_all, pixel=c["pixel"])
     return dict(q=q_all[1:-1, 1:-1], q_all=q_all, q_inner=q_inner, traces=g, n_pseudo=m)
+
+
+def neighbour_qmaps(field: Field, jobs, streams=2, **kw):
+    """``neighbour_qmap`` for each (seeds, shape, stride) of ``jobs``, the
+    jobs dealt round-robin to ``streams`` host threads, each issuing its
+    jobs in order on its own HIP stream (default priority).  Keyword
+    arguments go to every ``neighbour_qmap``.  Returns the results in job
+    order; they are complete when this returns (the streams are
+    synchronised)."""
+    import threading
+    import torch
+    jobs = list(jobs)
+    out = [None] * len(jobs)
+    errors = []
+    nstreams = max(1, min(int(streams), len(jobs)))
+    pool = [torch.cuda.Stream() for _ in range(nstreams)]
+    base = torch.cuda.current_stream()
+    for s in pool:
+        s.wait_stream(base)   # inputs written on the caller's stream are visible
+
+    def worker(k):
+        try:
+            with torch.cuda.stream(pool[k]):
+                for j in range(k, len(jobs), nstreams):
+                    seeds, shape, stride = jobs[j]
+                    out[j] = neighbour_qmap(field, seeds, shape, stride, **kw)
+                pool[k].synchronize()
+        except BaseException as e:  # re-raised on the calling thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(nstreams)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    for s in pool:
+        base.wait_stream(s)
+
+    def keep_for(v):   # results allocated on a pool stream, used on the caller's
+        if isinstance(v, torch.Tensor) and v.is_cuda:
+            v.record_stream(base)
+        elif isinstance(v, dict):
+            for u in v.values():
+                keep_for(u)
+        elif isinstance(v, (tuple, list)):
+            for u in v:
+                keep_for(u)
+    keep_for(out)
+    return out
 
 
 def mixed_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), method="rkf45", step=0.125,

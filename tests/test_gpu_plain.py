@@ -325,3 +325,26 @@ def test_empty_inputs(fq):
         r = fq.trace_plain(f, e, e, e, step=0.01)
         assert r["end_x"].numel() == 0
     torch.cuda.synchronize()
+
+
+def test_neighbour_qmaps_over_streams_match_serial(fq):
+    """Maps run as one job over two streams (qmap.neighbour_qmaps) equal the
+    same maps run one at a time, bit for bit (a line's result does not
+    depend on the launch or stream that traces it)."""
+    import torch
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=48)
+    jobs = []
+    for (x0, x1), z in (((2, 45), 1e-4), ((5, 40), 0.5), ((1, 46), 1e-4)):
+        seeds, shape = synth.plane_seeds_z(72, 60, (x0, x1), (x0, x1), z=z)
+        jobs.append((seeds, shape, (x1 - x0) / 71.0))
+    kw = dict(method="rkf45", step=0.125, tol_coef=0.5, pseudo_tol_scale=0.2)
+    with fq.Field(*field) as f:
+        serial = [fq.qmap.neighbour_qmap(f, s, sh, st, **kw) for s, sh, st in jobs]
+        torch.cuda.synchronize()
+        piped = fq.qmap.neighbour_qmaps(f, jobs, streams=2, **kw)
+        torch.cuda.synchronize()
+        for a, b in zip(serial, piped):
+            assert a["n_pseudo"] == b["n_pseudo"]
+            np.testing.assert_array_equal(a["q_all"].cpu().numpy(), b["q_all"].cpu().numpy())
+            np.testing.assert_array_equal(a["traces"]["end_x"].cpu().numpy(), b["traces"]["end_x"].cpu().numpy())
+    assert any(a["n_pseudo"] > 0 for a in serial)
