@@ -1009,8 +1009,12 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
     }
 }
 
+// One thread per line.  Two waves per SIMD as the register target: the FP64
+// variants otherwise take 256 VGPRs plus AGPRs and run one wave per SIMD
+// (1.2-1.3x slower); the FP32 variants fit two waves either way.
 template <typename R, class F, class FJ, class G>
-__global__ void __launch_bounds__(256) k_plain_per_line(F f, PlainParams P, Seeds s, unsigned long long n,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_plain_per_line(F f, PlainParams P, Seeds s, unsigned long long n,
                                                         PlainOut out, DevScalars ds, const float* ncross, FJ fj,
                                                         G g) {
     apply_plain_scalars(f, P, n, ds, ncross);
