@@ -42,13 +42,14 @@ def parse():
     ap.add_argument("--duration", type=float, default=2.0,
                     help="serve: seconds of arrivals per rate")
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "volume", "batch", "serve", "stretch", "neighbour"],
+                    choices=["plane", "volume", "batch", "serve", "stretch", "neighbour", "twist"],
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
                          "of the 512x256x256 rope-arcade field; batch: 256 uneven plane requests "
                          "over 3 resident fields in one graph-captured run; serve: Poisson "
                          "arrivals of 128^2..2048^2 maps over 16 resident snapshots; stretch: "
                          "1024^2 plain RKF45 lines on a stretched 256^3 grid; neighbour: the "
-                         "1024^2 neighbour-difference Q map (plain RKF45, QCalcPlane, pseudo lines)")
+                         "1024^2 neighbour-difference Q map (plain RKF45, QCalcPlane, pseudo lines); "
+                         "twist: the default map with the twist number (J = index-space curl B)")
     ap.add_argument("--pixels", type=int, default=1024)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch knobs instead of autotuning them")
@@ -500,6 +501,114 @@ def bench_neighbour(args, fq, torch, log):
     print(json.dumps(rec), flush=True)
 
 
+def bench_twist(args, fq, torch, log):
+    """Scott Q with the twist number (TraceBlineScott.cu:423-457) on the
+    random multipole (default_rng(0)) 256^3 float32, RKF45 tol 1e-6, 1024^2
+    seeds on z = 0; J from the twist notebooks' index-space curl
+    (postproc.curl_index, demo/Q_RK4.ipynb cell [11]) as a second resident
+    field.  One step = one full map with twist; the same map without twist
+    is timed beside it, and the FP64 oracle (with J) runs on a sample of the
+    seeds."""
+    from fastqsl_amd import postproc
+    from fastqsl_amd.trace import SCOTT_OUTPUTS
+    dev = torch.device("cuda", 0)
+    steps = args.steps or 10
+    warm = args.warmup if args.warmup is not None else 2
+    if args.calibrate:
+        calibration_copy(torch, dev, log)
+    bx, by, bz = fq.synth.multipole_256(seed=0, device="cuda")
+    jx, jy, jz = postproc.curl_index(bx, by, bz)
+    field = fq.Field(bx, by, bz, layouts=("soa64", "cell32"), current=(jx, jy, jz))
+    n = args.pixels
+    g = np.linspace(0, 255, n, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size, np.float32)])
+    sx, sy, sz = (torch.as_tensor(np.ascontiguousarray(seeds[:, i]), device=dev) for i in range(3))
+    nlines = seeds.shape[0]
+    persistent = args.schedule == "persistent"
+    outs = {}
+
+    def run(twist):
+        o = SCOTT_OUTPUTS + (("twist",) if twist else ())
+        outs[twist] = fq.trace_scott(field, sx, sy, sz, method="rkf45", step=0.25, tol=1e-6,
+                                     precision=args.precision, schedule=args.schedule, outputs=o,
+                                     out=outs.get(twist))
+        return outs[twist]
+
+    def timed(twist):
+        for _ in range(warm):
+            run(twist)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            run(twist)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        kms = []
+        for _ in range(max(3, steps // 2)):
+            run(twist)
+            kms.append(fq._lib.last_step_ms() if persistent else fq.last_kernel_ms())
+        return wall, float(np.mean(kms))
+
+    wall, k = timed(True)
+    wall0, k0 = timed(False)
+    out = outs[True]
+    evals = int(out["evals"].to(torch.int64).sum().item())
+    value = nlines * steps / wall
+    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    achieved = evals * bpe / (k * 1e-3)
+    log(f"twist {args.precision}/{args.schedule}: {value:.4g} lines/s ({wall / steps * 1e3:.2f} ms per map; "
+        f"without twist {wall0 / steps * 1e3:.2f} ms), stepping kernel {k:.2f} ms (without twist {k0:.2f})")
+    extra = {"without_twist": {"lines_per_s": nlines * steps / wall0, "ms_per_map": wall0 / steps * 1e3,
+                               "kernel_ms": k0}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        rng = np.random.default_rng(1)
+        idx = np.sort(rng.choice(nlines, size=min(args.cpu_sample, nlines), replace=False))
+        f64 = tuple(np.asarray(b.cpu().numpy() if hasattr(b, "cpu") else b, np.float64) for b in (bx, by, bz))
+        j64 = tuple(np.asarray(b.cpu().numpy() if hasattr(b, "cpu") else b, np.float64) for b in (jx, jy, jz))
+        nt = O.num_threads()
+        t3 = time.perf_counter()
+        ref = O.trace_scott(f64, seeds[idx], method="rkf45", s0=0.25, tol=1e-6, nthreads=nt, current=j64)
+        dt = time.perf_counter() - t3
+        cpu = {"value": len(idx) / dt, "unit": "lines/s", "cores": nt, "kind": "port",
+               "sample": f"{len(idx)} seeds drawn uniformly (default_rng(1)) from the same {nlines} seeds, "
+                         "FP64 C oracle with the same J, OpenMP dynamic schedule", "cpu_model": cpu_model()}
+        tw = out["twist"][idx].cpu().numpy()
+        cls = (out["flag_start"][idx].cpu().numpy() == ref["flag_start"]) & \
+              (out["flag_end"][idx].cpu().numpy() == ref["flag_end"])
+        d = np.abs(tw - ref["twist"])[cls]
+        scale = np.abs(ref["twist"])[cls]
+        extra["error_vs_oracle"] = {
+            "lines": int(len(idx)), "class_match": float(cls.mean()),
+            "twist_abs_err_median": float(np.median(d)),
+            "frac_twist_rel_err_le_1e-3": float(np.mean(d <= 1e-3 * np.maximum(scale, 1e-3))),
+            "note": "RKF45 step sequences fork on round-off; twist integrates J.B/|B|^2/4pi along them"}
+        log(f"cpu baseline: {len(idx)} lines in {dt:.2f} s on {nt} threads")
+    rec = {
+        "metric": "Scott-Q field lines/s with twist",
+        "value": value, "unit": "lines/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32c" else "f64", "data": "synthetic",
+        "config": {"workload": "random multipole (40 sources, default_rng(0)) 256^3 float32 with J = index-space "
+                               "curl B (current-free up to the discretisation, so twists are small), Scott Q + "
+                               "twist, RKF45 tol 1e-6, z=0 plane %dx%d" % (n, n),
+                   "lines_per_step": nlines, "precision": args.precision, "schedule": args.schedule,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": ("stepping kernel k_scott_persistent_w2 (twist)" if persistent
+                                else "k_scott_per_line (twist)"),
+                     "kernel_ms": k, "evals_per_launch": evals, "bytes_per_eval": bpe,
+                     "note": "stage evaluations only; the two J and B interpolations per accepted step "
+                             "for the twist are not counted"},
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def calibration_copy(torch, dev, log):
     """A 1 GiB device-to-device copy of known bytes, for scaling the HBM
     byte counters of a profiled run."""
@@ -653,6 +762,8 @@ def main():
         return bench_stretch(args, fq, torch, log)
     if args.workload == "neighbour":
         return bench_neighbour(args, fq, torch, log)
+    if args.workload == "twist":
+        return bench_twist(args, fq, torch, log)
     if args.workload == "batch":
         if args.steps is None:
             args.steps = 3

@@ -61,6 +61,7 @@ struct fqsl_field {
     float4* cell32;
     float* cur32[3];   // current density J (twist), float and double SoA
     double* cur64[3];
+    float4* curcell;   // J in the cell layout (with a cell32 field): the FP32 twist reads it
     size_t bytes;
     // stretched grid: node coordinates per axis and their lookup tables
     void* axes_mem;
@@ -120,6 +121,7 @@ static void field_free(fqsl_field* f) {
         if (f->cur64[q]) (void)hipFree(f->cur64[q]);
     }
     if (f->cell32) (void)hipFree(f->cell32);
+    if (f->curcell) (void)hipFree(f->curcell);
     if (f->axes_mem) (void)hipFree(f->axes_mem);
     delete f;
 }
@@ -219,10 +221,12 @@ extern "C" int fqsl_field_set_current(fqsl_field* f, const void* jx, const void*
     const long nn = (long)f->n[0] * f->n[1] * f->n[2];
     const size_t esz = dtype == FQSL_DTYPE_F64 ? 8 : 4;
     const void* src[3] = {jx, jy, jz};
+    const long ncell = (long)(f->n[0] - 1) * (f->n[1] - 1) * (f->n[2] - 1);
     for (int q = 0; q < 3; ++q) {
         if (f->cur32[q]) { (void)hipFree(f->cur32[q]); f->cur32[q] = nullptr; f->bytes -= nn * 4; }
         if (f->cur64[q]) { (void)hipFree(f->cur64[q]); f->cur64[q] = nullptr; f->bytes -= nn * 8; }
     }
+    if (f->curcell) { (void)hipFree(f->curcell); f->curcell = nullptr; f->bytes -= ncell * 128; }
     for (int q = 0; q < 3; ++q) {
         void* staged = const_cast<void*>(src[q]);
         if (!on_device) {
@@ -249,6 +253,16 @@ extern "C" int fqsl_field_set_current(fqsl_field* f, const void* jx, const void*
         if (!on_device) (void)hipFree(staged);
         if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "current layout: %s", hipGetErrorString(e));
         f->bytes += nn * 12;
+    }
+    if (f->cell32) {  // the FP32 tracers read B from cells; J the same way (6 vector loads, not 24 gathers)
+        hipError_t e = hipMalloc(&f->curcell, ncell * 128);
+        if (e != hipSuccess) return set_err(FQSL_ERR_NOMEM, "hipMalloc(current cells): %s", hipGetErrorString(e));
+        k_build_cells<<<grid_for(ncell, 256), 256>>>(f->cur32[0], f->cur32[1], f->cur32[2], f->n[0], f->n[1], f->n[2],
+                                                     f->curcell);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "current cells: %s", hipGetErrorString(e));
+        f->bytes += ncell * 128;
     }
     return FQSL_OK;
 }
@@ -1220,7 +1234,11 @@ extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params
     const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
     if (field->cell32) {
         FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, fj);
+        if (o.tw) {
+            if (!field->curcell) return set_err(FQSL_ERR_INVALID, "current density has no cell layout");
+            const FieldCell32 fjc{field->curcell, {nn[0], nn[1], nn[2]}};
+            return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, fjc);
+        }
         return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
     }
     if (field->soa32[0]) {
@@ -1853,7 +1871,11 @@ static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, 
     const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
     if (field->cell32) {
         FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, g);
+        if (o.tw) {
+            if (!field->curcell) return set_err(FQSL_ERR_INVALID, "current density has no cell layout");
+            const FieldCell32 fjc{field->curcell, {nn[0], nn[1], nn[2]}};
+            return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fjc, g);
+        }
         return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
     }
     if (field->soa32[0]) {
@@ -1900,7 +1922,11 @@ extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params
     const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
     if (field->cell32) {
         FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n, fj);
+        if (o.tw) {
+            if (!field->curcell) return set_err(FQSL_ERR_INVALID, "current density has no cell layout");
+            const FieldCell32 fjc{field->curcell, {nn[0], nn[1], nn[2]}};
+            return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n, fjc);
+        }
         return launch_plain<float>(f, P, s, n, o, ds, nullptr, p->schedule, st, n);
     }
     if (field->soa32[0]) {
