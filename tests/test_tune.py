@@ -40,3 +40,20 @@ def test_apply_sets_the_library_knobs():
     finally:
         tune.apply(before)
     assert tune.current() == before
+
+
+def test_knob_ranges_are_checked():
+    """The C ABI rejects out-of-range knob values with a message and keeps the
+    old value (host code only: no GPU needed)."""
+    from fastqsl_amd import _lib
+    before = _lib.tune_get("plain_refill")
+    for knob, bad in (("plain_refill", 0), ("plain_refill", 65), ("refill_min", 0), ("queues", 3),
+                      ("scott_waves", 1), ("order", 3), ("order_buckets", 3), ("grid_pct", 0)):
+        with pytest.raises(_lib.FqslError):
+            _lib.tune_set(knob, bad)
+    assert _lib.tune_get("plain_refill") == before
+    _lib.tune_set("plain_refill", 16)
+    try:
+        assert _lib.tune_get("plain_refill") == 16
+    finally:
+        _lib.tune_set("plain_refill", before)
