@@ -742,7 +742,7 @@ def bench_stretch(args, fq, torch, log):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(lines are independent, so a multi-GPU run would shard the seeds over one process per GPU)",
               file=sys.stderr)
         sys.exit(2)

@@ -36,7 +36,7 @@ extern "C" {
 #define FQSL_ERR_INVALID 1     /* bad argument */
 #define FQSL_ERR_HIP 2         /* a HIP runtime call failed */
 #define FQSL_ERR_NOMEM 3       /* device allocation failed */
-#define FQSL_ERR_UNSUPPORTED 4 /* valid request this build does not implement */
+#define FQSL_ERR_UNSUPPORTED 4 /* valid request the library does not implement */
 
 #define FQSL_DTYPE_F32 0
 #define FQSL_DTYPE_F64 1
