@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -379,7 +380,8 @@ extern "C" int fqsl_field_set_axes(fqsl_field* f, const void* xa, const void* ya
 extern "C" int fqsl_field_stretched(const fqsl_field* f) { return f && f->stretched ? 1 : 0; }
 
 // ------------------------------------------------------------- tuning knobs
-static int g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 32};
+// Read by every launch, written by fqsl_tune_set from any host thread.
+static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 32};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
@@ -1095,7 +1097,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         unsigned long long* q = nullptr;
         int rc = queue_slot(&q);
         if (rc) return rc;
-        const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES] : 0;
+        const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES].load() : 0;
         auto kern = waves == 2 ? k_scott_persistent_w2<R, F, FJ>
                   : waves == 3 ? k_scott_persistent_w3<R, F, FJ>
                                : k_scott_persistent<R, F, FJ>;
