@@ -381,7 +381,7 @@ extern "C" int fqsl_field_stretched(const fqsl_field* f) { return f && f->stretc
 
 // ------------------------------------------------------------- tuning knobs
 // Read by every launch, written by fqsl_tune_set from any host thread.
-static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 32};
+static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 64};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);

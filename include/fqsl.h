@@ -346,7 +346,8 @@ int fqsl_TraceAllBlineStretch(float *Bx, float *By, float *Bz, int *BshapeN, flo
                                    persistent lane (ordering changes timing, never results) */
 #define FQSL_TUNE_ORDER_BUCKETS 5 /* length buckets of that order: power of two in [2, 256], default 64 */
 #define FQSL_TUNE_PLAIN_REFILL 6  /* idle lanes a wavefront of the persistent plain tracer gathers before it
-                                     refills them (1..64, default 1) */
+                                     refills them (1..64, default 64: a wavefront claims new lines
+                                     once all its lanes are idle) */
 #define FQSL_TUNE_COUNT 7
 int fqsl_tune_set(int knob, int value);
 int fqsl_tune_get(int knob, int *value);
