@@ -582,17 +582,19 @@ __device__ __forceinline__ bool claim(unsigned long long* queues, int nq, unsign
 // Lines take 10 to 10^4 steps.  A persistent launch that hands them out in
 // seed order ends with a tail of long lines running in nearly empty
 // wavefronts (about a third of the launch on the 1024^2 multipole map).  A
-// coarse estimate of each line's length -- a plain RK4 trace of b with 4-cell
-// steps, at most kPredSteps per direction -- sorts the queue longest first
-// (counting sort into 2..256 length buckets; within a bucket lines keep
-// roughly their seed order, so fewer buckets keep more of the field locality
-// of neighbouring seeds), which removes most of that tail for ~3 % more work.
+// coarse estimate of each line's length -- a plain RK4 trace of b with 8-cell
+// steps, at most kPredSteps per direction, so lines up to 256 cells are told
+// apart -- sorts the queue longest first (counting sort into 2..256 length
+// buckets; within a bucket lines keep roughly their seed order, so fewer
+// buckets keep more of the field locality of neighbouring seeds), which
+// removes most of that tail for ~3 % more work.  (4-cell steps, twice as
+// many, ordered no better and cost 0.25 ms more per 1024^2 map.)
 // With many lines per lane the tail is short anyway and the lost locality
 // costs more than it saves, so by default lines are only ordered up to 64
 // lines per persistent lane.  The order only decides which lane traces a line
 // and when; the line's result is the same.
-constexpr int kPredSteps = 32;
-constexpr float kPredStep = 4.0f;
+constexpr int kPredSteps = 16;
+constexpr float kPredStep = 8.0f;
 constexpr int kOrderBuckets = 256;
 
 template <typename R>
