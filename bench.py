@@ -423,8 +423,10 @@ def bench_neighbour(args, fq, torch, log):
     value = n * n * steps / wall
     same = all(bool(torch.equal(p["q_all"].nan_to_num(-1.0), res["q_all"].nan_to_num(-1.0))) for p in piped)
     del piped
-    # the dominant kernel: the main trace, timed alone with HIP events on its stream
-    x, y, z = (st[:, i].contiguous() for i in range(3))
+    # the dominant kernel: the main trace in the pipeline's seed order (16 x 16
+    # pixel blocks), timed alone with HIP events on its stream
+    perm = qmap._tile_order(shape, 16)
+    x, y, z = (st[:, i][perm].contiguous() for i in range(3))
     tr, kms = None, []
     for _ in range(max(3, steps // 2)):
         tr = fq.trace_plain(field, x, y, z, method="rkf45", step=0.125, tol_coef=0.5,

@@ -18,6 +18,8 @@ Everything stays on the GPU; results are torch CUDA tensors.
 """
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 
 from . import FastQSL, postproc
@@ -72,22 +74,62 @@ def scott_qcube(field: Field, x_axis, y_axis, z_axis, method="rkf45", step=0.25,
     return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
 
 
+@functools.lru_cache(maxsize=16)
+def _tile_order_host(shape, tile):
+    """Pixel indices of a (rows, cols) grid in tile x tile blocks (block
+    rows, then blocks, then rows and columns inside a block)."""
+    rows, cols = shape
+    r = np.arange(rows).reshape(-1, 1)
+    c = np.arange(cols).reshape(1, -1)
+    ntc = -(-cols // tile)
+    key = ((r // tile) * ntc + c // tile) * (tile * tile) + (r % tile) * tile + (c % tile)
+    return np.argsort(key.reshape(-1), kind="stable")
+
+
+_tile_order_dev = {}
+
+
+def _tile_order(shape, tile):
+    import torch
+    key = (tuple(int(v) for v in shape), int(tile), torch.cuda.current_device())
+    if key not in _tile_order_dev:
+        t = torch.as_tensor(_tile_order_host(key[0], key[1]), device="cuda")
+        torch.cuda.current_stream().synchronize()  # complete before another stream or thread reads it
+        _tile_order_dev[key] = t
+    return _tile_order_dev[key]
+
+
 def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), method="rkf45",
                    step=0.125, tol_coef=1.0, pseudo_stride=None, pseudo_tol_scale=0.1,
-                   pseudo_normal=None, precision="fp32c", schedule="persistent"):
+                   pseudo_normal=None, precision="fp32c", schedule="persistent", tile=16):
     """Neighbour-difference Q (QCalcPlane) with the pseudo-line fix-up for
     pixels where B is nearly tangent to the plane (B_flag == 1).
 
     ``stride`` is the pixel spacing in cells (``stride_step`` of the
     notebooks); ``pseudo_stride`` the offset of the four pseudo seeds (default
     ``stride``); the pseudo lines are traced with ``tol_coef *
-    pseudo_tol_scale`` and ``pseudo_normal`` (default ``normal``).  Returns a
-    dict with ``q`` = Q_final of shape (rows-2, cols-2), ``q_all`` (rows,
-    cols), the per-pixel traces and the number of pseudo pixels."""
+    pseudo_tol_scale`` and ``pseudo_normal`` (default ``normal``).  The main
+    lines are traced in ``tile`` x ``tile`` pixel blocks (0: row by row), so
+    a wavefront's lines start close together and share cached cells; results
+    do not depend on the order.  Returns a dict with ``q`` = Q_final of
+    shape (rows-2, cols-2), ``q_all`` (rows, cols), the per-pixel traces and
+    the number of pseudo pixels."""
     import torch
     x, y, z = _grid(seeds, shape)
+    perm = _tile_order(shape, tile) if tile and tile > 1 else None
+    if perm is not None:
+        x, y, z = x[perm], y[perm], z[perm]
     r = trace_plain(field, x, y, z, ncross=normal, method=method, step=step, tol_coef=tol_coef,
                     precision=precision, schedule=schedule)
+    if perm is not None:  # back to pixel order
+        unp = {}
+        for k, v in r.items():
+            if k == "_seeds":
+                continue
+            u = torch.empty_like(v)
+            u[perm] = v
+            unp[k] = u
+        r = unp
     g = {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
     q_inner = FastQSL.QCalcPlane(g["end_x"], g["end_y"], g["end_z"], g["flag_end"],
                                  g["start_x"], g["start_y"], g["start_z"], g["flag_start"],

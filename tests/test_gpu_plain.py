@@ -348,3 +348,19 @@ def test_neighbour_qmaps_over_streams_match_serial(fq):
             np.testing.assert_array_equal(a["q_all"].cpu().numpy(), b["q_all"].cpu().numpy())
             np.testing.assert_array_equal(a["traces"]["end_x"].cpu().numpy(), b["traces"]["end_x"].cpu().numpy())
     assert any(a["n_pseudo"] > 0 for a in serial)
+
+
+def test_neighbour_qmap_tile_order_does_not_change_results(fq):
+    """Tracing the main lines in 16 x 16 pixel blocks (the default) or row by
+    row gives the same map and traces, bit for bit."""
+    import torch
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=48)
+    seeds, shape = synth.plane_seeds_z(70, 52, (2, 45), (3, 44), z=1e-4)   # not a multiple of the tile
+    kw = dict(method="rkf45", step=0.125, tol_coef=0.5, pseudo_tol_scale=0.2)
+    with fq.Field(*field) as f:
+        a = fq.qmap.neighbour_qmap(f, seeds, shape, 43.0 / 69, tile=0, **kw)
+        b = fq.qmap.neighbour_qmap(f, seeds, shape, 43.0 / 69, **kw)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a["q_all"].cpu().numpy(), b["q_all"].cpu().numpy())
+        for k in ("end_x", "start_y", "flag_end", "b_flag", "b_end_z", "line_len"):
+            np.testing.assert_array_equal(a["traces"][k].cpu().numpy(), b["traces"][k].cpu().numpy())
