@@ -425,7 +425,7 @@ def bench_neighbour(args, fq, torch, log):
     del piped
     # the dominant kernel: the main trace in the pipeline's seed order (16 x 16
     # pixel blocks), timed alone with HIP events on its stream
-    perm = qmap._tile_order(shape, 16)
+    perm = qmap.block_order(shape, (16, 16))
     x, y, z = (st[:, i][perm].contiguous() for i in range(3))
     tr, kms = None, []
     for _ in range(max(3, steps // 2)):
@@ -783,13 +783,18 @@ def main():
         from fastqsl_amd import qmap
         shape = bx.shape
         sx, sy, sz, _ = qmap.cube_seeds(*(np.arange(m, dtype=np.float32) for m in shape))
+        # traced in 4^3 bricks of voxels, as qmap.scott_qcube does
+        brick = qmap.block_order(shape, (4, 4, 4))
+        sx, sy, sz = sx[brick].contiguous(), sy[brick].contiguous(), sz[brick].contiguous()
+        brick_host = brick.cpu().numpy()
         nx_, ny_, nz_ = shape
 
-        def seeds_at(idx):   # cube_seeds order: x slowest, z fastest
-            return np.column_stack([idx // (ny_ * nz_), (idx // nz_) % ny_, idx % nz_]).astype(np.float32)
+        def seeds_at(idx):   # line idx traces voxel brick[idx] (cube order: x slowest, z fastest)
+            v = brick_host[idx]
+            return np.column_stack([v // (ny_ * nz_), (v // nz_) % ny_, v % nz_]).astype(np.float32)
         nlines = int(sx.numel())
         workload = ("Gold-Hoyle rope (1.5 turns, r 0.2, axis z 0.4) + 2 bipoles, 512x256x256 "
-                    "float32, 3-D Scott Q on every grid node, RKF45 tol 1e-6")
+                    "float32, 3-D Scott Q on every grid node (traced in 4^3 bricks), RKF45 tol 1e-6")
     else:
         bx, by, bz = fq.synth.multipole_256(seed=0, device="cuda")
         n = args.pixels

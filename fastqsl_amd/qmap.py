@@ -64,39 +64,70 @@ def cube_seeds(x_axis, y_axis, z_axis):
 
 
 def scott_qcube(field: Field, x_axis, y_axis, z_axis, method="rkf45", step=0.25, tol=1e-6,
-                precision="fp32c", schedule="persistent", outputs=CUBE_OUTPUTS):
+                precision="fp32c", schedule="persistent", outputs=CUBE_OUTPUTS, brick=4):
     """3-D Scott Q: one line per sample of the box x_axis x y_axis x z_axis
-    (grid-index coordinates), all voxels in one launch.  Returns a dict of
-    (nx, ny, nz) tensors (``Qube`` of the reference's 3-D notebooks)."""
+    (grid-index coordinates), all voxels in one launch, traced in ``brick``^3
+    blocks of samples (0: x slowest, z fastest; on the 512x256x256 rope
+    arcade 4^3 bricks trace 5 % faster).  Returns a dict of (nx, ny, nz)
+    tensors (``Qube`` of the reference's 3-D notebooks)."""
     x, y, z, shape = cube_seeds(x_axis, y_axis, z_axis)
+    perm = block_order(shape, (brick,) * 3) if brick and brick > 1 else None
+    if perm is not None:
+        x, y, z = x[perm], y[perm], z[perm]
     r = trace_scott(field, x, y, z, method=method, step=step, tol=tol, precision=precision,
                     schedule=schedule, outputs=outputs)
+    if perm is not None:
+        r = _unpermute(r, perm)
     return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
 
 
 @functools.lru_cache(maxsize=16)
-def _tile_order_host(shape, tile):
-    """Pixel indices of a (rows, cols) grid in tile x tile blocks (block
-    rows, then blocks, then rows and columns inside a block)."""
-    rows, cols = shape
-    r = np.arange(rows).reshape(-1, 1)
-    c = np.arange(cols).reshape(1, -1)
-    ntc = -(-cols // tile)
-    key = ((r // tile) * ntc + c // tile) * (tile * tile) + (r % tile) * tile + (c % tile)
+def _block_order_host(shape, block):
+    """Flat indices of a C-ordered grid of ``shape`` visited block by block
+    (blocks in C order, C order inside a block; partial blocks at the far
+    edges)."""
+    shape, block = tuple(int(v) for v in shape), tuple(int(v) for v in block)
+    nd = len(shape)
+    if all(s % b == 0 for s, b in zip(shape, block)):
+        idx = np.arange(int(np.prod(shape))).reshape([v for s, b in zip(shape, block) for v in (s // b, b)])
+        return idx.transpose(list(range(0, 2 * nd, 2)) + list(range(1, 2 * nd, 2))).reshape(-1).copy()
+    grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+    key = np.zeros(shape, np.int64)
+    for g, s, b in zip(grids, shape, block):  # block index
+        key = key * (-(-s // b)) + g // b
+    for g, b in zip(grids, block):            # position inside the block
+        key = key * b + g % b
     return np.argsort(key.reshape(-1), kind="stable")
 
 
-_tile_order_dev = {}
+_block_order_dev = {}
 
 
-def _tile_order(shape, tile):
+def block_order(shape, block):
+    """Device tensor of ``_block_order_host`` (cached per shape, block and
+    device): tracing seeds in this order starts a wavefront's lines close
+    together, so they share cached cells; a line's result does not depend on
+    the order."""
     import torch
-    key = (tuple(int(v) for v in shape), int(tile), torch.cuda.current_device())
-    if key not in _tile_order_dev:
-        t = torch.as_tensor(_tile_order_host(key[0], key[1]), device="cuda")
+    key = (tuple(int(v) for v in shape), tuple(int(v) for v in block), torch.cuda.current_device())
+    if key not in _block_order_dev:
+        t = torch.as_tensor(_block_order_host(key[0], key[1]), device="cuda")
         torch.cuda.current_stream().synchronize()  # complete before another stream or thread reads it
-        _tile_order_dev[key] = t
-    return _tile_order_dev[key]
+        _block_order_dev[key] = t
+    return _block_order_dev[key]
+
+
+def _unpermute(r, perm):
+    """Per-line outputs traced in ``perm`` order, back in seed order."""
+    import torch
+    out = {}
+    for k, v in r.items():
+        if k == "_seeds":
+            continue
+        u = torch.empty_like(v)
+        u[perm] = v
+        out[k] = u
+    return out
 
 
 def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), method="rkf45",
@@ -116,20 +147,13 @@ def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), m
     the number of pseudo pixels."""
     import torch
     x, y, z = _grid(seeds, shape)
-    perm = _tile_order(shape, tile) if tile and tile > 1 else None
+    perm = block_order(shape, (tile, tile)) if tile and tile > 1 else None
     if perm is not None:
         x, y, z = x[perm], y[perm], z[perm]
     r = trace_plain(field, x, y, z, ncross=normal, method=method, step=step, tol_coef=tol_coef,
                     precision=precision, schedule=schedule)
     if perm is not None:  # back to pixel order
-        unp = {}
-        for k, v in r.items():
-            if k == "_seeds":
-                continue
-            u = torch.empty_like(v)
-            u[perm] = v
-            unp[k] = u
-        r = unp
+        r = _unpermute(r, perm)
     g = {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
     q_inner = FastQSL.QCalcPlane(g["end_x"], g["end_y"], g["end_z"], g["flag_end"],
                                  g["start_x"], g["start_y"], g["start_z"], g["flag_start"],

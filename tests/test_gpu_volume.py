@@ -134,3 +134,21 @@ def test_subbox_rk4_matches_oracle(fq, oracle, rope):
                 assert cls[cond].all() and d[cond].max() <= 1e-5
             else:
                 assert s_["class_match"] >= 0.999 and s_["frac_le_1e3"] >= 0.995
+
+
+def test_qcube_brick_order_does_not_change_results():
+    """3-D Q traced in 4^3 bricks (the default) or x-slowest order: the same
+    cube, bit for bit."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import fastqsl_amd as fq
+    from fastqsl_amd import qmap
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=24)
+    ax = [np.linspace(1, 22, m, dtype=np.float32) for m in (13, 10, 9)]   # not multiples of 4
+    with fq.Field(*field) as f:
+        a = qmap.scott_qcube(f, *ax, brick=0)
+        b = qmap.scott_qcube(f, *ax)
+        torch.cuda.synchronize()
+        for k in a:
+            np.testing.assert_array_equal(a[k].cpu().numpy(), b[k].cpu().numpy())
