@@ -68,7 +68,7 @@ def scott_qcube(field: Field, x_axis, y_axis, z_axis, method="rkf45", step=0.25,
     """3-D Scott Q: one line per sample of the box x_axis x y_axis x z_axis
     (grid-index coordinates), all voxels in one launch, traced in ``brick``^3
     blocks of samples (0: x slowest, z fastest; on the 512x256x256 rope
-    arcade 4^3 bricks trace 5 % faster).  Returns a dict of (nx, ny, nz)
+    arcade 4^3 bricks trace 5-10 % faster).  Returns a dict of (nx, ny, nz)
     tensors (``Qube`` of the reference's 3-D notebooks)."""
     x, y, z, shape = cube_seeds(x_axis, y_axis, z_axis)
     perm = block_order(shape, (brick,) * 3) if brick and brick > 1 else None
@@ -110,11 +110,14 @@ def block_order(shape, block):
     the order."""
     import torch
     key = (tuple(int(v) for v in shape), tuple(int(v) for v in block), torch.cuda.current_device())
-    if key not in _block_order_dev:
+    t = _block_order_dev.get(key)
+    if t is None:
         t = torch.as_tensor(_block_order_host(key[0], key[1]), device="cuda")
         torch.cuda.current_stream().synchronize()  # complete before another stream or thread reads it
+        while len(_block_order_dev) >= 8:          # a few shapes at a time
+            _block_order_dev.pop(next(iter(_block_order_dev)), None)
         _block_order_dev[key] = t
-    return _block_order_dev[key]
+    return t
 
 
 def _unpermute(r, perm):
