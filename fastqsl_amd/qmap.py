@@ -117,6 +117,7 @@ def block_order(shape, block):
         while len(_block_order_dev) >= 8:          # a few shapes at a time
             _block_order_dev.pop(next(iter(_block_order_dev)), None)
         _block_order_dev[key] = t
+    t.record_stream(torch.cuda.current_stream())  # an evicted order outlives its last use on this stream
     return t
 
 
