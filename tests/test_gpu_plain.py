@@ -364,3 +364,17 @@ def test_neighbour_qmap_tile_order_does_not_change_results(fq):
         np.testing.assert_array_equal(a["q_all"].cpu().numpy(), b["q_all"].cpu().numpy())
         for k in ("end_x", "start_y", "flag_end", "b_flag", "b_end_z", "line_len"):
             np.testing.assert_array_equal(a["traces"][k].cpu().numpy(), b["traces"][k].cpu().numpy())
+
+
+def test_scott_qmap_tile_order_does_not_change_results(fq):
+    """The Scott Q map traced in 16 x 16 pixel blocks (the default) or row by
+    row: the same map, bit for bit."""
+    import torch
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=48)
+    seeds, shape = synth.plane_seeds_z(70, 52, (2, 45), (3, 44), z=0.0)
+    with fq.Field(*field) as f:
+        a = fq.qmap.scott_qmap(f, seeds, shape, tile=0)
+        b = fq.qmap.scott_qmap(f, seeds, shape)
+        torch.cuda.synchronize()
+        for k in a:
+            np.testing.assert_array_equal(a[k].cpu().numpy(), b[k].cpu().numpy())
