@@ -121,6 +121,43 @@ def cook_pseudo_lines(x_start, y_start, z_start, flag_start, bz_start, bz_end, b
     return out
 
 
+def cook_pseudo_lines_async(x_start, y_start, z_start, flag_start, bz_start, bz_end, b_flag, stride):
+    """``cook_pseudo_lines`` without waiting for the count: the arrays are
+    sized for every pixel (the most there can be) and ``counts`` is a CUDA
+    int64 tensor [pixels, 4 x pixels] written on the stream; only its leading
+    entries of each array are set.  Feed ``counts[1:]`` to
+    ``trace_plain(..., n_device=...)`` and ``counts[:1]`` to
+    ``Qcut(..., m_device=...)`` so the pseudo-line pipeline never waits on the
+    host."""
+    torch = _torch()
+    d = _Dev()
+    f32, i32 = np.float32, np.int32
+    bf = d(b_flag, i32).reshape(-1)
+    xs, ys, zs = (d(a, f32).reshape(-1) for a in (x_start, y_start, z_start))
+    fs = d(flag_start, i32).reshape(-1)
+    bzs, bze = d(bz_start, f32).reshape(-1), d(bz_end, f32).reshape(-1)
+    n = bf.numel()
+    L = _lib.lib()
+    wsb = int(L.fqsl_pseudo_workspace_bytes(n))
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    out = dict(seed_x=torch.empty(4 * n, dtype=torch.float32, device="cuda"),
+               seed_y=torch.empty(4 * n, dtype=torch.float32, device="cuda"),
+               seed_z=torch.empty(4 * n, dtype=torch.float32, device="cuda"),
+               bz0_start=torch.empty(n, dtype=torch.float32, device="cuda"),
+               bz0_end=torch.empty(n, dtype=torch.float32, device="cuda"),
+               pixel=torch.empty(n, dtype=torch.int64, device="cuda"),
+               counts=torch.empty(2, dtype=torch.int64, device="cuda"))
+    _lib.check(L.fqsl_pseudo_seeds_async(n, bf.data_ptr(), xs.data_ptr(), ys.data_ptr(), zs.data_ptr(),
+                                         fs.data_ptr(), bzs.data_ptr(), bze.data_ptr(),
+                                         ctypes.c_float(float(stride)), n, out["seed_x"].data_ptr(),
+                                         out["seed_y"].data_ptr(), out["seed_z"].data_ptr(),
+                                         out["bz0_start"].data_ptr(), out["bz0_end"].data_ptr(),
+                                         out["pixel"].data_ptr(), out["counts"].data_ptr(),
+                                         ws.data_ptr(), wsb, _stream()))
+    out["_keep"] = (bf, xs, ys, zs, fs, bzs, bze, ws)  # inputs and scratch live until the stream ran
+    return out
+
+
 def CookPseudoLine(x_end_arr, y_end_arr, z_end_arr, flag_end_arr,
                    x_start_arr, y_start_arr, z_start_arr, flag_start_arr,
                    Bx_in_arr, By_in_arr, Bz_in_arr,
@@ -154,11 +191,13 @@ def CookPseudoLine(x_end_arr, y_end_arr, z_end_arr, flag_end_arr,
 
 def Qcut(cut_start_x, cut_start_y, cut_start_z, flag_cut_start,
          cut_end_x, cut_end_y, cut_end_z, flag_cut_end, Bz0_start, Bz0_end, stride_this,
-         Q_all=None, B_flag_arr=None, pixel=None):
+         Q_all=None, B_flag_arr=None, pixel=None, m_device=None):
     """Q of the pseudo-line pixels from their four traces (demo/Q_RKF45CuPy.ipynb
     cell [10]).  With ``Q_all`` and the pixel indices (or ``B_flag_arr``) the
     values are scattered into ``Q_all`` in place (cell [11],
-    ``Q_all[B_flag_arr==1] = Qcut``); otherwise a length-m array is returned."""
+    ``Q_all[B_flag_arr==1] = Qcut``); otherwise a length-m array is returned.
+    ``m_device`` (CUDA int64 tensor, with ``Q_all`` and ``pixel``): only the
+    first of its value pixels are assembled, the count read on the device."""
     torch = _torch()
     d = _Dev()
     f32, i32 = np.float32, np.int32
@@ -181,6 +220,16 @@ def Qcut(cut_start_x, cut_start_y, cut_start_z, flag_cut_start,
             pix = pixel.to(device="cuda", dtype=torch.int64).contiguous()
         if pix.numel() != m:
             raise ValueError(f"{pix.numel()} flagged pixels but {m} pseudo-line groups")
+    if m_device is not None:
+        if Q_all is None or pixel is None:
+            raise ValueError("m_device needs Q_all and pixel")
+        if not (isinstance(m_device, torch.Tensor) and m_device.is_cuda and m_device.dtype == torch.int64):
+            raise ValueError("m_device must be a CUDA int64 tensor")
+        _lib.check(_lib.lib().fqsl_qcut_scatter_n(
+            m, m_device.data_ptr(), sx.data_ptr(), sy.data_ptr(), sz.data_ptr(), fs.data_ptr(),
+            ex.data_ptr(), ey.data_ptr(), ez.data_ptr(), fe.data_ptr(), bzs.data_ptr(), bze.data_ptr(),
+            ctypes.c_float(float(stride_this)), pix.data_ptr(), target.data_ptr(), _stream()))
+        return Q_all
     _lib.check(_lib.lib().fqsl_qcut_scatter(
         m, sx.data_ptr(), sy.data_ptr(), sz.data_ptr(), fs.data_ptr(), ex.data_ptr(), ey.data_ptr(),
         ez.data_ptr(), fe.data_ptr(), bzs.data_ptr(), bze.data_ptr(), ctypes.c_float(float(stride_this)),

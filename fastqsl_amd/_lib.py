@@ -82,6 +82,12 @@ PROTOTYPES = {
     "fqsl_pseudo_seeds": (i32, [u64, vp, vp, vp, vp, vp, vp, vp, ctypes.c_float, u64,
                                 vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64), vp]),
     "fqsl_qcut_scatter": (i32, [u64] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),
+    "fqsl_trace_plain_n": (i32, [vp, ctypes.POINTER(fqsl_plain_params), u64, vp, vp, vp, vp,
+                                 ctypes.POINTER(fqsl_plain_out), vp]),
+    "fqsl_pseudo_workspace_bytes": (ctypes.c_size_t, [u64]),
+    "fqsl_pseudo_seeds_async": (i32, [u64, vp, vp, vp, vp, vp, vp, vp, ctypes.c_float, u64,
+                                      vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, vp]),
+    "fqsl_qcut_scatter_n": (i32, [u64, vp] + [vp] * 10 + [ctypes.c_float, vp, vp, vp]),
     "fqsl_slog10q": (i32, [u64, vp, vp, vp, vp, vp]),
     "fqsl_bad_pixels": (i32, [u64, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, ctypes.POINTER(u64), vp]),
     "fqsl_scatter_f32": (i32, [u64, vp, vp, vp, vp]),

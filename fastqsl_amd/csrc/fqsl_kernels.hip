@@ -461,6 +461,10 @@ struct DevScalars {
     const float* tol_coef;
     const bool* twist_flag;
     double* twist;
+    // plain tracers: line count written on the device by an earlier kernel of
+    // the stream (the pseudo-line compaction); the kernel traces
+    // min(n, *n_dev) lines, n being the capacity of the seed and output arrays
+    const unsigned long long* n_dev;
 };
 
 struct Seeds {
@@ -969,6 +973,10 @@ __device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, co
 template <class F>
 __device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsigned long long& n,
                                                     const DevScalars& ds, const float* ncross) {
+    if (ds.n_dev) {
+        const unsigned long long m = ds.n_dev[0];
+        if (m < n) n = m;
+    }
     if (!ds.shape) return;
     f.n[0] = ds.shape[0]; f.n[1] = ds.shape[1]; f.n[2] = ds.shape[2];
     P.s0 = (double)ds.s_len[0];
@@ -1080,6 +1088,20 @@ extern "C" int fqsl_last_kernel_ms(float* ms) {
     return FQSL_OK;
 }
 
+// Grid of a one-thread-per-line launch.  When the line count lives on the
+// device (the reference-shaped entry points, the pseudo-line traces) n_hint
+// is only a cap, so the grid
+// is bounded by the chip instead: 4096 threads per CU (eight occupancy waves
+// at two waves per SIMD), the grid-stride loop covering larger counts.
+static int per_line_grid(unsigned long long n_hint, int block, const DevScalars& ds) {
+    int g = grid_for((long)n_hint, block);
+    if (ds.N || ds.n_dev) {
+        const int cap = cu_count() * (4096 / block);
+        if (g > cap) g = cap;
+    }
+    return g;
+}
+
 template <typename K>
 static int persistent_grid(K kernel, int* grid) {
     int per_cu = 0;
@@ -1158,7 +1180,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         if (rc) return rc;
         auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F, FJ>
                                                                         : k_scott_per_line<R, F, FJ>;
-        kern<<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, o, ds, fj);
+        kern<<<per_line_grid(n_hint, 256, ds), 256, 0, st>>>(f, P, s, n, o, ds, fj);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
@@ -1198,8 +1220,8 @@ static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsign
         int rc = record_start(st);
         if (rc) return rc;
         const int block = per_line_block(g);
-        k_plain_per_line<R, F, FJ, G><<<grid_for((long)n_hint, block), block, 0, st>>>(f, P, s, n, o, ds, ncross,
-                                                                                     fj, g);
+        k_plain_per_line<R, F, FJ, G><<<per_line_grid(n_hint, block, ds), block, 0, st>>>(f, P, s, n, o, ds, ncross,
+                                                                                        fj, g);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
@@ -1867,11 +1889,11 @@ extern "C" int fqsl_scott_batch_destroy(fqsl_scott_batch* b) {
 
 // The stretched tracer on a resident field (RKF45 only, as the reference).
 static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, const Seeds& s, unsigned long long n,
-                                 const PlainOut& o, int precision, int schedule, hipStream_t st) {
+                                 const PlainOut& o, int precision, int schedule, hipStream_t st,
+                                 const DevScalars& ds) {
     if (P.method != FQSL_METHOD_RKF45)
         return set_err(FQSL_ERR_UNSUPPORTED, "a stretched grid is traced with RKF45 only (TraceBlineAdaptiveStretch.cu)");
     const int* nn = field->n;
-    DevScalars ds{};
     if (precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
         const StretchRef<double> g{field->sg64};
@@ -1901,8 +1923,9 @@ static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, 
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
 }
 
-extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n, const float* x,
-                                const float* y, const float* z, const fqsl_plain_out* out, void* stream) {
+static int trace_plain_impl(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n,
+                            const unsigned long long* n_dev, const float* x, const float* y, const float* z,
+                            const fqsl_plain_out* out, void* stream) {
     if (!field || !p || !out) return set_err(FQSL_ERR_INVALID, "NULL field/params/out");
     int rc = check_common(p->method, p->precision, p->schedule);
     if (rc) return rc;
@@ -1921,10 +1944,11 @@ extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params
         (o.bsx && (!o.bsy || !o.bsz)) || (o.bex && (!o.bey || !o.bez)))
         return set_err(FQSL_ERR_INVALID, "x/y/z output arrays must be all set or all NULL");
     DevScalars ds{};
+    ds.n_dev = n_dev;
     hipStream_t st = (hipStream_t)stream;
     const int* nn = field->n;
     if (o.tw && !field->cur32[0]) return set_err(FQSL_ERR_INVALID, "twist requested but the field has no current density");
-    if (field->stretched) return trace_plain_stretched(field, P, s, n, o, p->precision, p->schedule, st);
+    if (field->stretched) return trace_plain_stretched(field, P, s, n, o, p->precision, p->schedule, st, ds);
     if (p->precision == FQSL_PREC_FP64) {
         if (!field->soa64[0]) return set_err(FQSL_ERR_INVALID, "field has no FP64 layout");
         FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
@@ -1952,10 +1976,23 @@ extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
 }
 
+extern "C" int fqsl_trace_plain(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n, const float* x,
+                                const float* y, const float* z, const fqsl_plain_out* out, void* stream) {
+    return trace_plain_impl(field, p, n, nullptr, x, y, z, out, stream);
+}
+
+extern "C" int fqsl_trace_plain_n(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n_cap,
+                                  const uint64_t* n_dev, const float* x, const float* y, const float* z,
+                                  const fqsl_plain_out* out, void* stream) {
+    if (!n_dev) return set_err(FQSL_ERR_INVALID, "n_dev is NULL");
+    return trace_plain_impl(field, p, n_cap, (const unsigned long long*)n_dev, x, y, z, out, stream);
+}
+
 // ------------------------------------------- reference-shaped entry points
 // The seed count N lives on the device; the per-line launch needs a grid on
-// the host, so it covers up to kPerLineMaxBlocks * 256 threads with a
-// grid-stride loop (as the reference's fixed grids do).
+// the host, so it runs a chip-sized grid (per_line_grid) whose grid-stride
+// loop covers any N, as the reference's fixed grids do.  kPerLineMaxBlocks *
+// 256 is only the cap passed as the line-count hint.
 static constexpr int kPerLineMaxBlocks = 65536;
 
 extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* BshapeN, float* curB_x, float* curB_y,
@@ -2087,9 +2124,12 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return set_err(FQSL_ERR_UNSUPPORTED, "device index %d", dev);
+    // The lock is held until the launch is enqueued: a later call from another
+    // thread that replaces the tables first synchronises the device, which
+    // then covers this launch, before it frees them.
+    std::lock_guard<std::mutex> lock(g_dropin_mu);
     StretchRef<float> g{nullptr};
     {
-        std::lock_guard<std::mutex> lock(g_dropin_mu);
         DropinAxes& c = g_dropin_axes[dev];
         const bool same = c.mem && c.rule == rule && c.a[0] == a[0] && c.a[1] == a[1] && c.a[2] == a[2];
         if (!same) {
@@ -2099,7 +2139,7 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
             const StretchGrid<float>* g32 = nullptr;
             rc = build_stretch(a, shape, rule, &mem, &bytes, &g32, &g64);
             if (rc) return rc;
-            if (c.mem) {  // earlier launches may still read the old tables
+            if (c.mem) {  // launches enqueued earlier, on any stream, may still read the old tables
                 HIP_TRY(hipDeviceSynchronize());
                 (void)hipFree(c.mem);
             }
@@ -2183,7 +2223,9 @@ __global__ void k_qcut_scatter(unsigned long long m, const float* __restrict__ s
                                const float* __restrict__ ex, const float* __restrict__ ey,
                                const float* __restrict__ ez, const int32_t* __restrict__ fe,
                                const float* __restrict__ bzs, const float* __restrict__ bze, float scale,
-                               const uint64_t* __restrict__ pixel, float* __restrict__ qall) {
+                               const uint64_t* __restrict__ pixel, float* __restrict__ qall,
+                               const unsigned long long* __restrict__ m_dev) {
+    if (m_dev && m_dev[0] < m) m = m_dev[0];
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < m;
          i += (unsigned long long)gridDim.x * blockDim.x) {
         float X1[4], Y1[4], X2[4], Y2[4];
@@ -2225,6 +2267,7 @@ __global__ void k_pseudo_count(unsigned long long n, const int32_t* __restrict__
     if (threadIdx.x == 0) counts[blockIdx.x] = s;
 }
 
+// total[0] = selected pixels, total[1] = their pseudo lines (four each).
 __global__ void k_pseudo_scan(unsigned nblocks, unsigned* __restrict__ counts, unsigned long long* __restrict__ total) {
     // single block of 1024 threads, each scans a contiguous chunk
     __shared__ unsigned long long part[1024];
@@ -2237,7 +2280,8 @@ __global__ void k_pseudo_scan(unsigned nblocks, unsigned* __restrict__ counts, u
     if (threadIdx.x == 0) {
         unsigned long long run = 0;
         for (int t = 0; t < 1024; ++t) { const unsigned long long v = part[t]; part[t] = run; run += v; }
-        *total = run;
+        total[0] = run;
+        total[1] = 4 * run;
     }
     __syncthreads();
     unsigned long long run = part[threadIdx.x];
@@ -2307,6 +2351,71 @@ extern "C" int fqsl_qcalc_plane(int ny, int nx, const float* x_end, const float*
     return FQSL_OK;
 }
 
+// Scratch of the compaction: per-block counts (offsets after the scan).
+static size_t pseudo_ws_bytes(uint64_t n) {
+    const size_t nblocks = (size_t)((n + kCompactBlock - 1) / kCompactBlock);
+    return ((nblocks * sizeof(unsigned) + 255) & ~(size_t)255) + 256;
+}
+
+extern "C" size_t fqsl_pseudo_workspace_bytes(uint64_t n) { return pseudo_ws_bytes(n); }
+
+static int pseudo_check(uint64_t n, const int32_t* b_flag, const float* x_start, const float* y_start,
+                        const float* z_start, const int32_t* flag_start, const float* bz_start, const float* bz_end,
+                        uint64_t capacity, float* seed_x, float* seed_y, float* seed_z, float* bz0_start,
+                        float* bz0_end, uint64_t* pixel) {
+    if (!b_flag || !x_start || !y_start || !z_start || !flag_start || !bz_start || !bz_end)
+        return set_err(FQSL_ERR_INVALID, "NULL input array");
+    if (capacity && (!seed_x || !seed_y || !seed_z || !bz0_start || !bz0_end || !pixel))
+        return set_err(FQSL_ERR_INVALID, "NULL output array");
+    if (n > 0xffffffffull) return set_err(FQSL_ERR_INVALID, "pseudo-line compaction supports < 2^32 pixels");
+    return FQSL_OK;
+}
+
+// count, scan, write on the stream; counts_dev[0] = selected pixels,
+// counts_dev[1] = 4 x that (the pseudo lines)
+static int pseudo_launch(uint64_t n, const int32_t* b_flag, const float* x_start, const float* y_start,
+                         const float* z_start, const int32_t* flag_start, const float* bz_start, const float* bz_end,
+                         float stride, uint64_t capacity, float* seed_x, float* seed_y, float* seed_z,
+                         float* bz0_start, float* bz0_end, uint64_t* pixel, unsigned long long* counts_dev,
+                         void* workspace, hipStream_t st) {
+    const unsigned nblocks = (unsigned)((n + kCompactBlock - 1) / kCompactBlock);
+    unsigned* counts = (unsigned*)workspace;
+    k_pseudo_count<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, counts);
+    k_pseudo_scan<<<1, 1024, 0, st>>>(nblocks, counts, counts_dev);
+    k_pseudo_write<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, x_start, y_start, z_start, flag_start, bz_start,
+                                                      bz_end, stride, counts, capacity, seed_x, seed_y, seed_z,
+                                                      bz0_start, bz0_end, pixel);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "pseudo-line compaction: %s", hipGetErrorString(e));
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_pseudo_seeds_async(uint64_t n, const int32_t* b_flag, const float* x_start, const float* y_start,
+                                       const float* z_start, const int32_t* flag_start, const float* bz_start,
+                                       const float* bz_end, float stride, uint64_t capacity, float* seed_x,
+                                       float* seed_y, float* seed_z, float* bz0_start, float* bz0_end,
+                                       uint64_t* pixel, uint64_t* counts_dev, void* workspace,
+                                       size_t workspace_bytes, void* stream) {
+    if (!counts_dev) return set_err(FQSL_ERR_INVALID, "counts_dev is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    if (n == 0) {  // nothing selected: clear the two counts on the stream
+        k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)counts_dev, 2);
+        HIP_TRY(hipGetLastError());
+        return FQSL_OK;
+    }
+    int rc = pseudo_check(n, b_flag, x_start, y_start, z_start, flag_start, bz_start, bz_end, capacity, seed_x, seed_y,
+                          seed_z, bz0_start, bz0_end, pixel);
+    if (rc) return rc;
+    if (!workspace || workspace_bytes < pseudo_ws_bytes(n))
+        return set_err(FQSL_ERR_INVALID, "workspace of %zu bytes, %zu needed (fqsl_pseudo_workspace_bytes)",
+                       workspace_bytes, pseudo_ws_bytes(n));
+    return pseudo_launch(n, b_flag, x_start, y_start, z_start, flag_start, bz_start, bz_end, stride, capacity, seed_x,
+                         seed_y, seed_z, bz0_start, bz0_end, pixel, (unsigned long long*)counts_dev, workspace, st);
+}
+
+// Pinned read-back word per host thread (allocated once, kept until exit).
+static thread_local unsigned long long* g_pinned_count = nullptr;
+
 extern "C" int fqsl_pseudo_seeds(uint64_t n, const int32_t* b_flag, const float* x_start, const float* y_start,
                                  const float* z_start, const int32_t* flag_start, const float* bz_start,
                                  const float* bz_end, float stride, uint64_t capacity, float* seed_x, float* seed_y,
@@ -2315,30 +2424,24 @@ extern "C" int fqsl_pseudo_seeds(uint64_t n, const int32_t* b_flag, const float*
     if (!count_host) return set_err(FQSL_ERR_INVALID, "count_host is NULL");
     *count_host = 0;
     if (n == 0) return FQSL_OK;
-    if (!b_flag || !x_start || !y_start || !z_start || !flag_start || !bz_start || !bz_end)
-        return set_err(FQSL_ERR_INVALID, "NULL input array");
-    if (capacity && (!seed_x || !seed_y || !seed_z || !bz0_start || !bz0_end || !pixel))
-        return set_err(FQSL_ERR_INVALID, "NULL output array");
-    if (n > 0xffffffffull) return set_err(FQSL_ERR_INVALID, "pseudo-line compaction supports < 2^32 pixels");
+    int rc = pseudo_check(n, b_flag, x_start, y_start, z_start, flag_start, bz_start, bz_end, capacity, seed_x, seed_y,
+                          seed_z, bz0_start, bz0_end, pixel);
+    if (rc) return rc;
+    if (!g_pinned_count) HIP_TRY(hipHostMalloc((void**)&g_pinned_count, 2 * sizeof(unsigned long long)));
     hipStream_t st = (hipStream_t)stream;
-    const unsigned nblocks = (unsigned)((n + kCompactBlock - 1) / kCompactBlock);
-    unsigned* counts = nullptr;
-    unsigned long long* total = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&counts, nblocks * sizeof(unsigned) + 16, st));
-    total = (unsigned long long*)nullptr;
-    HIP_TRY(hipMallocAsync((void**)&total, sizeof(unsigned long long), st));
-    k_pseudo_count<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, counts);
-    k_pseudo_scan<<<1, 1024, 0, st>>>(nblocks, counts, total);
-    k_pseudo_write<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, x_start, y_start, z_start, flag_start, bz_start,
-                                                      bz_end, stride, counts, capacity, seed_x, seed_y, seed_z,
-                                                      bz0_start, bz0_end, pixel);
-    hipError_t e = hipGetLastError();
-    unsigned long long cnt = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, total, sizeof(cnt), hipMemcpyDeviceToHost, st);
-    (void)hipFreeAsync(counts, st);
-    (void)hipFreeAsync(total, st);
+    const size_t wsb = pseudo_ws_bytes(n);
+    void* ws = nullptr;
+    HIP_TRY(hipMallocAsync(&ws, wsb + 2 * sizeof(unsigned long long), st));
+    unsigned long long* total = (unsigned long long*)((char*)ws + wsb);
+    rc = pseudo_launch(n, b_flag, x_start, y_start, z_start, flag_start, bz_start, bz_end, stride, capacity, seed_x,
+                       seed_y, seed_z, bz0_start, bz0_end, pixel, total, ws, st);
+    hipError_t e = rc ? hipSuccess : hipMemcpyAsync(g_pinned_count, total, sizeof(unsigned long long),
+                                                     hipMemcpyDeviceToHost, st);
+    (void)hipFreeAsync(ws, st);
+    if (rc) return rc;
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "pseudo-line compaction: %s", hipGetErrorString(e));
+    const unsigned long long cnt = g_pinned_count[0];
     *count_host = cnt;
     if (capacity == 0) return FQSL_OK;  // count only
     if (cnt > capacity)
@@ -2347,11 +2450,10 @@ extern "C" int fqsl_pseudo_seeds(uint64_t n, const int32_t* b_flag, const float*
     return FQSL_OK;
 }
 
-extern "C" int fqsl_qcut_scatter(uint64_t m, const float* start_x, const float* start_y, const float* start_z,
-                                 const int32_t* flag_start, const float* end_x, const float* end_y,
-                                 const float* end_z, const int32_t* flag_end, const float* bz0_start,
-                                 const float* bz0_end, float stride, const uint64_t* pixel, float* q_all,
-                                 void* stream) {
+static int qcut_impl(uint64_t m, const unsigned long long* m_dev, const float* start_x, const float* start_y,
+                     const float* start_z, const int32_t* flag_start, const float* end_x, const float* end_y,
+                     const float* end_z, const int32_t* flag_end, const float* bz0_start, const float* bz0_end,
+                     float stride, const uint64_t* pixel, float* q_all, void* stream) {
     if (m == 0) return FQSL_OK;
     if (!start_x || !start_y || !start_z || !flag_start || !end_x || !end_y || !end_z || !flag_end || !bz0_start ||
         !bz0_end || !pixel || !q_all)
@@ -2361,9 +2463,28 @@ extern "C" int fqsl_qcut_scatter(uint64_t m, const float* start_x, const float* 
     const float scale = powf((1.0f / stride) / 2.0f, 4.0f);
     k_qcut_scatter<<<grid_for((long)m, 256), 256, 0, (hipStream_t)stream>>>(
         m, start_x, start_y, start_z, flag_start, end_x, end_y, end_z, flag_end, bz0_start, bz0_end, scale, pixel,
-        q_all);
+        q_all, m_dev);
     HIP_TRY(hipGetLastError());
     return FQSL_OK;
+}
+
+extern "C" int fqsl_qcut_scatter(uint64_t m, const float* start_x, const float* start_y, const float* start_z,
+                                 const int32_t* flag_start, const float* end_x, const float* end_y,
+                                 const float* end_z, const int32_t* flag_end, const float* bz0_start,
+                                 const float* bz0_end, float stride, const uint64_t* pixel, float* q_all,
+                                 void* stream) {
+    return qcut_impl(m, nullptr, start_x, start_y, start_z, flag_start, end_x, end_y, end_z, flag_end, bz0_start,
+                     bz0_end, stride, pixel, q_all, stream);
+}
+
+extern "C" int fqsl_qcut_scatter_n(uint64_t m_cap, const uint64_t* m_dev, const float* start_x, const float* start_y,
+                                   const float* start_z, const int32_t* flag_start, const float* end_x,
+                                   const float* end_y, const float* end_z, const int32_t* flag_end,
+                                   const float* bz0_start, const float* bz0_end, float stride, const uint64_t* pixel,
+                                   float* q_all, void* stream) {
+    if (!m_dev) return set_err(FQSL_ERR_INVALID, "m_dev is NULL");
+    return qcut_impl(m_cap, (const unsigned long long*)m_dev, start_x, start_y, start_z, flag_start, end_x, end_y,
+                     end_z, flag_end, bz0_start, bz0_end, stride, pixel, q_all, stream);
 }
 
 // Pixels of a neighbour-difference map that the Scott method re-traces

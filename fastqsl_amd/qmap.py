@@ -18,7 +18,8 @@ Everything stays on the GPU; results are torch CUDA tensors.
 """
 from __future__ import annotations
 
-import functools
+import collections
+import threading
 
 import numpy as np
 
@@ -89,15 +90,11 @@ def scott_qcube(field: Field, x_axis, y_axis, z_axis, method="rkf45", step=0.25,
     return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
 
 
-@functools.lru_cache(maxsize=16)
-def _block_order_host(shape, block):
-    """Flat indices of a C-ordered grid of ``shape`` visited block by block
-    (blocks in C order, C order inside a block; partial blocks at the far
-    edges)."""
-    shape, block = tuple(int(v) for v in shape), tuple(int(v) for v in block)
+def _block_order_compute(shape, block):
     nd = len(shape)
     if all(s % b == 0 for s, b in zip(shape, block)):
-        idx = np.arange(int(np.prod(shape))).reshape([v for s, b in zip(shape, block) for v in (s // b, b)])
+        idx = np.arange(int(np.prod(shape)), dtype=np.int32).reshape(
+            [v for s, b in zip(shape, block) for v in (s // b, b)])
         return idx.transpose(list(range(0, 2 * nd, 2)) + list(range(1, 2 * nd, 2))).reshape(-1).copy()
     grids = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
     key = np.zeros(shape, np.int64)
@@ -105,26 +102,64 @@ def _block_order_host(shape, block):
         key = key * (-(-s // b)) + g // b
     for g, b in zip(grids, block):            # position inside the block
         key = key * b + g % b
-    return np.argsort(key.reshape(-1), kind="stable")
+    return np.argsort(key.reshape(-1), kind="stable").astype(np.int32)
 
 
-_block_order_dev = {}
+# Seed orders are cached on the host and per device, each cache bounded by
+# bytes (least recently used out first; an order larger than the bound is not
+# kept), under one lock: the multi-stream jobs call in from several threads.
+ORDER_CACHE_BYTES = {"host": 256 << 20, "device": 512 << 20}
+_order_lock = threading.Lock()
+_order_host = collections.OrderedDict()
+_order_dev = collections.OrderedDict()
+
+
+def _cache_put(cache, key, value, nbytes, budget):
+    if nbytes > budget:
+        return
+    cache[key] = value
+    cache.move_to_end(key)
+    while sum(v.nbytes if isinstance(v, np.ndarray) else v.numel() * v.element_size()
+              for v in cache.values()) > budget:
+        cache.popitem(last=False)
+
+
+def _block_order_host(shape, block):
+    """Flat indices (int32) of a C-ordered grid of ``shape`` visited block by
+    block (blocks in C order, C order inside a block; partial blocks at the
+    far edges)."""
+    shape, block = tuple(int(v) for v in shape), tuple(int(v) for v in block)
+    if int(np.prod(shape)) >= 2 ** 31:
+        raise ValueError(f"grid {shape} has more than 2^31 - 1 seeds")
+    key = (shape, block)
+    with _order_lock:
+        p = _order_host.get(key)
+        if p is not None:
+            _order_host.move_to_end(key)
+            return p
+    p = _block_order_compute(shape, block)
+    p.setflags(write=False)
+    with _order_lock:
+        _cache_put(_order_host, key, p, p.nbytes, ORDER_CACHE_BYTES["host"])
+    return p
 
 
 def block_order(shape, block):
-    """Device tensor of ``_block_order_host`` (cached per shape, block and
-    device): tracing seeds in this order starts a wavefront's lines close
+    """Device tensor (int32) of ``_block_order_host`` (cached per shape, block
+    and device): tracing seeds in this order starts a wavefront's lines close
     together, so they share cached cells; a line's result does not depend on
     the order."""
     import torch
     key = (tuple(int(v) for v in shape), tuple(int(v) for v in block), torch.cuda.current_device())
-    t = _block_order_dev.get(key)
+    with _order_lock:
+        t = _order_dev.get(key)
+        if t is not None:
+            _order_dev.move_to_end(key)
     if t is None:
         t = torch.as_tensor(_block_order_host(key[0], key[1]), device="cuda")
         torch.cuda.current_stream().synchronize()  # complete before another stream or thread reads it
-        while len(_block_order_dev) >= 8:          # a few shapes at a time
-            _block_order_dev.pop(next(iter(_block_order_dev)), None)
-        _block_order_dev[key] = t
+        with _order_lock:
+            _cache_put(_order_dev, key, t, t.numel() * t.element_size(), ORDER_CACHE_BYTES["device"])
     t.record_stream(torch.cuda.current_stream())  # an evicted order outlives its last use on this stream
     return t
 
@@ -156,7 +191,8 @@ def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), m
     a wavefront's lines start close together and share cached cells; results
     do not depend on the order.  Returns a dict with ``q`` = Q_final of
     shape (rows-2, cols-2), ``q_all`` (rows, cols), the per-pixel traces and
-    the number of pseudo pixels."""
+    the number of pseudo pixels (``n_pseudo``, a CUDA int64 scalar written on
+    the stream: ``int(res["n_pseudo"])`` waits for it)."""
     import torch
     x, y, z = _grid(seeds, shape)
     perm = block_order(shape, (tile, tile)) if tile and tile > 1 else None
@@ -174,22 +210,25 @@ def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), m
                                  g["b_this_x"], g["b_this_y"], g["b_this_z"],
                                  g["b_flag"], stride, normal)
     ps = stride if pseudo_stride is None else pseudo_stride
-    c = FastQSL.cook_pseudo_lines(g["start_x"], g["start_y"], g["start_z"], g["flag_start"],
-                                  g["b_start_z"], g["b_end_z"], g["b_flag"], ps)
+    # the pseudo-line pipeline stays on the stream: the compaction writes its
+    # count on the device and the pseudo trace and Qcut read it there, so the
+    # host never waits (and no host thread blocks while other streams run)
+    c = FastQSL.cook_pseudo_lines_async(g["start_x"], g["start_y"], g["start_z"], g["flag_start"],
+                                        g["b_start_z"], g["b_end_z"], g["b_flag"], ps)
     q_all = torch.zeros(shape, dtype=torch.float32, device="cuda")
     q_all[1:-1, 1:-1] = q_inner
-    m = int(c["pixel"].numel())
-    if m:
-        rc = trace_plain(field, c["seed_x"], c["seed_y"], c["seed_z"],
-                         ncross=normal if pseudo_normal is None else pseudo_normal, method=method,
-                         step=step, tol_coef=tol_coef * pseudo_tol_scale, precision=precision,
-                         schedule=schedule,
-                         outputs=("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
-                                  "flag_start", "flag_end"))
-        FastQSL.Qcut(rc["start_x"], rc["start_y"], rc["start_z"], rc["flag_start"],
-                     rc["end_x"], rc["end_y"], rc["end_z"], rc["flag_end"],
-                     c["bz0_start"], c["bz0_end"], ps, Q_all=q_all, pixel=c["pixel"])
-    return dict(q=q_all[1:-1, 1:-1], q_all=q_all, q_inner=q_inner, traces=g, n_pseudo=m)
+    rc = trace_plain(field, c["seed_x"], c["seed_y"], c["seed_z"],
+                     ncross=normal if pseudo_normal is None else pseudo_normal, method=method,
+                     step=step, tol_coef=tol_coef * pseudo_tol_scale, precision=precision,
+                     schedule=schedule, n_device=c["counts"][1:],
+                     outputs=("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
+                              "flag_start", "flag_end"))
+    FastQSL.Qcut(rc["start_x"], rc["start_y"], rc["start_z"], rc["flag_start"],
+                 rc["end_x"], rc["end_y"], rc["end_z"], rc["flag_end"],
+                 c["bz0_start"], c["bz0_end"], ps, Q_all=q_all, pixel=c["pixel"],
+                 m_device=c["counts"][:1])
+    return dict(q=q_all[1:-1, 1:-1], q_all=q_all, q_inner=q_inner, traces=g,
+                n_pseudo=c["counts"][0], _pseudo=(c, rc))
 
 
 def neighbour_qmaps(field: Field, jobs, streams=2, **kw):

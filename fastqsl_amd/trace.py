@@ -90,9 +90,14 @@ def trace_scott(field: Field, x, y, z, method="rkf45", step=0.25, tol=1e-6, prec
 
 def trace_plain(field: Field, x, y, z, ncross=(0.0, 0.0, 1.0), method="rkf45", step=0.25,
                 tol_coef=1.0, precision="fp32c", schedule="persistent", outputs=PLAIN_OUTPUTS,
-                out=None, stream=None):
+                out=None, stream=None, n_device=None):
     """Position-only tracing, forward then backward (TraceAllBline,
-    TraceBlineAdaptive.cu:429-510 for RKF45, TraceBline.cu:383-464 for RK4)."""
+    TraceBlineAdaptive.cu:429-510 for RKF45, TraceBline.cu:383-464 for RK4).
+
+    ``n_device``: a CUDA int64 tensor whose first element, written earlier on
+    the stream, is the number of seeds to trace (at most the seed arrays'
+    length); it is read on the device, so nothing waits for it on the host.
+    Outputs past that count are left unwritten."""
     torch = _torch()
     xs, ys, zs = _seed_tensors(x, y, z)
     n = xs.numel()
@@ -108,8 +113,16 @@ def trace_plain(field: Field, x, y, z, ncross=(0.0, 0.0, 1.0), method="rkf45", s
     p = _lib.fqsl_plain_params(method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision],
                                schedule=_lib.SCHEDULES[schedule], reserved=0, step=float(step),
                                tol_coef=float(tol_coef), ncross=nc, reserved2=0.0)
-    _lib.check(_lib.lib().fqsl_trace_plain(field.handle, ctypes.byref(p), n, xs.data_ptr(),
-                                           ys.data_ptr(), zs.data_ptr(), ctypes.byref(o),
-                                           _stream(stream)))
+    if n_device is None:
+        _lib.check(_lib.lib().fqsl_trace_plain(field.handle, ctypes.byref(p), n, xs.data_ptr(),
+                                               ys.data_ptr(), zs.data_ptr(), ctypes.byref(o),
+                                               _stream(stream)))
+    else:
+        if not (isinstance(n_device, torch.Tensor) and n_device.is_cuda
+                and n_device.dtype == torch.int64 and n_device.numel() >= 1):
+            raise ValueError("n_device must be a CUDA int64 tensor")
+        _lib.check(_lib.lib().fqsl_trace_plain_n(field.handle, ctypes.byref(p), n,
+                                                 n_device.data_ptr(), xs.data_ptr(), ys.data_ptr(),
+                                                 zs.data_ptr(), ctypes.byref(o), _stream(stream)))
     res["_seeds"] = (xs, ys, zs)
     return res

@@ -225,6 +225,16 @@ int fqsl_trace_plain(const fqsl_field *field, const fqsl_plain_params *params, u
                      const float *x, const float *y, const float *z, const fqsl_plain_out *out,
                      void *stream);
 
+/* fqsl_trace_plain of min(n_cap, *n_dev) seeds, the count read on the device
+ * when the kernel runs (n_dev: one device word written earlier on the stream,
+ * e.g. counts_dev + 1 of fqsl_pseudo_seeds_async); seed and output arrays
+ * hold n_cap entries.  Nothing is read back to the host.  Replaces the second
+ * TraceAllBline launch of the pseudo lines (demo/Q_RKF45CuPy.ipynb cell [9]),
+ * whose N the notebook sizes on the host. */
+int fqsl_trace_plain_n(const fqsl_field *field, const fqsl_plain_params *params, uint64_t n_cap,
+                       const uint64_t *n_dev, const float *x, const float *y, const float *z,
+                       const fqsl_plain_out *out, void *stream);
+
 /* ------------------------------------------- neighbour-difference assembly */
 /* QCalcPlane (fastqslpy/FastQSL.py:65-117) on a (ny, nx) row-major pixel grid:
  * writes q of (ny-2)*(nx-2) floats.  Arithmetic follows the reference's
@@ -248,6 +258,22 @@ int fqsl_pseudo_seeds(uint64_t n, const int32_t *b_flag, const float *x_start, c
                       float *seed_y, float *seed_z, float *bz0_start, float *bz0_end,
                       uint64_t *pixel, uint64_t *count_host, void *stream);
 
+/* Device bytes of the workspace fqsl_pseudo_seeds_async needs for n pixels. */
+size_t fqsl_pseudo_workspace_bytes(uint64_t n);
+
+/* fqsl_pseudo_seeds without a host round trip (FastQSL.py:4-57 as the first
+ * half of a pipeline that stays on the stream): the counts go to two device
+ * words, counts_dev[0] = selected pixels and counts_dev[1] = their 4x pseudo
+ * lines; outputs hold the first min(count, capacity) pixels.  The caller
+ * passes a device workspace of fqsl_pseudo_workspace_bytes(n); nothing is
+ * allocated and the stream is not synchronised. */
+int fqsl_pseudo_seeds_async(uint64_t n, const int32_t *b_flag, const float *x_start,
+                            const float *y_start, const float *z_start, const int32_t *flag_start,
+                            const float *bz_start, const float *bz_end, float stride, uint64_t capacity,
+                            float *seed_x, float *seed_y, float *seed_z, float *bz0_start,
+                            float *bz0_end, uint64_t *pixel, uint64_t *counts_dev, void *workspace,
+                            size_t workspace_bytes, void *stream);
+
 /* Qcut (demo/Q_RKF45CuPy.ipynb cell [10]) from the traces of the 4*m pseudo
  * seeds, scattered into q_all[pixel[i]] (cell [11]). */
 int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, const float *start_z,
@@ -255,6 +281,14 @@ int fqsl_qcut_scatter(uint64_t m, const float *start_x, const float *start_y, co
                       const float *end_z, const int32_t *flag_end, const float *bz0_start,
                       const float *bz0_end, float stride, const uint64_t *pixel, float *q_all,
                       void *stream);
+
+/* fqsl_qcut_scatter over min(m_cap, *m_dev) pseudo pixels, the count read on
+ * the device (counts_dev of fqsl_pseudo_seeds_async); same cells. */
+int fqsl_qcut_scatter_n(uint64_t m_cap, const uint64_t *m_dev, const float *start_x,
+                        const float *start_y, const float *start_z, const int32_t *flag_start,
+                        const float *end_x, const float *end_y, const float *end_z,
+                        const int32_t *flag_end, const float *bz0_start, const float *bz0_end,
+                        float stride, const uint64_t *pixel, float *q_all, void *stream);
 
 /* ------------------------------------------------- around the Q map */
 /* slog10 Q = log10(q) * sign(bn) [* (flag <= 6) when flag is not NULL]

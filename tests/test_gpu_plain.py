@@ -190,7 +190,7 @@ def test_neighbour_pipeline_hyperbolic(fq):
             out = fq.qmap.neighbour_qmap(f, seeds, sh, stride=6.0 / 23, method=method, step=0.125,
                                          tol_coef=1e-4)
             q = out["q"].cpu().numpy()
-            assert out["n_pseudo"] == 0
+            assert int(out["n_pseudo"]) == 0
             np.testing.assert_allclose(q, 2 * np.cosh(2 * k * shape[2]), rtol=3e-3)
 
 
@@ -344,10 +344,58 @@ def test_neighbour_qmaps_over_streams_match_serial(fq):
         piped = fq.qmap.neighbour_qmaps(f, jobs, streams=2, **kw)
         torch.cuda.synchronize()
         for a, b in zip(serial, piped):
-            assert a["n_pseudo"] == b["n_pseudo"]
+            assert int(a["n_pseudo"]) == int(b["n_pseudo"])
             np.testing.assert_array_equal(a["q_all"].cpu().numpy(), b["q_all"].cpu().numpy())
             np.testing.assert_array_equal(a["traces"]["end_x"].cpu().numpy(), b["traces"]["end_x"].cpu().numpy())
-    assert any(a["n_pseudo"] > 0 for a in serial)
+    assert any(int(a["n_pseudo"]) > 0 for a in serial)
+
+
+def test_pseudo_pipeline_counts_on_the_device(fq):
+    """The no-round-trip pseudo-line path: the compaction's device counts
+    equal the host-counted CookPseudoLine core, its leading outputs are the
+    same bits, and a trace / Qcut bounded by a device count equals the same
+    calls on exactly that many seeds / pixels."""
+    import torch
+    from fastqsl_amd import FastQSL
+    field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=48)
+    seeds, shape = synth.plane_seeds_z(72, 60, (2, 45), (2, 45), z=1e-4)
+    with fq.Field(*field) as f:
+        r = fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rkf45", step=0.125,
+                           tol_coef=0.5)
+        args = (r["start_x"], r["start_y"], r["start_z"], r["flag_start"], r["b_start_z"],
+                r["b_end_z"], r["b_flag"], 43.0 / 71)
+        host = FastQSL.cook_pseudo_lines(*args)
+        dev = FastQSL.cook_pseudo_lines_async(*args)
+        m = host["pixel"].numel()
+        assert m > 0
+        assert dev["counts"].cpu().tolist() == [m, 4 * m]
+        for k in ("seed_x", "seed_y", "seed_z", "bz0_start", "bz0_end", "pixel"):
+            np.testing.assert_array_equal(dev[k][:host[k].numel()].cpu().numpy(), host[k].cpu().numpy())
+        kw = dict(method="rkf45", step=0.125, tol_coef=0.05,
+                  outputs=("start_x", "end_x", "end_y", "end_z", "flag_start", "flag_end"))
+        for schedule in ("persistent", "per_line"):
+            a = fq.trace_plain(f, host["seed_x"], host["seed_y"], host["seed_z"], schedule=schedule, **kw)
+            out = {k: torch.full((4 * shape[0] * shape[1],), -7, dtype=v.dtype, device="cuda")
+                   for k, v in a.items() if k != "_seeds"}
+            b = fq.trace_plain(f, dev["seed_x"], dev["seed_y"], dev["seed_z"], schedule=schedule,
+                               n_device=dev["counts"][1:], out=out, **kw)
+            for k in out:
+                np.testing.assert_array_equal(b[k][:4 * m].cpu().numpy(), a[k].cpu().numpy())
+                assert bool((b[k][4 * m:] == -7).all()), k   # nothing past the count is written
+        rc = fq.trace_plain(f, host["seed_x"], host["seed_y"], host["seed_z"],
+                            outputs=("start_x", "start_y", "start_z", "end_x", "end_y", "end_z",
+                                     "flag_start", "flag_end"), method="rkf45", step=0.125, tol_coef=0.05)
+        qa = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        qb = torch.zeros(shape, dtype=torch.float32, device="cuda")
+        cut = [rc[k] for k in ("start_x", "start_y", "start_z", "flag_start", "end_x", "end_y", "end_z",
+                               "flag_end")]
+        FastQSL.Qcut(*cut, host["bz0_start"], host["bz0_end"], 43.0 / 71, Q_all=qa, pixel=host["pixel"])
+        pad = lambda t, n: torch.cat([t, torch.zeros(n - t.numel(), dtype=t.dtype, device="cuda")])
+        ntot = shape[0] * shape[1]
+        FastQSL.Qcut(*[pad(t, 4 * ntot) for t in cut], pad(host["bz0_start"], ntot),
+                     pad(host["bz0_end"], ntot), 43.0 / 71, Q_all=qb,
+                     pixel=pad(host["pixel"], ntot), m_device=dev["counts"][:1])
+        np.testing.assert_array_equal(qa.cpu().numpy(), qb.cpu().numpy())
 
 
 def test_neighbour_qmap_tile_order_does_not_change_results(fq):
