@@ -138,7 +138,6 @@ def _block_order_host(shape, block):
             _order_host.move_to_end(key)
             return p
     p = _block_order_compute(shape, block)
-    p.setflags(write=False)
     with _order_lock:
         _cache_put(_order_host, key, p, p.nbytes, ORDER_CACHE_BYTES["host"])
     return p

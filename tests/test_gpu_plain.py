@@ -358,12 +358,14 @@ def test_pseudo_pipeline_counts_on_the_device(fq):
     import torch
     from fastqsl_amd import FastQSL
     field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=48)
-    seeds, shape = synth.plane_seeds_z(72, 60, (2, 45), (2, 45), z=1e-4)
+    # 73 x 73 pixels over (2, 45): the middle row and column lie on the
+    # polarity inversion lines x, y = 23.5 (Bz = 0 by symmetry), so B_flag is set there
+    seeds, shape = synth.plane_seeds_z(73, 73, (2, 45), (2, 45), z=1e-4)
     with fq.Field(*field) as f:
         r = fq.trace_plain(f, seeds[:, 0], seeds[:, 1], seeds[:, 2], method="rkf45", step=0.125,
                            tol_coef=0.5)
         args = (r["start_x"], r["start_y"], r["start_z"], r["flag_start"], r["b_start_z"],
-                r["b_end_z"], r["b_flag"], 43.0 / 71)
+                r["b_end_z"], r["b_flag"], 43.0 / 72)
         host = FastQSL.cook_pseudo_lines(*args)
         dev = FastQSL.cook_pseudo_lines_async(*args)
         m = host["pixel"].numel()
@@ -372,7 +374,8 @@ def test_pseudo_pipeline_counts_on_the_device(fq):
         for k in ("seed_x", "seed_y", "seed_z", "bz0_start", "bz0_end", "pixel"):
             np.testing.assert_array_equal(dev[k][:host[k].numel()].cpu().numpy(), host[k].cpu().numpy())
         kw = dict(method="rkf45", step=0.125, tol_coef=0.05,
-                  outputs=("start_x", "end_x", "end_y", "end_z", "flag_start", "flag_end"))
+                  outputs=("start_x", "start_y", "start_z", "end_x", "end_y", "end_z", "flag_start",
+                           "flag_end"))
         for schedule in ("persistent", "per_line"):
             a = fq.trace_plain(f, host["seed_x"], host["seed_y"], host["seed_z"], schedule=schedule, **kw)
             out = {k: torch.full((4 * shape[0] * shape[1],), -7, dtype=v.dtype, device="cuda")
@@ -389,11 +392,11 @@ def test_pseudo_pipeline_counts_on_the_device(fq):
         qb = torch.zeros(shape, dtype=torch.float32, device="cuda")
         cut = [rc[k] for k in ("start_x", "start_y", "start_z", "flag_start", "end_x", "end_y", "end_z",
                                "flag_end")]
-        FastQSL.Qcut(*cut, host["bz0_start"], host["bz0_end"], 43.0 / 71, Q_all=qa, pixel=host["pixel"])
+        FastQSL.Qcut(*cut, host["bz0_start"], host["bz0_end"], 43.0 / 72, Q_all=qa, pixel=host["pixel"])
         pad = lambda t, n: torch.cat([t, torch.zeros(n - t.numel(), dtype=t.dtype, device="cuda")])
         ntot = shape[0] * shape[1]
         FastQSL.Qcut(*[pad(t, 4 * ntot) for t in cut], pad(host["bz0_start"], ntot),
-                     pad(host["bz0_end"], ntot), 43.0 / 71, Q_all=qb,
+                     pad(host["bz0_end"], ntot), 43.0 / 72, Q_all=qb,
                      pixel=pad(host["pixel"], ntot), m_device=dev["counts"][:1])
         np.testing.assert_array_equal(qa.cpu().numpy(), qb.cpu().numpy())
 
