@@ -55,7 +55,7 @@ def parse():
                     help="keep the library's default launch knobs instead of autotuning them")
     ap.add_argument("--per-line-fp32", action="store_true",
                     help="also time the FP32 one-thread-per-line kernel (always on for volume)")
-    ap.add_argument("--precision", default="fp32c", choices=["fp32c", "fp64"])
+    ap.add_argument("--precision", default="fp32c", choices=["fp32c", "mixed", "fp64"])
     ap.add_argument("--schedule", default=None, choices=["persistent", "per_line"],
                     help="persistent (default) or per_line; the stretch workload defaults to per_line, "
                          "which is the faster schedule for its short lines")
@@ -199,7 +199,7 @@ def bench_batch(args, fq, torch, log):
     evals = int(sum(int(r["evals"].to(torch.int64).sum().item()) for r in b.results))
     log(f"batch {args.precision} graph: {value:.4g} lines/s, {ms_step:.1f} ms per replay, "
         f"{evals / nlines:.0f} evals/line")
-    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
     extra = {}
 
     def one_by_one(precision, schedule):
@@ -434,7 +434,7 @@ def bench_neighbour(args, fq, torch, log):
         kms.append(fq.last_kernel_ms())
     evals = int(tr["evals"].to(torch.int64).sum().item())
     k = float(np.mean(kms))
-    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
     achieved = evals * bpe / (k * 1e-3)
     log(f"neighbour {args.precision}/{args.schedule}: {value:.4g} pixels/s over 2 streams "
         f"({wall / steps * 1e3:.2f} ms per map; one at a time {wall1 / steps * 1e3:.2f} ms), {npseudo} pseudo "
@@ -557,7 +557,7 @@ def bench_twist(args, fq, torch, log):
     out = outs[True]
     evals = int(out["evals"].to(torch.int64).sum().item())
     value = nlines * steps / wall
-    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
     achieved = evals * bpe / (k * 1e-3)
     log(f"twist {args.precision}/{args.schedule}: {value:.4g} lines/s ({wall / steps * 1e3:.2f} ms per map; "
         f"without twist {wall0 / steps * 1e3:.2f} ms), stepping kernel {k:.2f} ms (without twist {k0:.2f})")
@@ -674,7 +674,7 @@ def bench_stretch(args, fq, torch, log):
     evals = int(out["evals"].to(torch.int64).sum().item())
     value = nlines * steps / wall
     k = float(np.mean(kms))
-    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
     achieved = evals * bpe / (k * 1e-3)
     log(f"stretch {args.precision}/{args.schedule}: {value:.4g} lines/s, kernel {k:.2f} ms, "
         f"{evals / nlines:.1f} evals/line")
@@ -861,7 +861,7 @@ def main():
     ms_step = wall / args.steps * 1e3
     value = nlines * args.steps / wall
     kms = float(np.mean(kernel_ms))
-    bpe = BYTES_PER_EVAL_F32 if args.precision == "fp32c" else BYTES_PER_EVAL_F64
+    bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
     achieved = evals * bpe / (kms * 1e-3)
     log(f"{args.precision}/{args.schedule}: {value:.4g} lines/s, {ms_step:.2f} ms/step, "
         f"trace {np.mean(trace_ms):.2f} ms, stepping kernel {kms:.2f} ms, {evals / nlines:.1f} evals/line")

@@ -15,7 +15,7 @@ FQSL_OK = 0
 LOOKUPS = {"bisect": 0, "uniform": 1, "exact": 2}  # FQSL_LOOKUP_*
 DTYPE_F32, DTYPE_F64 = 0, 1
 METHODS = {"rkf45": 0, "rk4": 1}
-PRECISIONS = {"fp32c": 0, "fp64": 1}
+PRECISIONS = {"fp32c": 0, "fp64": 1, "mixed": 2}
 SCHEDULES = {"persistent": 0, "per_line": 1}
 LAYOUT_SOA32, LAYOUT_SOA64, LAYOUT_CELL32, LAYOUT_ALL = 1, 2, 4, 7
 

@@ -493,8 +493,8 @@ struct PlainOut {
     double* tw;
 };
 
-template <typename R, class F>
-__device__ __forceinline__ void scott_write(const ScottLine<R>& L, const ScottEnd<R>& e, int status,
+template <typename R, typename X, class F>
+__device__ __forceinline__ void scott_write(const ScottLine<R, X>& L, const ScottEnd<R>& e, int status,
                                             const F& f, unsigned long long i, const ScottOut& o,
                                             const DevScalars& ds) {
     double q0, qp, nb0 = 0.0;
@@ -739,7 +739,7 @@ __global__ void __launch_bounds__(256) k_scott_prep(F f, ScottParams P, Seeds s,
 // queue entries once at least `refill_min` lanes of the wavefront are idle (or
 // all are), so the setup of refilled lines runs once for the group.  A wave
 // exits once every queue is empty and all its lanes are idle.
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, typename X>
 __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                       const R* __restrict__ frames, ScottRec<R>* __restrict__ recs,
                                                       DevScalars ds, unsigned long long* queues, int nq,
@@ -747,7 +747,7 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
     apply_scalars(f, P, n, ds);
     fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
     P.ctl = scott_ctl(P.tol);  // on the device, so every path forms the same bits
-    ScottLine<R> L;
+    ScottLine<R, X> L;
     bool active = false;
     bool drained = false;
     unsigned long long idx = 0;
@@ -784,7 +784,7 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
         }
         if (active) {
             ScottEnd<R> e;
-            const int st = scott_attempt<R, F, FJ>(L, f, P, e, fj);
+            const int st = scott_attempt<R, F, FJ, X>(L, f, P, e, fj);
             if (st != kTracing) {
                 recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                 active = false;
@@ -795,20 +795,20 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
 
 // W = waves per SIMD the register allocation targets.
 #define FQSL_PERSISTENT_KERNEL(W)                                                                     \
-    template <typename R, class F, class FJ>                                                          \
+    template <typename R, class F, class FJ, typename X>                                              \
     __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))                \
     k_scott_persistent_w##W(F f, ScottParams P, Seeds s, unsigned long long n, const R* frames,       \
                             ScottRec<R>* recs, DevScalars ds, unsigned long long* queues, int nq,     \
                             int refill_min, const unsigned* order, FJ fj) {                           \
-        scott_persistent_body<R, F, FJ>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj); \
+        scott_persistent_body<R, F, FJ, X>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj); \
     }
 
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, typename X>
 __global__ void __launch_bounds__(256) k_scott_persistent(F f, ScottParams P, Seeds s, unsigned long long n,
                                                           const R* frames, ScottRec<R>* recs, DevScalars ds,
                                                           unsigned long long* queues, int nq, int refill_min,
                                                           const unsigned* order, FJ fj) {
-    scott_persistent_body<R, F, FJ>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj);
+    scott_persistent_body<R, F, FJ, X>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj);
 }
 FQSL_PERSISTENT_KERNEL(2)
 FQSL_PERSISTENT_KERNEL(3)
@@ -840,7 +840,7 @@ __global__ void __launch_bounds__(256) k_scott_finish(F f, ScottParams P, unsign
 
 // One thread per line (grid-stride), the reference's mapping
 // (TraceBlineScott.cu:530-553).
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, typename X>
 __device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                     ScottOut out, DevScalars ds, FJ fj) {
     apply_scalars(f, P, n, ds);
@@ -848,25 +848,25 @@ __device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s,
     P.ctl = scott_ctl(P.tol);  // on the device, so every path forms the same bits
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
-        ScottLine<R> L;
+        ScottLine<R, X> L;
         ScottEnd<R> e;
         int st = scott_init<R>(L, f, P, s.x[i], s.y[i], s.z[i], e);
-        while (st == kTracing) st = scott_attempt<R, F, FJ>(L, f, P, e, fj);
+        while (st == kTracing) st = scott_attempt<R, F, FJ, X>(L, f, P, e, fj);
         scott_write<R>(L, e, st, f, i, out, ds);
     }
 }
 
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, typename X>
 __global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seeds s, unsigned long long n,
                                                         ScottOut out, DevScalars ds, FJ fj) {
-    scott_per_line_body<R, F, FJ>(f, P, s, n, out, ds, fj);
+    scott_per_line_body<R, F, FJ, X>(f, P, s, n, out, ds, fj);
 }
 
 // FP32 variant compiled for two waves per SIMD (like the persistent kernel).
-template <typename R, class F, class FJ>
+template <typename R, class F, class FJ, typename X>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_scott_per_line_w2(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds, FJ fj) {
-    scott_per_line_body<R, F, FJ>(f, P, s, n, out, ds, fj);
+    scott_per_line_body<R, F, FJ, X>(f, P, s, n, out, ds, fj);
 }
 
 // ------------------------------------------------------------ plain kernels
@@ -1111,75 +1111,119 @@ static int persistent_grid(K kernel, int* grid) {
     return FQSL_OK;
 }
 
-template <typename R, class F, class FJ = NoCurrent>
+// Workspace of one persistent Scott trace: seed frames, finished-line
+// records and, when the queue is ordered, the order, the bucket keys and
+// counts; each part 256-byte aligned.
+struct ScottWs {
+    size_t frames, recs, order, key, counts, bytes;
+};
+
+constexpr size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+template <typename R>
+constexpr ScottWs scott_ws_layout(unsigned long long n, bool ordered) {
+    constexpr auto up = align256;
+    ScottWs w{};
+    w.frames = 0;
+    w.recs = up((size_t)kFrameWords * sizeof(R) * n);
+    w.order = up(w.recs + sizeof(ScottRec<R>) * n);
+    w.key = w.order + (ordered ? up(sizeof(unsigned) * n) : 0);
+    w.counts = w.key + (ordered ? up(n) : 0);
+    w.bytes = w.counts + (ordered ? sizeof(unsigned) * kOrderBuckets : 0);
+    return w;
+}
+
+// Stepping kernel and grid of the persistent Scott schedule.
+template <typename R, class F, class FJ, typename X>
+static int scott_persistent_kernel(decltype(&k_scott_persistent<R, F, FJ, X>)* kern, int* grid) {
+    const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES].load() : 0;
+    *kern = waves == 2 ? k_scott_persistent_w2<R, F, FJ, X>
+          : waves == 3 ? k_scott_persistent_w3<R, F, FJ, X>
+                       : k_scott_persistent<R, F, FJ, X>;
+    int rc = persistent_grid(*kern, grid);
+    if (rc) return rc;
+    *grid = (int)((long)*grid * g_tune[FQSL_TUNE_GRID_PCT] / 100);
+    if (*grid < 1) *grid = 1;
+    return FQSL_OK;
+}
+
+template <typename R>
+static bool scott_ordered(const DevScalars& ds, unsigned long long n_hint, int grid) {
+    const int om = g_tune[FQSL_TUNE_ORDER];
+    return !ds.shape && n_hint < (1ull << 32) &&
+           (om == 2 || (om == 1 && n_hint <= 64ull * 256ull * (unsigned long long)grid));
+}
+
+// The persistent schedule's launches on `st` over a workspace `ws` laid out
+// by scott_ws_layout and the queue counters `q` (kQueueSlotWidth words).
+// `timed` records the trace and stepping-kernel events of this thread.
+template <typename R, class F, class FJ, typename X>
+static hipError_t scott_persistent_launches(const F& f, const ScottParams& P, const Seeds& s, unsigned long long n,
+                                            const ScottOut& o, const DevScalars& ds, hipStream_t st,
+                                            unsigned long long n_hint, const FJ& fj, char* ws, const ScottWs& w,
+                                            bool ordered, unsigned long long* q,
+                                            decltype(&k_scott_persistent<R, F, FJ, X>) kern, int grid, bool timed) {
+    R* frames = (R*)(ws + w.frames);
+    ScottRec<R>* recs = (ScottRec<R>*)(ws + w.recs);
+    unsigned* order = ordered ? (unsigned*)(ws + w.order) : nullptr;
+    unsigned char* key = ordered ? (unsigned char*)ws + w.key : nullptr;
+    unsigned* counts = ordered ? (unsigned*)(ws + w.counts) : nullptr;
+    const int nq = g_tune[FQSL_TUNE_QUEUES];
+    const int refill_min = g_tune[FQSL_TUNE_REFILL_MIN];
+    k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (timed && record_start(st)) return hipErrorUnknown;
+    if (ordered) {
+        k_zero_u32<<<1, 256, 0, st>>>(counts, kOrderBuckets);
+        k_order_predict<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(
+            f, s, n, g_tune[FQSL_TUNE_ORDER_BUCKETS], key, counts);
+        k_order_scan<<<1, kOrderBuckets, 0, st>>>(counts);
+        const int sg = grid_for((long)n_hint, 256) < 4096 ? grid_for((long)n_hint, 256) : 4096;
+        k_order_scatter<<<sg, 256, 0, st>>>(key, n, counts, order);
+    }
+    k_scott_prep<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, frames, o, ds);
+    if (timed) e = hipEventRecord(g_evs0, st);
+    kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min, order, fj);
+    if (timed && e == hipSuccess) e = hipEventRecord(g_evs1, st);
+    if (timed && e == hipSuccess) g_evs_valid = true;
+    k_scott_finish<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, n, frames, recs, o, ds);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (timed && e == hipSuccess && record_end(st)) return hipErrorUnknown;
+    return e;
+}
+
+// X: the position path's type (R, or double for the mixed variant).
+template <typename R, class F, class FJ = NoCurrent, typename X = R>
 static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsigned long long n, const ScottOut& o,
                         const DevScalars& ds, int schedule, hipStream_t st, unsigned long long n_hint,
-                        const FJ& fj = FJ{}) {
+                        char* noalloc, const FJ& fj = FJ{}) {
     if (schedule == FQSL_SCHED_PERSISTENT) {
         // n_hint: number of lines, known on the host (sizes the workspace)
         if (n_hint == 0) return FQSL_OK;
-        unsigned long long* q = nullptr;
-        int rc = queue_slot(&q);
-        if (rc) return rc;
-        const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES].load() : 0;
-        auto kern = waves == 2 ? k_scott_persistent_w2<R, F, FJ>
-                  : waves == 3 ? k_scott_persistent_w3<R, F, FJ>
-                               : k_scott_persistent<R, F, FJ>;
+        decltype(&k_scott_persistent<R, F, FJ, X>) kern = nullptr;
         int grid = 0;
-        rc = persistent_grid(kern, &grid);
+        int rc = scott_persistent_kernel<R, F, FJ, X>(&kern, &grid);
         if (rc) return rc;
-        grid = (int)((long)grid * g_tune[FQSL_TUNE_GRID_PCT] / 100);
-        if (grid < 1) grid = 1;
-        const int nq = g_tune[FQSL_TUNE_QUEUES];
-        const int refill_min = g_tune[FQSL_TUNE_REFILL_MIN];
-        const int om = g_tune[FQSL_TUNE_ORDER];
-        const bool ordered = !ds.shape && n_hint < (1ull << 32) &&
-                             (om == 2 || (om == 1 && n_hint <= 64ull * 256ull * (unsigned long long)grid));
-        const size_t frame_bytes = (size_t)kFrameWords * sizeof(R) * n_hint;
-        const size_t rec_off = (frame_bytes + 255) & ~(size_t)255;
-        const size_t order_off = (rec_off + sizeof(ScottRec<R>) * n_hint + 255) & ~(size_t)255;
-        const size_t key_off = order_off + (ordered ? ((sizeof(unsigned) * n_hint + 255) & ~(size_t)255) : 0);
-        const size_t cnt_off = key_off + (ordered ? ((n_hint + 255) & ~(size_t)255) : 0);
-        const size_t ws_bytes = cnt_off + (ordered ? sizeof(unsigned) * kOrderBuckets : 0);
+        const bool ordered = scott_ordered<R>(ds, n_hint, grid);
+        const ScottWs w = scott_ws_layout<R>(n_hint, ordered);
+        unsigned long long* q = nullptr;
+        rc = queue_slot(&q);
+        if (rc) return rc;
         void* ws = nullptr;
-        HIP_TRY(hipMallocAsync(&ws, ws_bytes, st));
-        R* frames = (R*)ws;
-        ScottRec<R>* recs = (ScottRec<R>*)((char*)ws + rec_off);
-        unsigned* order = ordered ? (unsigned*)((char*)ws + order_off) : nullptr;
-        unsigned char* key = ordered ? (unsigned char*)ws + key_off : nullptr;
-        unsigned* counts = ordered ? (unsigned*)((char*)ws + cnt_off) : nullptr;
-        k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) rc = record_start(st);
-        if (e == hipSuccess && !rc) {
-            if (ordered) {
-                k_zero_u32<<<1, 256, 0, st>>>(counts, kOrderBuckets);
-                k_order_predict<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(
-                    f, s, n, g_tune[FQSL_TUNE_ORDER_BUCKETS], key, counts);
-                k_order_scan<<<1, kOrderBuckets, 0, st>>>(counts);
-                const int sg = grid_for((long)n_hint, 256) < 4096 ? grid_for((long)n_hint, 256) : 4096;
-                k_order_scatter<<<sg, 256, 0, st>>>(key, n, counts, order);
-            }
-            k_scott_prep<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, s, n, frames, o, ds);
-            e = hipEventRecord(g_evs0, st);
-            kern<<<grid, 256, 0, st>>>(f, P, s, n, frames, recs, ds, q, nq, refill_min, order, fj);
-            if (e == hipSuccess) e = hipEventRecord(g_evs1, st);
-            if (e == hipSuccess) g_evs_valid = true;
-            k_scott_finish<R, F><<<grid_for((long)n_hint, 256), 256, 0, st>>>(f, P, n, frames, recs, o, ds);
-            e = hipGetLastError();
-            if (e == hipSuccess) rc = record_end(st);
-        }
+        HIP_TRY(hipMallocAsync(&ws, w.bytes, st));
+        const hipError_t e = scott_persistent_launches<R, F, FJ, X>(f, P, s, n, o, ds, st, n_hint, fj, (char*)ws,
+                                                                    w, ordered, q, kern, grid, true);
         const hipError_t ef = hipFreeAsync(ws, st);
         if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "Scott trace launch: %s", hipGetErrorString(e));
-        if (rc) return rc;
         if (ef != hipSuccess) return set_err(FQSL_ERR_HIP, "hipFreeAsync: %s", hipGetErrorString(ef));
         return FQSL_OK;
     }
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
         if (rc) return rc;
-        auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F, FJ>
-                                                                        : k_scott_per_line<R, F, FJ>;
+        auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F, FJ, X>
+                                                                        : k_scott_per_line<R, F, FJ, X>;
         kern<<<per_line_grid(n_hint, 256, ds), 256, 0, st>>>(f, P, s, n, o, ds, fj);
         HIP_TRY(hipGetLastError());
         return record_end(st);
@@ -1228,24 +1272,30 @@ static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsign
     return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
 }
 
-static int check_common(int method, int precision, int schedule) {
+static int check_common(int method, int precision, int schedule, bool scott = false) {
     if (method != FQSL_METHOD_RKF45 && method != FQSL_METHOD_RK4) return set_err(FQSL_ERR_INVALID, "method %d", method);
-    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64)
+    if (precision == FQSL_PREC_MIXED && !scott)
+        return set_err(FQSL_ERR_UNSUPPORTED, "FQSL_PREC_MIXED is a Scott-tracer precision");
+    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64 && precision != FQSL_PREC_MIXED)
         return set_err(FQSL_ERR_INVALID, "precision %d", precision);
     if (schedule != FQSL_SCHED_PERSISTENT && schedule != FQSL_SCHED_PER_LINE)
         return set_err(FQSL_ERR_INVALID, "schedule %d", schedule);
     return FQSL_OK;
 }
 
-extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params* p, uint64_t n, const float* x,
-                                const float* y, const float* z, const fqsl_scott_out* out, void* stream) {
+static int trace_scott_impl(const fqsl_field* field, const fqsl_scott_params* p, uint64_t n, const float* x,
+                            const float* y, const float* z, const fqsl_scott_out* out, void* stream, char* noalloc) {
     if (!field || !p || !out) return set_err(FQSL_ERR_INVALID, "NULL field/params/out");
-    int rc = check_common(p->method, p->precision, p->schedule);
+    int rc = check_common(p->method, p->precision, p->schedule, true);
     if (rc) return rc;
     if (n == 0) return FQSL_OK;
     if (!x || !y || !z) return set_err(FQSL_ERR_INVALID, "NULL seed array");
     if (!(p->step > 0) || !(p->step <= 100.0)) return set_err(FQSL_ERR_INVALID, "step %g outside (0, 100]", p->step);
     if (p->method == FQSL_METHOD_RKF45 && !(p->tol > 0)) return set_err(FQSL_ERR_INVALID, "tol %g must be > 0", p->tol);
+    if (p->method == FQSL_METHOD_RKF45 && p->precision == FQSL_PREC_FP32C && p->tol < kFp32cMinTol)
+        return set_err(FQSL_ERR_INVALID,
+                       "tol %g is below what the FP32 error norm resolves (%g cells); use FQSL_PREC_MIXED or "
+                       "FQSL_PREC_FP64", p->tol, kFp32cMinTol);
     ScottParams P{p->method, p->step, p->tol, 0.0};  // ctl is formed on the device
     Seeds s{x, y, z};
     ScottOut o{out->start_x, out->start_y, out->start_z, out->end_x, out->end_y, out->end_z,
@@ -1264,26 +1314,44 @@ extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params
         FieldSoA<double> f{field->soa64[0], field->soa64[1], field->soa64[2], {nn[0], nn[1], nn[2]}};
         if (o.tw) {
             FieldSoA<double> fj{field->cur64[0], field->cur64[1], field->cur64[2], {nn[0], nn[1], nn[2]}};
-            return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n, fj);
+            return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n, noalloc, fj);
         }
-        return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n);
+        return launch_scott<double>(f, P, s, n, o, ds, p->schedule, st, n, noalloc);
     }
     const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
+    const bool mixed = p->precision == FQSL_PREC_MIXED;
     if (field->cell32) {
         FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
         if (o.tw) {
             if (!field->curcell) return set_err(FQSL_ERR_INVALID, "current density has no cell layout");
             const FieldCell32 fjc{field->curcell, {nn[0], nn[1], nn[2]}};
-            return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, fjc);
+            if (mixed)
+                return launch_scott<float, FieldCell32, FieldCell32, double>(f, P, s, n, o, ds, p->schedule, st, n,
+                                                                             noalloc, fjc);
+            return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, noalloc, fjc);
         }
-        return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
+        if (mixed)
+            return launch_scott<float, FieldCell32, NoCurrent, double>(f, P, s, n, o, ds, p->schedule, st, n, noalloc);
+        return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, noalloc);
     }
     if (field->soa32[0]) {
         FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, fj);
-        return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n);
+        if (mixed) {
+            if (o.tw)
+                return launch_scott<float, FieldSoA<float>, FieldSoA<float>, double>(f, P, s, n, o, ds, p->schedule,
+                                                                                     st, n, noalloc, fj);
+            return launch_scott<float, FieldSoA<float>, NoCurrent, double>(f, P, s, n, o, ds, p->schedule, st, n,
+                                                                           noalloc);
+        }
+        if (o.tw) return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, noalloc, fj);
+        return launch_scott<float>(f, P, s, n, o, ds, p->schedule, st, n, noalloc);
     }
     return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
+extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params* p, uint64_t n, const float* x,
+                                const float* y, const float* z, const fqsl_scott_out* out, void* stream) {
+    return trace_scott_impl(field, p, n, x, y, z, out, stream, nullptr);
 }
 
 // ---------------------------------------------------- current density (curl)
@@ -1561,11 +1629,11 @@ __global__ void __launch_bounds__(256) k_batch_prep(F f, BatchGroupArgs g, R* __
 
 // Persistent stepping over one group: as scott_persistent_body, with the
 // step and tolerance of each lane's request.
-template <typename R, class F>
+template <typename R, class F, typename X>
 __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs g, const R* __restrict__ frames,
                                                  ScottRec<R>* __restrict__ recs, BatchCtl c, int qi,
                                                  int refill_min) {
-    ScottLine<R> L;
+    ScottLine<R, X> L;
     ScottParams P{method, 0.0, 0.0, 0.0};
     bool active = false;
     bool drained = false;
@@ -1609,7 +1677,7 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
         }
         if (active) {
             ScottEnd<R> e;
-            const int st = scott_attempt<R>(L, f, P, e);
+            const int st = scott_attempt<R, F, NoCurrent, X>(L, f, P, e);
             if (st != kTracing) {
                 recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                 active = false;
@@ -1619,17 +1687,17 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
     }
 }
 
-template <typename R, class F>
+template <typename R, class F, typename X>
 __global__ void __launch_bounds__(256) k_batch_step(F f, int method, BatchGroupArgs g, const R* frames,
                                                     ScottRec<R>* recs, BatchCtl c, int qi, int refill_min) {
-    scott_batch_body<R, F>(f, method, g, frames, recs, c, qi, refill_min);
+    scott_batch_body<R, F, X>(f, method, g, frames, recs, c, qi, refill_min);
 }
 
-template <typename R, class F>
+template <typename R, class F, typename X>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_batch_step_w2(F f, int method, BatchGroupArgs g, const R* frames, ScottRec<R>* recs, BatchCtl c, int qi,
                 int refill_min) {
-    scott_batch_body<R, F>(f, method, g, frames, recs, c, qi, refill_min);
+    scott_batch_body<R, F, X>(f, method, g, frames, recs, c, qi, refill_min);
 }
 
 template <typename R, class F>
@@ -1686,8 +1754,6 @@ struct BatchLayout {
     size_t queues, remaining, stamps, reqs, offs, frames, recs, total;
 };
 
-size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
-
 BatchLayout batch_layout(int nreq, int ngroups, unsigned long long lines, int precision) {
     const size_t rsz = precision == FQSL_PREC_FP64 ? sizeof(double) : sizeof(float);
     const size_t recsz = precision == FQSL_PREC_FP64 ? sizeof(ScottRec<double>) : sizeof(ScottRec<float>);
@@ -1711,6 +1777,9 @@ int check_request(const fqsl_scott_request& q, int precision, int k) {
         return set_err(FQSL_ERR_INVALID, "request %d: method %d", k, q.method);
     if (!(q.step > 0) || !(q.step <= 100.0)) return set_err(FQSL_ERR_INVALID, "request %d: step %g outside (0, 100]", k, q.step);
     if (q.method == FQSL_METHOD_RKF45 && !(q.tol > 0)) return set_err(FQSL_ERR_INVALID, "request %d: tol %g must be > 0", k, q.tol);
+    if (q.method == FQSL_METHOD_RKF45 && precision == FQSL_PREC_FP32C && q.tol < kFp32cMinTol)
+        return set_err(FQSL_ERR_INVALID, "request %d: tol %g is below what the FP32 error norm resolves (%g); use "
+                       "FQSL_PREC_MIXED or FQSL_PREC_FP64", k, q.tol, kFp32cMinTol);
     if (q.n && (!q.x || !q.y || !q.z)) return set_err(FQSL_ERR_INVALID, "request %d: NULL seed array", k);
     if (q.out.twist) return set_err(FQSL_ERR_UNSUPPORTED, "request %d: twist is not available in batches", k);
     const fqsl_scott_out& o = q.out;
@@ -1719,7 +1788,7 @@ int check_request(const fqsl_scott_request& q, int precision, int k) {
         return set_err(FQSL_ERR_INVALID, "request %d: x/y/z output arrays must be all set or all NULL", k);
     if (precision == FQSL_PREC_FP64 && !q.field->soa64[0])
         return set_err(FQSL_ERR_INVALID, "request %d: field has no FP64 layout", k);
-    if (precision == FQSL_PREC_FP32C && !q.field->cell32 && !q.field->soa32[0])
+    if (precision != FQSL_PREC_FP64 && !q.field->cell32 && !q.field->soa32[0])
         return set_err(FQSL_ERR_INVALID, "request %d: field has no FP32 layout", k);
     return FQSL_OK;
 }
@@ -1735,12 +1804,12 @@ std::vector<int> batch_order(const fqsl_scott_request* reqs, int nreq) {
     return ord;
 }
 
-template <typename R, class F>
+template <typename R, class F, typename X = R>
 int batch_launch_group(const fqsl_scott_batch* b, const BatchGroup& G, int gi, const F& f, hipStream_t st) {
     BatchGroupArgs g{b->reqs, b->offs, G.r0, G.r1, G.base, G.n};
     R* frames = (R*)b->frames;
     ScottRec<R>* recs = (ScottRec<R>*)b->recs;
-    auto kern = sizeof(R) == 4 && b->waves == 2 ? k_batch_step_w2<R, F> : k_batch_step<R, F>;
+    auto kern = sizeof(R) == 4 && b->waves == 2 ? k_batch_step_w2<R, F, X> : k_batch_step<R, F, X>;
     int grid = 0;
     int rc = persistent_grid(kern, &grid);
     if (rc) return rc;
@@ -1771,7 +1840,7 @@ extern "C" int fqsl_scott_batch_create(const fqsl_scott_request* reqs, int nreq,
     if (!out) return set_err(FQSL_ERR_INVALID, "out is NULL");
     *out = nullptr;
     if (!reqs || nreq <= 0) return set_err(FQSL_ERR_INVALID, "empty batch");
-    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64)
+    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64 && precision != FQSL_PREC_MIXED)
         return set_err(FQSL_ERR_INVALID, "precision %d", precision);
     for (int k = 0; k < nreq; ++k) {
         int rc = check_request(reqs[k], precision, k);
@@ -1853,10 +1922,12 @@ extern "C" int fqsl_scott_batch_run(const fqsl_scott_batch* b, void* stream) {
             rc = batch_launch_group<double>(b, G, gi, f, st);
         } else if (fd->cell32) {
             FieldCell32 f{fd->cell32, {fd->n[0], fd->n[1], fd->n[2]}};
-            rc = batch_launch_group<float>(b, G, gi, f, st);
+            rc = b->precision == FQSL_PREC_MIXED ? batch_launch_group<float, FieldCell32, double>(b, G, gi, f, st)
+                                                 : batch_launch_group<float>(b, G, gi, f, st);
         } else {
             FieldSoA<float> f{fd->soa32[0], fd->soa32[1], fd->soa32[2], {fd->n[0], fd->n[1], fd->n[2]}};
-            rc = batch_launch_group<float>(b, G, gi, f, st);
+            rc = b->precision == FQSL_PREC_MIXED ? batch_launch_group<float, FieldSoA<float>, double>(b, G, gi, f, st)
+                                                 : batch_launch_group<float>(b, G, gi, f, st);
         }
         if (rc) return rc;
     }
@@ -2012,10 +2083,10 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
     }
     if (twist_on && (!curB_x || !curB_y || !curB_z || !twist))
         return set_err(FQSL_ERR_INVALID, "curB_flag is set: curB_x/y/z and twist must be device arrays");
-    int rc = check_common(FQSL_METHOD_RKF45, precision, schedule);
+    int rc = check_common(FQSL_METHOD_RKF45, precision, schedule, true);
     if (rc) return rc;
-    if (precision != FQSL_PREC_FP32C)
-        return set_err(FQSL_ERR_UNSUPPORTED, "the in-place float field supports FQSL_PREC_FP32C only");
+    if (precision == FQSL_PREC_FP64)
+        return set_err(FQSL_ERR_UNSUPPORTED, "the in-place float field supports FQSL_PREC_FP32C and FQSL_PREC_MIXED");
     ScottParams P{FQSL_METHOD_RKF45, 0.0, 0.0, 0.0};
     Seeds s{inp_x, inp_y, inp_z};
     ScottOut o{start_x, start_y, start_z, end_x, end_y, end_z, flag_start, flag_end, B_this_x, B_this_y, B_this_z,
@@ -2028,11 +2099,18 @@ extern "C" int fqsl_TraceAllBlineScott(float* Bx, float* By, float* Bz, int* Bsh
         HIP_TRY(hipMemcpyAsync(&n_hint, N, sizeof(n_hint), hipMemcpyDeviceToHost, (hipStream_t)stream));
         HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
     }
+    const hipStream_t st = (hipStream_t)stream;
     if (twist_on) {
         const FieldSoA<float> fj{curB_x, curB_y, curB_z, {2, 2, 2}};  // shape taken from B's on the device
-        return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream, n_hint, fj);
+        if (precision == FQSL_PREC_MIXED)
+            return launch_scott<float, FieldSoA<float>, FieldSoA<float>, double>(f, P, s, 0ull, o, ds, schedule, st,
+                                                                                 n_hint, nullptr, fj);
+        return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, st, n_hint, nullptr, fj);
     }
-    return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, (hipStream_t)stream, n_hint);
+    if (precision == FQSL_PREC_MIXED)
+        return launch_scott<float, FieldSoA<float>, NoCurrent, double>(f, P, s, 0ull, o, ds, schedule, st, n_hint,
+                                                                       nullptr);
+    return launch_scott<float>(f, P, s, 0ull, o, ds, schedule, st, n_hint, nullptr);
 }
 
 extern "C" int fqsl_TraceAllBline(int method, float* Bx, float* By, float* Bz, int* BshapeN, float* curB_x,

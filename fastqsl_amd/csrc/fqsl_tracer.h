@@ -13,6 +13,12 @@
 //             accuracy at any distance from the origin; arc length, the RKF45
 //             error norm, the step controller, landing and Q assembly in FP64.
 //   double -- everything in FP64.
+// The Scott tracer has a third, mixed variant (R = float, X = double): the
+// position path -- b = B/|B| at the stage points, the stage positions, the
+// increments and the RKF45 error norm -- in FP64 from the float cell
+// coefficients, the Jacobian path (grad b, the deviation vectors u, v) in
+// FP32.  Its error estimate has no FP32 round-off floor, so its step-size
+// sequence follows the FP64 controller's.
 //
 // Reference behaviour reproduced here (file:line in peijin94/FastQSL):
 //   field indexing Arr[k*ny*nx + j*nx + i]         TraceBlineScott.cu:36-39
@@ -73,6 +79,12 @@ constexpr int kPlainRk4StepLim = 100000;     // MAX_NUM_STEP, TraceBline.cu:12
 constexpr double kPlainRk4LenRatio = 2.0;    // MAX_STEP_RATIO, TraceBline.cu:10
 constexpr double kTolBase = 1e-3;            // TOL, TraceBlineAdaptive.cu:12
 constexpr int kAttemptCap = 1000000;         // hard bound on integrator attempts per line
+// Smallest RKF45 tolerance (cells) the FP32 path accepts.  Its error norm is
+// formed from float stages, whose round-off puts a floor of ~1e-8 x step
+// under the estimate; down to 1e-9 the controller still steers by the true
+// error (on a 256^3 multipole it keeps the oracle's accuracy with fewer
+// evaluations), below it the floor pins the step near its lower clamp.
+constexpr double kFp32cMinTol = 1e-9;
 
 enum Method : int { kRKF45 = 0, kRK4 = 1 };
 
@@ -451,6 +463,74 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     scott_axis<2, R, F>(f, p, c, w, simple[2], B, b, inb, d[2], J[2]);
 }
 
+// The mixed variant's right-hand side at a double stage point: b in FP64
+// (the trilinear value from the cell's float data, evaluated in double and
+// normalised in double) and grad b in FP32 as scott_eval's float path takes
+// it (closed form on simple axes, the exact difference quotient elsewhere).
+template <class F>
+FQSL_HD void scott_eval_mixed(const F& f, const Pt<double>& pd, double bd[3], float J[3][3]) {
+    const double h = kFdH;
+    int c[3];
+    double wd[3];
+    float w[3];
+    bool simple[3];
+    Pt<float> p;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int i = pd.i[a], n = f.n[a];
+        const double r = pd.r[a];
+        const bool low = i < 0 || (i == 0 && r == 0.0);  // x <= 0
+        const bool high = i >= n - 1;                     // x >= N-1
+        c[a] = low ? 0 : (high ? n - 2 : i);
+        wd[a] = (r != r) ? r : (low ? 0.0 : (high ? 1.0 : r));
+        w[a] = (float)wd[a];
+        simple[a] = !low && !high && r >= h && r + h <= 1.0;
+        p.i[a] = i;
+        p.r[a] = (float)r;
+    }
+    float cc[3][8];
+    f.template load<float>(c[0], c[1], c[2], cc);
+    double Bd[3];
+    float B[3], d[3][3];
+    if constexpr (F::kCoef) {
+        const double x = wd[0], y = wd[1], z = wd[2];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double c0 = cc[q][0], c1 = cc[q][1], c2 = cc[q][2], c3 = cc[q][3];
+            const double c4 = cc[q][4], c5 = cc[q][5], c6 = cc[q][6], c7 = cc[q][7];
+            const double A = (c0 + c2 * y) + x * (c1 + c4 * y);
+            const double C = (c3 + c5 * y) + x * (c6 + c7 * y);
+            Bd[q] = A + z * C;
+            d[2][q] = (float)C;
+            d[0][q] = (cc[q][1] + cc[q][4] * w[1]) + w[2] * (cc[q][6] + cc[q][7] * w[1]);
+            d[1][q] = (cc[q][2] + cc[q][4] * w[0]) + w[2] * (cc[q][5] + cc[q][7] * w[0]);
+        }
+    } else {
+        double cd[3][8], dd[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cd[q][k] = cc[q][k];
+        cell_eval<double>(cd, wd[0], wd[1], wd[2], Bd, dd);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) d[j][q] = (float)dd[j][q];
+    }
+    const double inbd = rsq(dot3(Bd, Bd));
+    float b[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        bd[q] = Bd[q] * inbd;
+        b[q] = (float)bd[q];
+        B[q] = (float)Bd[q];
+    }
+    const float inb = (float)inbd;
+    scott_axis<0, float, F>(f, p, c, w, simple[0], B, b, inb, d[0], J[0]);
+    scott_axis<1, float, F>(f, p, c, w, simple[1], B, b, inb, d[1], J[1]);
+    scott_axis<2, float, F>(f, p, c, w, simple[2], B, b, inb, d[2], J[2]);
+}
+
 // Unit vector b only (plain tracer RHS: Interp3dxyzn(..., true)).
 template <typename R, class F>
 FQSL_HD void unit_b(const F& f, const Pt<R>& p, R b[3]) {
@@ -652,11 +732,12 @@ struct ScottEnd {
 // State of one Scott line (TraceBlineScott, TraceBlineScott.cu:334-519).  Only
 // what a line carries from one attempt to the next lives here; the seed frame
 // (b0, u0, v0) is recomputed when the second direction starts and the end
-// footpoint is handed back to the caller when the line finishes.
-template <typename R>
+// footpoint is handed back to the caller when the line finishes.  X is the
+// position path's type (R, or double in the mixed variant).
+template <typename R, typename X = R>
 struct ScottLine {
     float p0[3];
-    Pos<R> x;
+    Pos<X> x;
     R u[3], v[3];
     double s, len;
     double tw;                 // twist accumulated over both directions
@@ -700,8 +781,8 @@ FQSL_HD void seed_frame(const F& f, const float* p0, R u0[3], R v0[3], R* b0z, d
 // would force both, and the line state, out of registers into scratch.  For
 // the same reason no reference into a line's state, or to the field view,
 // is ever passed to an out-of-line (FQSL_COLD) function.)
-template <typename R>
-FQSL_HD void scott_record(ScottLine<R>& L, ScottEnd<R>& e, const double* x, const R* u, const R* v, int flag) {
+template <typename R, typename X>
+FQSL_HD void scott_record(ScottLine<R, X>& L, ScottEnd<R>& e, const double* x, const R* u, const R* v, int flag) {
     if (L.dir < 0) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) { L.st.x[a] = (R)x[a]; L.st.u[a] = u[a]; L.st.v[a] = v[a]; }
@@ -713,8 +794,8 @@ FQSL_HD void scott_record(ScottLine<R>& L, ScottEnd<R>& e, const double* x, cons
     }
 }
 
-template <typename R>
-FQSL_HD bool scott_can_step(const ScottLine<R>& L, int len_lim) {
+template <typename R, typename X>
+FQSL_HD bool scott_can_step(const ScottLine<R, X>& L, int len_lim) {
     return L.len < (double)len_lim && L.steps < kScottStepLim && L.attempts < kAttemptCap;
 }
 
@@ -727,8 +808,8 @@ FQSL_HD int scott_len_lim(const F& f) {
 // (TraceBlineScott.cu:396-401) and directions that cannot take a step because
 // the shared length/step budget is spent record their footpoint at once.
 // Returns kTracing, or kFinished once the forward side is recorded.
-template <typename R, class F>
-FQSL_HD int scott_begin_dir(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
+template <typename R, typename X, class F>
+FQSL_HD int scott_begin_dir(ScottLine<R, X>& L, const F& f, ScottEnd<R>& e) {
     for (;;) {
         R u0[3], v0[3], b0z;
         if (L.fr) {
@@ -755,8 +836,8 @@ FQSL_HD int scott_begin_dir(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
     }
 }
 
-template <typename R, class F>
-FQSL_HD int scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float px, float py, float pz,
+template <typename R, typename X, class F>
+FQSL_HD int scott_init(ScottLine<R, X>& L, const F& f, const ScottParams& P, float px, float py, float pz,
                          ScottEnd<R>& e, const R* frame = nullptr) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
     L.fr = frame;
@@ -774,27 +855,39 @@ FQSL_HD int scott_init(ScottLine<R>& L, const F& f, const ScottParams& P, float 
     return scott_begin_dir<R>(L, f, e);
 }
 
-// Right-hand side for a stage: k = s * f(y + delta) for the 9-vector.
-template <typename R, class F>
-FQSL_HD void scott_stage(const F& f, const ScottLine<R>& L, const R* dx, const R* du, const R* dv,
-                         R sc, R k[9]) {
-    const Pt<R> p = L.x.stage(dx);
-    R b[3], J[3][3];
-    scott_eval<R>(f, p, b, J);
+// Right-hand side for a stage: k = s * f(y + delta) for the 9-vector, the
+// position part kx (type X, step scx) and the deviation part ku = (u', v')
+// (type R, step sc).
+template <typename R, typename X, class F>
+FQSL_HD void scott_stage(const F& f, const ScottLine<R, X>& L, const X* dx, const R* du, const R* dv,
+                         X scx, R sc, X kx[3], R ku[6]) {
+    R J[3][3];
+    if constexpr (std::is_same<R, X>::value) {
+        const Pt<R> p = L.x.stage(dx);
+        R b[3];
+        scott_eval<R>(f, p, b, J);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) kx[q] = sc * b[q];
+    } else {
+        const Pt<X> p = L.x.stage(dx);
+        X b[3];
+        scott_eval_mixed<F>(f, p, b, J);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) kx[q] = scx * b[q];
+    }
     R uu[3], vv[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) { uu[a] = L.u[a] + du[a]; vv[a] = L.v[a] + dv[a]; }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        k[q] = sc * b[q];
-        k[3 + q] = sc * (uu[0] * J[0][q] + uu[1] * J[1][q] + uu[2] * J[2][q]);
-        k[6 + q] = sc * (vv[0] * J[0][q] + vv[1] * J[1][q] + vv[2] * J[2][q]);
+        ku[q] = sc * (uu[0] * J[0][q] + uu[1] * J[1][q] + uu[2] * J[2][q]);
+        ku[3 + q] = sc * (vv[0] * J[0][q] + vv[1] * J[1][q] + vv[2] * J[2][q]);
     }
 }
 
 // Close the active side; returns the line status after it.
-template <typename R, class F>
-FQSL_HD int scott_side_done(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
+template <typename R, typename X, class F>
+FQSL_HD int scott_side_done(ScottLine<R, X>& L, const F& f, ScottEnd<R>& e) {
     if (L.dir < 0) {
         L.dir = 1;
         return scott_begin_dir<R>(L, f, e);
@@ -805,49 +898,72 @@ FQSL_HD int scott_side_done(ScottLine<R>& L, const F& f, ScottEnd<R>& e) {
 // One integrator attempt (one pass of the while loop at
 // TraceBlineScott.cu:407-470).  Returns kTracing, kFinished (both footpoints
 // known: start in L.st, end in e) or kAborted (class 42: Q = 0).
-template <typename R, class F, class FJ = NoCurrent>
-FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, ScottEnd<R>& e,
+template <typename R, class F, class FJ = NoCurrent, typename X = R>
+FQSL_HD int scott_attempt(ScottLine<R, X>& L, const F& f, const ScottParams& P, ScottEnd<R>& e,
                           const FJ& fj = FJ{}) {
     const R sc = (R)(L.s * L.dir);
-    R inc[9];
+    const X scx = (X)(L.s * L.dir);
+    X incx[3];
+    R incu[6];
     L.attempts += 1;
     if (P.method == kRKF45) {
-        R k1[9], k2[9], k3[9], k4[9], k5[9], k6[9], t[9];
-        const R zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        scott_stage<R>(f, L, zero, zero + 3, zero + 6, sc, k1);
+        // stage k_i = (kx_i, ku_i); t = the stage offset sum_j a_ij k_j
+        X x1[3], x2[3], x3[3], x4[3], x5[3], x6[3], tx[3];
+        R u1[6], u2[6], u3[6], u4[6], u5[6], u6[6], tu[6];
+        const X zx[3] = {0, 0, 0};
+        const R zu[6] = {0, 0, 0, 0, 0, 0};
+        scott_stage<R, X>(f, L, zx, zu, zu + 3, scx, sc, x1, u1);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) t[c] = (R)Tab::a21 * k1[c];
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k2);
+        for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a21 * x1[c];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) t[c] = (R)Tab::a31 * k1[c] + (R)Tab::a32 * k2[c];
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k3);
+        for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a21 * u1[c];
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x2, u2);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) t[c] = (R)Tab::a41 * k1[c] + (R)Tab::a42 * k2[c] + (R)Tab::a43 * k3[c];
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k4);
+        for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a31 * x1[c] + (X)Tab::a32 * x2[c];
 #pragma unroll
-        for (int c = 0; c < 9; ++c)
-            t[c] = (R)Tab::a51 * k1[c] + (R)Tab::a52 * k2[c] + (R)Tab::a53 * k3[c] + (R)Tab::a54 * k4[c];
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k5);
+        for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a31 * u1[c] + (R)Tab::a32 * u2[c];
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x3, u3);
 #pragma unroll
-        for (int c = 0; c < 9; ++c)
-            t[c] = (R)Tab::a61 * k1[c] + (R)Tab::a62 * k2[c] + (R)Tab::a63 * k3[c] + (R)Tab::a64 * k4[c] +
-                   (R)Tab::a65 * k5[c];
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k6);
+        for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a41 * x1[c] + (X)Tab::a42 * x2[c] + (X)Tab::a43 * x3[c];
 #pragma unroll
-        for (int c = 0; c < 9; ++c)
-            inc[c] = (R)Tab::b1 * k1[c] + (R)Tab::b3 * k3[c] + (R)Tab::b4 * k4[c] + (R)Tab::b5 * k5[c] +
-                     (R)Tab::b6 * k6[c];
+        for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a41 * u1[c] + (R)Tab::a42 * u2[c] + (R)Tab::a43 * u3[c];
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x4, u4);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            tx[c] = (X)Tab::a51 * x1[c] + (X)Tab::a52 * x2[c] + (X)Tab::a53 * x3[c] + (X)Tab::a54 * x4[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            tu[c] = (R)Tab::a51 * u1[c] + (R)Tab::a52 * u2[c] + (R)Tab::a53 * u3[c] + (R)Tab::a54 * u4[c];
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x5, u5);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            tx[c] = (X)Tab::a61 * x1[c] + (X)Tab::a62 * x2[c] + (X)Tab::a63 * x3[c] + (X)Tab::a64 * x4[c] +
+                    (X)Tab::a65 * x5[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            tu[c] = (R)Tab::a61 * u1[c] + (R)Tab::a62 * u2[c] + (R)Tab::a63 * u3[c] + (R)Tab::a64 * u4[c] +
+                    (R)Tab::a65 * u5[c];
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x6, u6);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            incx[c] = (X)Tab::b1 * x1[c] + (X)Tab::b3 * x3[c] + (X)Tab::b4 * x4[c] + (X)Tab::b5 * x5[c] +
+                      (X)Tab::b6 * x6[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            incu[c] = (R)Tab::b1 * u1[c] + (R)Tab::b3 * u3[c] + (R)Tab::b4 * u4[c] + (R)Tab::b5 * u5[c] +
+                      (R)Tab::b6 * u6[c];
         double e2 = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double ec = Tab::e1 * (double)k1[c] + Tab::e3 * (double)k3[c] + Tab::e4 * (double)k4[c] +
-                              Tab::e5 * (double)k5[c] + Tab::e6 * (double)k6[c];
+            const double ec = Tab::e1 * (double)x1[c] + Tab::e3 * (double)x3[c] + Tab::e4 * (double)x4[c] +
+                              Tab::e5 * (double)x5[c] + Tab::e6 * (double)x6[c];
             e2 += ec * ec;
         }
         L.evals += 6;
         // step controller (TraceBlineScott.cu:413-419), FP64: the FP64 path
-        // evaluates the reference's expression as written; the FP32 path the
-        // same quantity as ctl * e2^(-1/10) (double-accurate, no pow/sqrt/div)
+        // evaluates the reference's expression as written; the FP32 and mixed
+        // paths the same quantity as ctl * e2^(-1/10) (double-accurate, no
+        // pow/sqrt/div)
         const double scale = sizeof(R) == 4 ? P.ctl * pow_minus_tenth(e2)
                                             : pow(P.tol / sqrt(e2) / 11.09, 0.2);
         if (scale < 0.618) {
@@ -864,33 +980,45 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
         if (L.s < 1. / 10.) L.s = 1. / 10.;
     } else {
         // RK4 (TraceBline.cu:226-234) on the 9-vector, fixed step
-        R k[9], t[9];
-        const R zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        X kx[3], tx[3];
+        R ku[6], tu[6];
+        const X zx[3] = {0, 0, 0};
+        const R zu[6] = {0, 0, 0, 0, 0, 0};
+        const X halfx = (X)0.5;
         const R half = (R)0.5;
-        scott_stage<R>(f, L, zero, zero + 3, zero + 6, sc, k);
+        scott_stage<R, X>(f, L, zx, zu, zu + 3, scx, sc, kx, ku);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) { inc[c] = k[c]; t[c] = half * k[c]; }
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k);
+        for (int c = 0; c < 3; ++c) { incx[c] = kx[c]; tx[c] = halfx * kx[c]; }
 #pragma unroll
-        for (int c = 0; c < 9; ++c) { inc[c] += (R)2 * k[c]; t[c] = half * k[c]; }
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k);
+        for (int c = 0; c < 6; ++c) { incu[c] = ku[c]; tu[c] = half * ku[c]; }
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) { inc[c] += (R)2 * k[c]; t[c] = k[c]; }
-        scott_stage<R>(f, L, t, t + 3, t + 6, sc, k);
+        for (int c = 0; c < 3; ++c) { incx[c] += (X)2 * kx[c]; tx[c] = halfx * kx[c]; }
 #pragma unroll
-        for (int c = 0; c < 9; ++c) inc[c] = (inc[c] + k[c]) * (R)(1.0 / 6.0);
+        for (int c = 0; c < 6; ++c) { incu[c] += (R)2 * ku[c]; tu[c] = half * ku[c]; }
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { incx[c] += (X)2 * kx[c]; tx[c] = kx[c]; }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) { incu[c] += (R)2 * ku[c]; tu[c] = ku[c]; }
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) incx[c] = (incx[c] + kx[c]) * (X)(1.0 / 6.0);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) incu[c] = (incu[c] + ku[c]) * (R)(1.0 / 6.0);
         L.evals += 4;
     }
 
     // accepted step: a = current state, b = a + inc
     const double xa[3] = {L.x.get(0), L.x.get(1), L.x.get(2)};
-    Pos<R> xb = L.x;
-    xb.add(inc);
+    Pos<X> xb = L.x;
+    xb.add(incx);
     const double xbv[3] = {xb.get(0), xb.get(1), xb.get(2)};
-    // step length: the increment is R, so its norm is taken in R; the sum over
+    // step length: the increment is X, so its norm is taken in X; the sum over
     // steps is kept in double
-    const double dL = sizeof(R) == 4 ? (double)sqrtf((float)(inc[0] * inc[0] + inc[1] * inc[1] + inc[2] * inc[2]))
-                                     : sqrt((double)inc[0] * inc[0] + (double)inc[1] * inc[1] + (double)inc[2] * inc[2]);
+    const double dL = sizeof(X) == 4 ? (double)sqrtf((float)(incx[0] * incx[0] + incx[1] * incx[1] + incx[2] * incx[2]))
+                                     : sqrt((double)incx[0] * incx[0] + (double)incx[1] * incx[1] +
+                                            (double)incx[2] * incx[2]);
     L.len += dL;
     // twist at the step's end point (TraceBlineScott.cu:423-427)
     double tw_rate = 0.0, Bend[3] = {0.0, 0.0, 0.0};
@@ -915,8 +1043,8 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     xl[a] = (xa[a] * (p2 - pm) + xbv[a] * (pm - p1)) / (p2 - p1);
-                    ul[a] = (R)((double)L.u[a] * wa + (double)(L.u[a] + inc[3 + a]) * wb);
-                    vl[a] = (R)((double)L.v[a] * wa + (double)(L.v[a] + inc[6 + a]) * wb);
+                    ul[a] = (R)((double)L.u[a] * wa + (double)(L.u[a] + incu[a]) * wb);
+                    vl[a] = (R)((double)L.v[a] * wa + (double)(L.v[a] + incu[3 + a]) * wb);
                 }
                 // arc length to the face: with twist on, the reference's B_P1
                 // is the raw B at the step's end point (:425) and also scales
@@ -926,9 +1054,9 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
                 if constexpr (has_current<FJ>()) {
                     bdim = Bend[dim];
                 } else {
-                    R ba[3];
-                    const R zero3[3] = {0, 0, 0};
-                    unit_b<R>(f, L.x.stage(zero3), ba);
+                    X ba[3];
+                    const X zero3[3] = {0, 0, 0};
+                    unit_b<X>(f, L.x.stage(zero3), ba);
                     bdim = (double)ba[dim];
                 }
                 const double arc = fabs(pm - p1) / (1e-4 + fabs(bdim));
@@ -953,7 +1081,7 @@ FQSL_HD int scott_attempt(ScottLine<R>& L, const F& f, const ScottParams& P, Sco
     }
     L.x = xb;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { L.u[a] += inc[3 + a]; L.v[a] += inc[6 + a]; }
+    for (int a = 0; a < 3; ++a) { L.u[a] += incu[a]; L.v[a] += incu[3 + a]; }
     L.steps += 1;
     if (!scott_can_step(L, scott_len_lim(f))) {  // length / step limit: class 0, last state
         scott_record(L, e, xbv, L.u, L.v, kClassLimit);

@@ -204,6 +204,21 @@ def hyperbolic(shape, k=0.02, b0=1.0, dtype=np.float64, center=None):
     return tuple(np.asfortranarray(np.array(b, dtype=dtype)) for b in (bx, by, bz))
 
 
+def rotation(shape, omega=0.05, b0=1.0, dtype=np.float64, center=None):
+    """Linear rotating field B = (-omega (y-yc), omega (x-xc), b0) in index
+    units: lines are helices about the axis (xc, yc), and trilinear
+    interpolation reproduces the field exactly, with no kinks at cell faces,
+    so an adaptive controller's error estimate varies smoothly along a line."""
+    nx, ny, nz = shape
+    xc, yc = center if center is not None else ((nx - 1) / 2.0, (ny - 1) / 2.0)
+    i = np.arange(nx, dtype=np.float64).reshape(-1, 1, 1)
+    j = np.arange(ny, dtype=np.float64).reshape(1, -1, 1)
+    bx = np.broadcast_to(-omega * (j - yc), shape)
+    by = np.broadcast_to(omega * (i - xc), shape)
+    bz = np.full(shape, b0)
+    return tuple(np.asfortranarray(np.array(b, dtype=dtype)) for b in (bx, by, bz))
+
+
 def uniform(shape, b, dtype=np.float64):
     return tuple(np.asfortranarray(np.full(shape, float(c), dtype=dtype)) for c in b)
 

@@ -47,6 +47,8 @@ extern "C" {
 /* precision of the tracing arithmetic */
 #define FQSL_PREC_FP32C 0 /* FP32 field/stages, compensated position, FP64 controller and Q */
 #define FQSL_PREC_FP64 1  /* FP64 throughout */
+#define FQSL_PREC_MIXED 2 /* Scott tracer: FP32 Jacobian (grad b, u, v); FP64 position path (b at the
+                             stage points, stage positions, increments, error norm), controller and Q */
 
 /* how lines are mapped to lanes */
 #define FQSL_SCHED_PERSISTENT 0 /* persistent grid, each lane refills from a device queue */
@@ -144,10 +146,15 @@ typedef struct {
 
 /* Trace n seeds (device float arrays x, y, z) with the Scott method and
  * assemble q_0 and q_perp per line.  Replaces a launch of
- * TraceAllBlineScott (fastqslpy/cu/TraceBlineScott.cu:521-595). */
+ * TraceAllBlineScott (fastqslpy/cu/TraceBlineScott.cu:521-595).
+ * FQSL_PREC_FP32C accepts RKF45 tolerances down to 1e-9 cells (its error
+ * norm is formed from float stages, with a round-off floor near 1e-8 x the
+ * step); a smaller tol fails with FQSL_ERR_INVALID -- FQSL_PREC_MIXED and
+ * FQSL_PREC_FP64 resolve any tolerance. */
 int fqsl_trace_scott(const fqsl_field *field, const fqsl_scott_params *params, uint64_t n,
                      const float *x, const float *y, const float *z, const fqsl_scott_out *out,
                      void *stream);
+
 
 /* ----------------------------------------------------------------- batches */
 /* Many Scott requests -- each with its own field, method, step, tolerance,
@@ -326,9 +333,9 @@ int fqsl_curl_index(const void *bx, const void *by, const void *bz, int nx, int 
  * (float, same layout as B); with it clear, twist[] and curB are not read. */
 
 /* TraceAllBlineScott (TraceBlineScott.cu:521-528).  tol_coef scales the
- * per-step tolerance 1e-3 * tol_coef.  precision / schedule select the path;
- * the persistent schedule reads N back to the host (one stream
- * synchronisation) to size its workspace. */
+ * per-step tolerance 1e-3 * tol_coef.  precision (FQSL_PREC_FP32C or
+ * FQSL_PREC_MIXED) / schedule select the path; the persistent schedule reads
+ * N back to the host (one stream synchronisation) to size its workspace. */
 int fqsl_TraceAllBlineScott(float *Bx, float *By, float *Bz, int *BshapeN, float *curB_x,
                             float *curB_y, float *curB_z, double *twist, bool *curB_flag,
                             float *inp_x, float *inp_y, float *inp_z, float *inp_cross_dir,

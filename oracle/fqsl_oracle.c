@@ -6,7 +6,8 @@
  * cpu_baseline leg of bench.py may load it, and only as the checker (or as the
  * timed CPU baseline).  The product path never calls it.
  *
- * Everything is double precision.  The field is three SoA arrays with x
+ * Everything is double precision, except the *_f functions, which restate
+ * the Scott tracer in the reference's own float arithmetic.  The field is three SoA arrays with x
  * fastest: B[k*ny*nx + j*nx + i] (TraceBlineScott.cu:36-39).
  *
  * What follows the reference, function by function:
@@ -18,6 +19,9 @@
  *   scott_rk4         RK4 tableau of TraceBline.cu:226-234 applied to f_scott
  *   check_flag        checkFlag               TraceBlineScott.cu:310-332
  *   trace_scott_line  TraceBlineScott         TraceBlineScott.cu:334-519
+ *   trace_scott_line_f32, interp_vec_f, grad_b_fd_f, scott_rhs_f, scott_rkf45_f
+ *                     the same in the reference's float arithmetic
+ *                                             TraceBlineScott.cu:95-508
  *   plain_rkf45       RKF45                   TraceBlineAdaptive.cu:215-254
  *   plain_rk4         RK4                     TraceBline.cu:226-234
  *   rk4_boundary      RK4_boundary            TraceBline.cu:253-274
@@ -569,6 +573,296 @@ static void trace_scott_line(const field_t *f, const field_t *cur, const double 
     res->steps = steps;
 }
 
+/* ------------------------------------ Scott driver in the reference's float */
+/* The Scott line driver again, in the reference kernel's own arithmetic
+ * (TraceBlineScott.cu:95-508 as written): float field values, positions,
+ * stages, deviation vectors, error norm, step size and controller (powf);
+ * the weight products (1.-r)(1.-r)(1.-r) in double rounded to float, as the
+ * double literals make them; the seven-stencil difference quotient in float
+ * (the +-0.001 offsets formed in double and rounded); arc length in double.
+ * It is compiled without FMA contraction, which nvcc applies by default, so
+ * it reproduces the reference's algorithm and precision, not its bits.  Its
+ * step sequences, evaluation counts and classes are what an FP32 evaluation
+ * of the reference produces.  RKF45 only (the reference's Scott kernel);
+ * the decision register above applies unchanged. */
+typedef struct { float x[3], u[3], v[3]; } state9f;
+
+static float dot3f(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Interp3dxyzn (TraceBlineScott.cu:95-201) in float. */
+static void interp_vec_f(const field_t *f, const float *p, float *out, int normalize) {
+    float r[3], w[8];
+    int idx[3], a, c;
+    long nx = f->n[0], nxy = (long)f->n[0] * f->n[1], base, off[8];
+    for (a = 0; a < 3; ++a) {
+        r[a] = p[a] - floorf(p[a]);
+        idx[a] = (int)floorf(p[a]);
+        if (p[a] <= 0.) { r[a] = 0; idx[a] = 0; }
+        if (p[a] >= f->n[a] - 1) { r[a] = 1; idx[a] = f->n[a] - 2; }
+    }
+    w[0] = (float)((1. - r[0]) * (1. - r[1]) * (1. - r[2]));
+    w[1] = (float)((1. - r[0]) * (1. - r[1]) * (r[2]));
+    w[2] = (float)((1. - r[0]) * (r[1]) * (1. - r[2]));
+    w[3] = (float)((1. - r[0]) * (r[1]) * (r[2]));
+    w[4] = (float)((r[0]) * (1. - r[1]) * (1. - r[2]));
+    w[5] = (float)((r[0]) * (1. - r[1]) * (r[2]));
+    w[6] = (float)((r[0]) * (r[1]) * (1. - r[2]));
+    w[7] = (float)((r[0]) * (r[1]) * (r[2]));
+    base = idx[2] * nxy + idx[1] * nx + idx[0];
+    off[0] = base;                 off[1] = base + nxy;
+    off[2] = base + nx;            off[3] = base + nx + nxy;
+    off[4] = base + 1;             off[5] = base + 1 + nxy;
+    off[6] = base + 1 + nx;        off[7] = base + 1 + nx + nxy;
+    out[0] = (float)f->bx[off[0]] * w[0];
+    out[1] = (float)f->by[off[0]] * w[0];
+    out[2] = (float)f->bz[off[0]] * w[0];
+    for (c = 1; c < 8; ++c) {
+        out[0] = out[0] + (float)f->bx[off[c]] * w[c];
+        out[1] = out[1] + (float)f->by[off[c]] * w[c];
+        out[2] = out[2] + (float)f->bz[off[c]] * w[c];
+    }
+    if (normalize) {
+        float nb = sqrtf(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+        out[0] = out[0] / nb; out[1] = out[1] / nb; out[2] = out[2] / nb;
+    }
+}
+
+/* grad_unit_vec_B (TraceBlineScott.cu:204-226) in float. */
+static void grad_b_fd_f(const field_t *f, const float *p, float J[3][3]) {
+    int j, i;
+    for (j = 0; j < 3; ++j) {
+        float p1[3] = {p[0], p[1], p[2]}, p2[3] = {p[0], p[1], p[2]};
+        float b1[3], b2[3];
+        p1[j] = (float)((double)p1[j] - 0.001);
+        p2[j] = (float)((double)p2[j] + 0.001);
+        interp_vec_f(f, p1, b1, 1);
+        interp_vec_f(f, p2, b2, 1);
+        for (i = 0; i < 3; ++i) J[j][i] = (b2[i] - b1[i]) / 0.002f;
+    }
+}
+
+/* f_scott (TraceBlineScott.cu:230-249) in float. */
+static void scott_rhs_f(const field_t *f, const state9f *y, state9f *dy) {
+    float J[3][3];
+    int i;
+    interp_vec_f(f, y->x, dy->x, 1);
+    grad_b_fd_f(f, y->x, J);
+    for (i = 0; i < 3; ++i) {
+        dy->u[i] = y->u[0] * J[0][i] + y->u[1] * J[1][i] + y->u[2] * J[2][i];
+        dy->v[i] = y->v[0] * J[0][i] + y->v[1] * J[1][i] + y->v[2] * J[2][i];
+    }
+}
+
+/* out = y + (sum_j coef[j] * k[j]), the float9 expression of RKF45_Scott. */
+static void s9f_axpy(state9f *out, const state9f *y, int nk, const float *coef, state9f *const *k) {
+    const float *py = (const float *)y;
+    float *po = (float *)out;
+    int c, j;
+    for (c = 0; c < 9; ++c) {
+        float acc = coef[0] * ((const float *)k[0])[c];
+        for (j = 1; j < nk; ++j) acc = acc + coef[j] * ((const float *)k[j])[c];
+        po[c] = py[c] + acc;
+    }
+}
+
+static void s9f_scale(state9f *k, float s) {
+    float *p = (float *)k;
+    int c;
+    for (c = 0; c < 9; ++c) p[c] = s * p[c];
+}
+
+/* RKF45_Scott (TraceBlineScott.cu:256-290) in float: the tableau rounded to
+ * float, the 5th-order advance, err = |sum ce_i k_i.x| in float. */
+static float scott_rkf45_f(const field_t *f, const state9f *y, state9f *out, float s) {
+    static const float a2[1] = {(float)(1. / 4.)};
+    static const float a3[2] = {(float)(3. / 32.), (float)(9. / 32.)};
+    static const float a4[3] = {(float)(1932. / 2197.), (float)(-7200. / 2197.), (float)(7296. / 2197.)};
+    static const float a5[4] = {(float)(439. / 216.), (float)(-8.), (float)(3680. / 513.), (float)(-845. / 4104.)};
+    static const float a6[5] = {(float)(-8. / 27.), (float)2., (float)(-3544. / 2565.), (float)(1859. / 4104.),
+                                (float)(-11. / 40.)};
+    static const float b5[5] = {(float)(16. / 135.), (float)(6656. / 12825.), (float)(28561. / 56430.),
+                                (float)(-9. / 50.), (float)(2. / 55.)};
+    static const float ce[5] = {(float)(1. / 360.), (float)(-128. / 4275.), (float)(-2197. / 75240.),
+                                (float)(1. / 50.), (float)(2. / 55.)};
+    state9f k1, k2, k3, k4, k5, k6, t;
+    state9f *kk[6] = {&k1, &k2, &k3, &k4, &k5, &k6};
+    state9f *kb[5] = {&k1, &k3, &k4, &k5, &k6};
+    float e[3];
+    int i;
+    scott_rhs_f(f, y, &k1); s9f_scale(&k1, s);
+    s9f_axpy(&t, y, 1, a2, kk); scott_rhs_f(f, &t, &k2); s9f_scale(&k2, s);
+    s9f_axpy(&t, y, 2, a3, kk); scott_rhs_f(f, &t, &k3); s9f_scale(&k3, s);
+    s9f_axpy(&t, y, 3, a4, kk); scott_rhs_f(f, &t, &k4); s9f_scale(&k4, s);
+    s9f_axpy(&t, y, 4, a5, kk); scott_rhs_f(f, &t, &k5); s9f_scale(&k5, s);
+    s9f_axpy(&t, y, 5, a6, kk); scott_rhs_f(f, &t, &k6); s9f_scale(&k6, s);
+    s9f_axpy(out, y, 5, b5, kb);
+    for (i = 0; i < 3; ++i)
+        e[i] = ce[0] * k1.x[i] + ce[1] * k3.x[i] + ce[2] * k4.x[i] + ce[3] * k5.x[i] + ce[4] * k6.x[i];
+    return sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+}
+
+static void normalize3f(float *a) {
+    float n = 1.0f / sqrtf(dot3f(a, a));
+    a[0] *= n; a[1] *= n; a[2] *= n;
+}
+
+/* TraceBlineScott (TraceBlineScott.cu:334-508) in float, RKF45. */
+static void trace_scott_line_f32(const field_t *f, const double *p0d, double s0, double tol, scott_result *res) {
+    const int *n = f->n;
+    const double len_lim = SCOTT_LEN_RATIO * (n[0] + n[1] + n[2]);
+    const float p0[3] = {(float)p0d[0], (float)p0d[1], (float)p0d[2]};
+    const float tolf = (float)tol;
+    float b0[3], B0raw[3], Nb0, u0[3], v0[3];
+    float s = (float)s0;
+    double len = 0.0;
+    long steps = 0, attempts = 0, evals = 0;
+    int dir, flags[2] = {0, 0}, side;
+    state9f ends[2];
+    int aborted = 0, abort_side = -1;
+
+    memset(ends, 0, sizeof(ends));
+    interp_clamped(f, p0d, res->bthis);
+    res->twist = 0.0;
+    if (p0[2] < -1e-8) {                           /* TraceBlineScott.cu:366 */
+        memcpy(res->start, p0d, sizeof(double) * 3);
+        memcpy(res->end, p0d, sizeof(double) * 3);
+        res->flag_start = res->flag_end = -1;
+        res->q0 = res->qperp = NAN;
+        res->len = 0; res->evals = 0; res->steps = 0;
+        return;
+    }
+    interp_vec_f(f, p0, b0, 1);                    /* :368-369 */
+    interp_vec_f(f, p0, B0raw, 0);
+    Nb0 = sqrtf(dot3f(B0raw, B0raw));
+    v0[0] = b0[1]; v0[1] = -b0[0]; v0[2] = 0.0f;  /* :380-386 */
+    {
+        float d = dot3f(v0, b0);
+        v0[0] = v0[0] - d * b0[0]; v0[1] = v0[1] - d * b0[1]; v0[2] = v0[2] - d * b0[2];
+        normalize3f(v0);
+    }
+    u0[0] = b0[1] * v0[2] - b0[2] * v0[1];
+    u0[1] = b0[2] * v0[0] - b0[0] * v0[2];
+    u0[2] = b0[0] * v0[1] - b0[1] * v0[0];
+    normalize3f(u0);
+    for (dir = -1; dir < 2 && !aborted; dir += 2) {
+        state9f a, b;
+        int flag = 0;
+        side = dir < 0 ? 0 : 1;
+        memcpy(a.x, p0, sizeof(a.x));
+        memcpy(a.u, u0, sizeof(a.u));
+        memcpy(a.v, v0, sizeof(a.v));
+        if (p0[2] < 1e-6 && b0[2] * dir < 0) {     /* :396-401 */
+            ends[side] = a; flags[side] = 5;
+            continue;
+        }
+        s = fabsf(s);
+        while (flag == 0 && len < len_lim && steps < SCOTT_STEP_LIM) {
+            float err, scale, dL, d[3];
+            if (attempts >= FQSL_ATTEMPT_CAP) break;
+            ++attempts;
+            err = scott_rkf45_f(f, &a, &b, s * (float)dir);
+            evals += 6;
+            scale = powf((float)(tolf / err / 11.09), 0.2f);      /* :413 */
+            if (scale < 0.618) { s = (float)(s * 0.618); continue; }
+            s = s * scale;
+            if (s > 100.) s = 100.f;
+            if (s < 1. / 10.) s = (float)(1. / 10.);
+            d[0] = b.x[0] - a.x[0]; d[1] = b.x[1] - a.x[1]; d[2] = b.x[2] - a.x[2];
+            dL = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            len += dL;
+            {
+                const double bx[3] = {b.x[0], b.x[1], b.x[2]};
+                flag = check_flag(n, bx);
+            }
+            if (flag > 0) {
+                len -= dL;
+                if (flag <= 6) {
+                    int dim = (flag - 1) / 2;
+                    float p1 = a.x[dim], p2 = b.x[dim];
+                    if (fabsf(p1 - p2) > 0) {
+                        float pm = (flag % 2 == 1) ? 0.0f : (float)n[dim];
+                        float ba[3];
+                        int c;
+                        const float *pa = (const float *)&a, *pb = (const float *)&b;
+                        float *pe = (float *)&ends[side];
+                        for (c = 0; c < 9; ++c) pe[c] = (pa[c] * (p2 - pm) + pb[c] * (pm - p1)) / (p2 - p1);
+                        interp_vec_f(f, a.x, ba, 1);   /* decision register: b at the last in-box state */
+                        len += fabsf(pm - p1) / (1e-4 + fabsf(ba[dim]));
+                    } else {
+                        ends[side] = a;
+                    }
+                    flags[side] = flag;
+                } else {
+                    ends[side] = a;
+                    flags[side] = 42;
+                    aborted = 1;
+                    abort_side = side;
+                }
+            }
+            a = b;
+            ++steps;
+            if (aborted) break;
+        }
+        if (!aborted && flag == 0) {
+            ends[side] = a;
+            flags[side] = 0;
+        }
+    }
+    if (aborted) {
+        if (abort_side == 0) {
+            ends[1].x[0] = p0[0]; ends[1].x[1] = p0[1]; ends[1].x[2] = p0[2];
+            flags[1] = 42;
+        }
+        res->q0 = 0.0; res->qperp = 0.0;
+    } else if (flags[0] == 0 || flags[1] == 0) {
+        res->q0 = NAN; res->qperp = NAN;
+    } else {
+        /* Q assembly (TraceBlineScott.cu:475-508) in float; the 2.0 literal
+         * makes the last term of each numerator double */
+        float Bs[3], Be[3], us1[3], vs1[3], ue1[3], ve1[3], BNs, BNe, nbs, nbe, den;
+        int ds = (flags[0] - 1) / 2, de = (flags[1] - 1) / 2, i;
+        const float *us = ends[0].u, *vs = ends[0].v, *ue = ends[1].u, *ve = ends[1].v;
+        double num;
+        interp_vec_f(f, ends[0].x, Bs, 0);
+        interp_vec_f(f, ends[1].x, Be, 0);
+        BNs = Bs[ds]; BNe = Be[de];
+        for (i = 0; i < 3; ++i) {
+            us1[i] = us[i] - us[ds] / Bs[ds] * Bs[i];
+            vs1[i] = vs[i] - vs[ds] / Bs[ds] * Bs[i];
+            ue1[i] = ue[i] - ue[de] / Be[de] * Be[i];
+            ve1[i] = ve[i] - ve[de] / Be[de] * Be[i];
+        }
+        num = (double)(dot3f(ue1, ue1) * dot3f(vs1, vs1) + dot3f(us1, us1) * dot3f(ve1, ve1)) -
+              2.0 * dot3f(ue1, ve1) * dot3f(us1, vs1);
+        den = fabsf((Nb0 * Nb0) / (BNs * BNe));
+        res->q0 = (float)(fabs(num) / den);
+        nbs = sqrtf(dot3f(Bs, Bs)); nbe = sqrtf(dot3f(Be, Be));
+        {
+            float due = dot3f(ue, Be) / nbe, dve = dot3f(ve, Be) / nbe;
+            float dus = dot3f(us, Bs) / nbs, dvs = dot3f(vs, Bs) / nbs;
+            for (i = 0; i < 3; ++i) {
+                ue1[i] = ue[i] - due * (Be[i] / nbe);
+                ve1[i] = ve[i] - dve * (Be[i] / nbe);
+                us1[i] = us[i] - dus * (Bs[i] / nbs);
+                vs1[i] = vs[i] - dvs * (Bs[i] / nbs);
+            }
+        }
+        num = (double)(dot3f(ue1, ue1) * dot3f(vs1, vs1) + dot3f(us1, us1) * dot3f(ve1, ve1)) -
+              2.0 * dot3f(ue1, ve1) * dot3f(us1, vs1);
+        den = (Nb0 * Nb0) / (nbs * nbe);
+        res->qperp = (float)(fabs(num) / den);
+    }
+    for (side = 0; side < 3; ++side) {
+        res->start[side] = ends[0].x[side];
+        res->end[side] = ends[1].x[side];
+    }
+    res->flag_start = flags[0];
+    res->flag_end = flags[1];
+    res->len = len;
+    res->evals = evals;
+    res->steps = steps;
+}
+
 /* ------------------------------------------------------- plain tracer */
 /* RKF45 (TraceBlineAdaptive.cu:215-254) on the position only. */
 static double plain_rkf45(const field_t *f, const double *p, double *out, double s) {
@@ -877,6 +1171,33 @@ int orc_trace_scott(const double *bx, const double *by, const double *bz, int nx
         scott_result r;
         trace_scott_line(&f, jx ? &cur : NULL, seeds + 3 * i, method, s0, tol, grad_mode, &r);
         if (twist) twist[i] = r.twist;
+        memcpy(start + 3 * i, r.start, sizeof(r.start));
+        memcpy(end + 3 * i, r.end, sizeof(r.end));
+        flag_start[i] = r.flag_start;
+        flag_end[i] = r.flag_end;
+        q0[i] = r.q0;
+        qperp[i] = r.qperp;
+        len[i] = r.len;
+        memcpy(bthis + 3 * i, r.bthis, sizeof(r.bthis));
+        if (evals) evals[i] = r.evals;
+        if (steps) steps[i] = r.steps;
+    }
+    return 0;
+}
+
+/* The Scott tracer in the reference kernel's float arithmetic
+ * (trace_scott_line_f32), RKF45 only, over n seeds. */
+int orc_trace_scott_f32(const double *bx, const double *by, const double *bz, int nx, int ny, int nz,
+                        long nseeds, const double *seeds, double s0, double tol, int nthreads,
+                        double *start, double *end, int *flag_start, int *flag_end,
+                        double *q0, double *qperp, double *len, double *bthis, long *evals, long *steps) {
+    field_t f = {bx, by, bz, {nx, ny, nz}, NULL};
+    long i;
+    if (nx < 2 || ny < 2 || nz < 2) return -1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads_or_default(nthreads))
+    for (i = 0; i < nseeds; ++i) {
+        scott_result r;
+        trace_scott_line_f32(&f, seeds + 3 * i, s0, tol, &r);
         memcpy(start + 3 * i, r.start, sizeof(r.start));
         memcpy(end + 3 * i, r.end, sizeof(r.end));
         flag_start[i] = r.flag_start;

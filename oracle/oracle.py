@@ -46,6 +46,10 @@ def lib():
                                       ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       _dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _lp, _lp,
                                       _dp, _dp, _dp, _dp]
+        L.orc_trace_scott_f32.restype = ctypes.c_int
+        L.orc_trace_scott_f32.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_long, _dp, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, _dp, _dp, _ip, _ip, _dp, _dp, _dp, _dp, _lp, _lp]
         L.orc_trace_plain.restype = ctypes.c_int
         L.orc_trace_plain.argtypes = [_dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_long, _dp, _dp, ctypes.c_int, ctypes.c_double,
@@ -168,6 +172,28 @@ def trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6, grad_mode=0, nt
                                _ptr(out["b_this"]), _ptr(out["evals"], _lp),
                                _ptr(out["steps"], _lp), _optr(cur[0]), _optr(cur[1]), _optr(cur[2]),
                                _ptr(out["twist"]) if current is not None else None)
+    if rc != 0:
+        raise ValueError("oracle rejected the field shape")
+    return out
+
+
+def trace_scott_f32(field, seeds, s0=0.25, tol=1e-6, nthreads=0):
+    """The Scott tracer in the reference kernel's own float arithmetic
+    (TraceBlineScott.cu:95-508), RKF45: a model of what the FP32 reference
+    produces (its step sequences, evaluation counts and classes), not of its
+    bits.  Same outputs as ``trace_scott`` (no twist)."""
+    (bx, by, bz), shape = _flat_field(*field)
+    s = _seeds(seeds)
+    n = s.shape[0]
+    out = dict(start=np.empty((n, 3)), end=np.empty((n, 3)),
+               flag_start=np.empty(n, np.int32), flag_end=np.empty(n, np.int32),
+               q0=np.empty(n), qperp=np.empty(n), length=np.empty(n),
+               b_this=np.empty((n, 3)), evals=np.empty(n, np.int64), steps=np.empty(n, np.int64))
+    rc = lib().orc_trace_scott_f32(_ptr(bx), _ptr(by), _ptr(bz), *shape, n, _ptr(s), float(s0), float(tol),
+                                   int(nthreads), _ptr(out["start"]), _ptr(out["end"]),
+                                   _ptr(out["flag_start"], _ip), _ptr(out["flag_end"], _ip),
+                                   _ptr(out["q0"]), _ptr(out["qperp"]), _ptr(out["length"]),
+                                   _ptr(out["b_this"]), _ptr(out["evals"], _lp), _ptr(out["steps"], _lp))
     if rc != 0:
         raise ValueError("oracle rejected the field shape")
     return out

@@ -33,12 +33,12 @@ std::vector<float4> build_cells(const float* bx, const float* by, const float* b
     return cells;
 }
 
-template <typename R, class F, class FJ = NoCurrent>
+template <typename R, class F, class FJ = NoCurrent, typename X = R>
 void run_scott(const F& f, const ScottParams& P, long n, const float* seeds, double* out_start, double* out_end,
                int* fs, int* fe, double* q0, double* qp, double* len, long* evals, const FJ& fj = FJ{},
                double* twist = nullptr) {
     for (long i = 0; i < n; ++i) {
-        ScottLine<R> L;
+        ScottLine<R, X> L;
         ScottEnd<R> e;
         int st = scott_init<R>(L, f, P, seeds[3 * i], seeds[3 * i + 1], seeds[3 * i + 2], e);
         while (st == kTracing) st = scott_attempt<R, F, FJ>(L, f, P, e, fj);
@@ -57,7 +57,8 @@ void run_scott(const F& f, const ScottParams& P, long n, const float* seeds, dou
 }  // namespace
 
 extern "C" {
-// precision 0: float + cell layout from the float field; 1: double SoA.
+// precision 0: float + cell layout from the float field; 1: double SoA;
+// 2: the mixed variant (FP64 position path, FP32 Jacobian) on the cell layout.
 // jx/jy/jz (may be NULL): current density for the twist output.
 int hc_trace_scott(const double* bx, const double* by, const double* bz, int nx, int ny, int nz, int precision,
                    int method, double s0, double tol, long n, const float* seeds, double* start, double* end,
@@ -81,7 +82,13 @@ int hc_trace_scott(const double* bx, const double* by, const double* bz, int nx,
     if (jx) {
         std::vector<float> gx(jx, jx + nn), gy(jy, jy + nn), gz(jz, jz + nn);
         FieldSoA<float> fj{gx.data(), gy.data(), gz.data(), {nx, ny, nz}};
-        run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals, fj, twist);
+        if (precision == 2)
+            run_scott<float, FieldCell32, FieldSoA<float>, double>(f, P, n, seeds, start, end, fs, fe, q0, qp, len,
+                                                                   evals, fj, twist);
+        else
+            run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals, fj, twist);
+    } else if (precision == 2) {
+        run_scott<float, FieldCell32, NoCurrent, double>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
     } else {
         run_scott<float>(f, P, n, seeds, start, end, fs, fe, q0, qp, len, evals);
     }

@@ -54,3 +54,17 @@ def summary(d, cls, mask=None):
                 p999=float(np.quantile(fin, 0.999)) if fin.size else 0.0,
                 median=float(np.median(fin)) if fin.size else 0.0,
                 max=float(fin.max()) if fin.size else 0.0)
+
+
+def ulp_moved(seeds, hi, rng):
+    """Seeds with x and y each moved by one float32 ulp, up or down at random
+    (default_rng stream `rng`), kept inside [0, hi]; z unchanged.  The oracle
+    traced from them measures how far round-off of float32 size alone moves
+    each line's class and Q: the floor an FP32 path is held to."""
+    s = np.array(seeds, np.float32, copy=True)
+    for a in (0, 1):
+        up = rng.random(s.shape[0]) < 0.5
+        moved = np.where(up, np.nextafter(s[:, a], np.float32(np.inf)),
+                         np.nextafter(s[:, a], np.float32(-np.inf)))
+        s[:, a] = np.clip(moved, np.float32(0), np.float32(hi[a]))
+    return s

@@ -74,7 +74,8 @@ def test_constants_match_the_header():
     assert mac["FQSL_OK"] == _lib.FQSL_OK
     assert (mac["FQSL_DTYPE_F32"], mac["FQSL_DTYPE_F64"]) == (_lib.DTYPE_F32, _lib.DTYPE_F64)
     assert {"rkf45": mac["FQSL_METHOD_RKF45"], "rk4": mac["FQSL_METHOD_RK4"]} == _lib.METHODS
-    assert {"fp32c": mac["FQSL_PREC_FP32C"], "fp64": mac["FQSL_PREC_FP64"]} == _lib.PRECISIONS
+    assert {"fp32c": mac["FQSL_PREC_FP32C"], "fp64": mac["FQSL_PREC_FP64"],
+            "mixed": mac["FQSL_PREC_MIXED"]} == _lib.PRECISIONS
     assert {"persistent": mac["FQSL_SCHED_PERSISTENT"],
             "per_line": mac["FQSL_SCHED_PER_LINE"]} == _lib.SCHEDULES
     assert (mac["FQSL_LAYOUT_SOA32"], mac["FQSL_LAYOUT_SOA64"], mac["FQSL_LAYOUT_CELL32"],

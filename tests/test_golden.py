@@ -1,7 +1,7 @@
 """Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py).
 
 CPU: the oracle, rebuilt from source, still produces them (drift pin).
-GPU: both device precisions against them, over the lines the fixture marks
+GPU: the device precisions against them, over the lines the fixture marks
 stable (same classes and Q / footpoints under +-1e-6-cell seed moves).
 """
 import glob
@@ -90,17 +90,42 @@ def _np(r):
     return out
 
 
+# (fixture, device precision): both precisions for every fixture, and the
+# mixed precision (a Scott-tracer mode) for the Scott fixtures
+GPU_CASES = [(n, p) for n in TRACE_CASES for p in ("fp64", "fp32c", "mixed")
+             if p != "mixed" or n.startswith("scott")]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["fp64", "fp32c"])
-@pytest.mark.parametrize("name", TRACE_CASES)
-def test_gpu_matches_fixture(fq, name, precision):
+def test_fp32c_refuses_a_tolerance_below_its_error_floor(fq):
+    """The converged fixture's tol 1e-10 is below what an error norm formed
+    from float stages resolves (~1e-8 x step): FP32C refuses it with a clear
+    error instead of stalling at the step's lower clamp; the mixed and FP64
+    precisions trace it (test_gpu_matches_fixture)."""
+    d = load("scott_multipole64_converged")
+    m = d["meta"]
+    kw = dict(m["tracer_kwargs"])
+    kw["step"] = kw.pop("s0")
+    assert kw["tol"] < 1e-9
+    s = d["seeds"][:64]
+    with fq.Field(*field_of(m)) as f:
+        with pytest.raises(fq.FqslError, match="below what the FP32 error norm resolves"):
+            fq.trace_scott(f, s[:, 0], s[:, 1], s[:, 2], precision="fp32c", **kw)
+        kw["tol"] = 1e-9                                  # the smallest it accepts
+        r = fq.trace_scott(f, s[:, 0], s[:, 1], s[:, 2], precision="fp32c", **kw)
+        assert int((r["flag_end"] > 0).sum()) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,precision", GPU_CASES)
+def test_gpu_matches_fixture(fq, oracle, name, precision):
     d = load(name)
     m = d["meta"]
     kw = dict(m["tracer_kwargs"])
     kw["step"] = kw.pop("s0")
     s = d["seeds"]
     if name.endswith("_converged") and precision == "fp32c":
-        pytest.skip("tol 1e-10 is below the FP32 error estimate's round-off floor")
+        kw["tol"] = 1e-9   # its smallest tolerance (test_fp32c_refuses_a_tolerance_below_its_error_floor)
     print(f"fixture {name} precision {precision}", flush=True)
     with fq.Field(*field_of(m)) as f:
         if m["tracer"] == "scott":
@@ -117,6 +142,13 @@ def test_gpu_matches_fixture(fq, name, precision):
             assert cls[st].all() and dq[st].max() <= 1e-5
         elif rk4:
             assert cls[st].mean() >= 0.995 and np.mean(dq[st] <= 1e-3) >= 0.99
+        elif name.endswith("_converged") and precision == "fp32c":
+            # tol 1e-9 against the fixture's 1e-10: as close as the oracle's own 1e-9 result
+            r9 = oracle.trace_scott(field_of(m), s, **dict(m["tracer_kwargs"], tol=1e-9))
+            dq9 = parity.log_q_diff(r9["q0"], d["q0"])
+            print("oracle at tol 1e-9:", parity.summary(dq9, cls, st))
+            assert cls[st].mean() >= 0.99
+            assert np.median(dq[st & cls]) <= 1.2 * np.median(dq9[st & cls]) + 1e-6
         elif name.endswith("_converged"):
             assert cls[st].mean() >= 0.99 and np.median(dq[st & cls]) <= 1e-5
         else:

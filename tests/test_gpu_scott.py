@@ -1,18 +1,24 @@
 """GPU parity of the Scott tracer against the FP64 oracle.
 
-Bars (per line, over lines whose oracle result is well conditioned -- stable
-under a perturbation far below float32 resolution, see parity.conditioned):
-
-* RK4 fixed step (the quadrupole workload, 64^3 float64 field, 256^2 seeds):
-  FP64 path -- classes identical, |dlog10 Q| <= 1e-5 (float32 output);
-  FP32-compensated path -- classes match on >= 99.99 %, |dlog10 Q| <= 1e-3
-  on >= 99.9 %.
-* RKF45 (random multipole, 256^3 float32, tol 1e-6): the reference's
-  controller forks on round-off (the oracle's own Q moves by > 1e-3 on about
-  a third of the lines when its tolerance is scaled by 1 + 1e-12), so a
-  line-by-line 1e-3 bar cannot hold for any other implementation.  The GPU
-  must be as accurate as the oracle against the converged answer (tol 1e-10)
-  and match its classes within the oracle's own round-off sensitivity.
+* RK4 fixed step (the quadrupole workload, 64^3 float64 field, 256^2 seeds),
+  on every pixel off the field's two mirror planes x = y and x + y = 63
+  (65,024 of 65,536): a line seeded in a mirror plane stays in it in exact
+  arithmetic and runs into the central null, so round-off of any size
+  decides its fate.  FP64 path -- classes identical, |dlog10 Q| <= 1e-5
+  (float32 output), the oracle's evaluation counts; FP32-compensated and
+  mixed paths -- classes match on >= 99.99 %, |dlog10 Q| <= 1e-3 on
+  >= 99.9 %.
+* RKF45 (random multipole, 256^3 float32, tol 1e-6, 65,536 seeds of the
+  1024^2 plane): the reference's controller forks on round-off, so every
+  bar is relative to the oracle's own spread, measured on the same lines.
+  Mixed and FP64 paths (the FP64 controller on an FP64 error norm): class
+  mismatches against the oracle at most the oracle's own under one-float32-
+  ulp seed moves plus 1e-4 of the lines; as close to the converged Q (tol
+  1e-10) as the oracle, p99.9 included.  FP32-compensated path (error norm
+  from float stages, whose round-off floor restrains the step growth): class
+  mismatches against the oracle at most those of the reference's own float
+  arithmetic (the oracle's float model) plus 1e-4 of the lines, and against
+  the converged answer no less accurate than the oracle.
 """
 import numpy as np
 import pytest
@@ -48,9 +54,10 @@ def c1(oracle):
     X, Y = np.meshgrid(g, g)
     seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]).astype(np.float32)
     ref = oracle.trace_scott(field, seeds, method="rk4", s0=0.25)
-    refp = [oracle.trace_scott(field, seeds.astype(np.float64) + sgn * np.array([1e-6, -1e-6, 0.0]),
-                               method="rk4", s0=0.25) for sgn in (1.0, -1.0)]
-    cond = parity.conditioned(ref, *refp)
+    # off the mirror planes x = y and x + y = 63 of the alternating quadrupole
+    row, col = np.divmod(np.arange(seeds.shape[0]), 256)
+    cond = (row != col) & (row + col != 255)
+    assert cond.sum() == 65536 - 512
     return field, seeds, ref, cond
 
 
@@ -60,8 +67,6 @@ def _trace(fq, f, seeds, **kw):
 
 def test_c1_fp64_matches_oracle(fq, c1):
     field, seeds, ref, cond = c1
-    print(f"well-conditioned lines: {cond.mean():.5f}")
-    assert cond.mean() > 0.99
     with fq.Field(*field) as f:
         for sched in ("per_line", "persistent"):
             r = _trace(fq, f, seeds, method="rk4", step=0.25, precision="fp64", schedule=sched)
@@ -81,14 +86,15 @@ def test_c1_fp64_matches_oracle(fq, c1):
             np.testing.assert_allclose(bt, ref["b_this"], rtol=1e-5, atol=1e-6)
 
 
-def test_c1_fp32c_parity(fq, c1):
+@pytest.mark.parametrize("precision", ["fp32c", "mixed"])
+def test_c1_fp32_parity(fq, c1, precision):
     field, seeds, ref, cond = c1
     with fq.Field(*field) as f:
-        r = _trace(fq, f, seeds, method="rk4", step=0.25, precision="fp32c", schedule="persistent")
+        r = _trace(fq, f, seeds, method="rk4", step=0.25, precision=precision, schedule="persistent")
     cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
     d = parity.log_q_diff(r["q_0"], ref["q0"])
     s = parity.summary(d, cls, cond)
-    print("fp32c conditioned:", s, "all:", parity.summary(d, cls))
+    print(precision, "off the mirror planes:", s, "all:", parity.summary(d, cls))
     assert s["class_match"] >= 0.9999
     assert s["frac_le_1e3"] >= 0.999
     end = np.column_stack([r["end_x"], r["end_y"], r["end_z"]])
@@ -111,53 +117,129 @@ def test_schedules_bitwise_identical(fq, c1):
                 assert np.array_equal(a[k], c[k][inv], equal_nan=True), (prec, k)
 
 
+C2_LINES = 1 << 16
+
+
 @pytest.fixture(scope="module")
 def c2(oracle):
+    """65,536 seeds drawn from the 1024^2 z = 0 plane of the random multipole;
+    the oracle at tol 1e-6, again from seeds moved by one float32 ulp, in its
+    float model, and converged (tol 1e-10)."""
     field = synth.multipole_256(seed=0, device="cuda")
     n = 1024
     rng = np.random.default_rng(0)
-    ij = rng.choice(n * n, size=8192, replace=False)
+    ij = rng.choice(n * n, size=C2_LINES, replace=False)
     g = np.linspace(0, 255, n, dtype=np.float32)
     seeds = np.column_stack([g[ij % n], g[ij // n], np.zeros(ij.size, np.float32)]).astype(np.float32)
-    ref = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6)
-    refp = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-6 * (1 + 1e-12))
-    truth = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=1e-10)
-    return field, seeds, ref, refp, truth
+    kw = dict(method="rkf45", s0=0.25)
+    ref = oracle.trace_scott(field, seeds, tol=1e-6, **kw)
+    ulp = oracle.trace_scott(field, parity.ulp_moved(seeds, (255, 255), np.random.default_rng(5)), tol=1e-6, **kw)
+    f32 = oracle.trace_scott_f32(field, seeds, s0=0.25, tol=1e-6)
+    truth = oracle.trace_scott(field, seeds, tol=1e-10, **kw)
+    return field, seeds, ref, ulp, f32, truth
 
 
-def test_c2_rkf45_as_accurate_as_the_oracle(fq, c2):
-    """RKF45 at tol 1e-6: the GPU's Q is as close to the converged Q (tol
-    1e-10) as the FP64 oracle's own tol-1e-6 result is, and its classes match
-    the oracle's within the oracle's own round-off sensitivity."""
-    field, seeds, ref, refp, truth = c2
-    dfl = parity.log_q_diff(refp["q0"], ref["q0"])
-    cfl = parity.class_match(refp["flag_start"], refp["flag_end"], ref["flag_start"], ref["flag_end"])
-    floor = parity.summary(dfl, cfl)
-    same = parity.class_match(ref["flag_start"], ref["flag_end"], truth["flag_start"], truth["flag_end"])
-    err_orc = parity.log_q_diff(ref["q0"], truth["q0"])
-    print("oracle vs perturbed oracle:", floor)
-    print("oracle(1e-6) vs converged:", parity.summary(err_orc, same))
+def _mismatches(a, b):
+    fs_a = a["flag_start"]
+    fe_a = a["flag_end"]
+    return int((~parity.class_match(fs_a, fe_a, b["flag_start"], b["flag_end"])).sum())
+
+
+def _vs_truth(q, flags, truth):
+    """|dlog10 Q| against the converged answer on the lines whose classes match it."""
+    cls = parity.class_match(flags[0], flags[1], truth["flag_start"], truth["flag_end"])
+    d = parity.log_q_diff(q, truth["q0"])[cls]
+    d = d[np.isfinite(d)]
+    return int((~cls).sum()), float(np.median(d)), float(np.quantile(d, 0.999))
+
+
+def _gpu_c2(fq, field, seeds, precision):
     with fq.Field(*field) as f:
-        for prec in ("fp64", "fp32c"):
-            r = _trace(fq, f, seeds, method="rkf45", step=0.25, tol=1e-6, precision=prec)
-            cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
-            d = parity.log_q_diff(r["q_0"], ref["q0"])
-            print(prec, "vs oracle:", parity.summary(d, cls))
-            cls_t = parity.class_match(r["flag_start"], r["flag_end"], truth["flag_start"], truth["flag_end"])
-            err_gpu = parity.log_q_diff(r["q_0"], truth["q0"])
-            m = same & cls_t & np.isfinite(err_orc) & np.isfinite(err_gpu)
-            print(prec, "vs converged:", parity.summary(err_gpu, cls_t),
-                  "median ratio %.3f" % (np.median(err_gpu[m]) / np.median(err_orc[m])))
-            assert cls.mean() >= floor["class_match"] - 2e-3
-            assert np.median(err_gpu[m]) <= 1.5 * np.median(err_orc[m]) + 1e-6
-            assert np.quantile(err_gpu[m], 0.99) <= 1.5 * np.quantile(err_orc[m], 0.99) + 1e-6
-            # The FP32 error estimate has a round-off floor (~1e-8 cells), so
-            # when the true error is far below the tolerance the controller
-            # grows the step less and wastes fewer rejected attempts: fewer
-            # evaluations for the same accuracy against the converged answer.
-            ev = r["evals"].astype(np.float64).sum() / ref["evals"].sum()
-            print(prec, "evals ratio vs oracle %.4f" % ev)
-            assert (0.95 < ev < 1.05) if prec == "fp64" else (0.6 < ev < 1.1)
+        r = _trace(fq, f, seeds, method="rkf45", step=0.25, tol=1e-6, precision=precision,
+                   schedule="per_line" if precision == "fp64" else "persistent")
+    r["q0"] = r["q_0"]
+    return r
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_c2_class_parity_within_the_oracles_own_spread(fq, c2, precision):
+    """Paths that run the FP64 controller on an FP64 error norm: within the
+    oracle's one-ulp spread plus 1e-4 of the lines, and as close to the
+    converged answer."""
+    field, seeds, ref, ulp, f32, truth = c2
+    N = seeds.shape[0]
+    floor = _mismatches(ulp, ref)
+    r = _gpu_c2(fq, field, seeds, precision)
+    mis = _mismatches(r, ref)
+    d = parity.log_q_diff(r["q_0"], ref["q0"])
+    d_ulp = parity.log_q_diff(ulp["q0"], ref["q0"])
+    t_orc = _vs_truth(ref["q0"], (ref["flag_start"], ref["flag_end"]), truth)
+    t_gpu = _vs_truth(r["q_0"], (r["flag_start"], r["flag_end"]), truth)
+    print(f"{precision}: class mismatches {mis} (one-ulp floor {floor}, budget {floor + 1e-4 * N:.1f}); "
+          f"within 1e-3 {np.mean(d <= 1e-3):.4f} (floor {np.mean(d_ulp <= 1e-3):.4f}); vs converged "
+          f"mismatches/median/p99.9 {t_gpu} (oracle {t_orc})")
+    assert mis <= floor + 1e-4 * N
+    assert np.mean(d <= 1e-3) >= np.mean(d_ulp <= 1e-3) - 0.02
+    assert t_gpu[0] <= t_orc[0] + 1e-4 * N
+    assert t_gpu[1] <= 1.2 * t_orc[1] + 1e-6
+    assert t_gpu[2] <= 1.25 * t_orc[2] + 1e-3
+    ev = r["evals"].astype(np.float64).sum() / ref["evals"].sum()
+    print(precision, "evals ratio vs oracle %.4f" % ev)
+    assert 0.98 < ev < 1.02
+
+
+def test_c2_fp32c_accuracy(fq, c2):
+    """The FP32-compensated path: its error norm is formed from float stages,
+    whose round-off floor (~1e-8 x the step) restrains the step growth the
+    FP64 controller takes when the true error is tiny, so its step sequences
+    are the reference's float arithmetic's, not the FP64 oracle's.  Against
+    the oracle: no further than the reference's own float arithmetic (the
+    oracle's float model) plus 1e-4 of the lines; against the converged
+    answer: at least as accurate as the oracle, p99.9 included."""
+    field, seeds, ref, ulp, f32, truth = c2
+    N = seeds.shape[0]
+    r = _gpu_c2(fq, field, seeds, "fp32c")
+    mis = _mismatches(r, ref)
+    mis_f32 = _mismatches(f32, ref)
+    t_orc = _vs_truth(ref["q0"], (ref["flag_start"], ref["flag_end"]), truth)
+    t_gpu = _vs_truth(r["q_0"], (r["flag_start"], r["flag_end"]), truth)
+    d = parity.log_q_diff(r["q_0"], ref["q0"])
+    d_f32 = parity.log_q_diff(f32["q0"], ref["q0"])
+    print(f"fp32c: class mismatches vs oracle {mis} (float model {mis_f32}, one-ulp floor {_mismatches(ulp, ref)}); "
+          f"within 1e-3 {np.mean(d <= 1e-3):.4f} (float model {np.mean(d_f32 <= 1e-3):.4f}); vs converged "
+          f"mismatches/median/p99.9 {t_gpu} (oracle {t_orc})")
+    assert mis <= mis_f32 + 1e-4 * N
+    assert np.mean(d <= 1e-3) >= np.mean(d_f32 <= 1e-3)
+    assert t_gpu[0] <= t_orc[0] + 1e-4 * N
+    assert t_gpu[1] <= 1.2 * t_orc[1] + 1e-6
+    assert t_gpu[2] <= 1.25 * t_orc[2] + 1e-3
+    ev = r["evals"].astype(np.float64).sum() / ref["evals"].sum()
+    print("fp32c evals ratio vs oracle %.4f (float model %.4f)" % (ev, f32["evals"].sum() / ref["evals"].sum()))
+    assert 0.6 < ev < 1.1
+
+
+def test_rkf45_line_by_line_where_the_controller_cannot_fork(fq, oracle):
+    """A linear rotating field (reproduced exactly by the interpolant, no
+    kinks at cell faces): the FP64 path takes the oracle's RKF45 step
+    sequence on every line -- the same evaluation count -- and its Q."""
+    field = synth.rotation((40, 40, 48), omega=0.05, b0=1.0)
+    rng = np.random.default_rng(3)
+    n = 4000
+    seeds = np.column_stack([rng.uniform(8, 31, n), rng.uniform(8, 31, n),
+                             np.r_[np.zeros(n // 2), rng.uniform(0, 40, n - n // 2)]]).astype(np.float32)
+    with fq.Field(*field) as f:
+        for tol in (1e-6, 1e-8):
+            ref = oracle.trace_scott(field, seeds, method="rkf45", s0=0.25, tol=tol)
+            for sched in ("per_line", "persistent"):
+                r = _trace(fq, f, seeds, method="rkf45", step=0.25, tol=tol, precision="fp64", schedule=sched)
+                np.testing.assert_array_equal(r["evals"].astype(np.int64), ref["evals"])
+                assert np.array_equal(r["flag_end"], ref["flag_end"])
+                d = parity.log_q_diff(r["q_0"], ref["q0"])
+                print(f"tol {tol:g} {sched}: |dlog10 Q| median {np.median(d):.2e} max {d.max():.2e}")
+                # q_0 is written as float32: its rounding (~3e-8 in log10) is the floor here;
+                # the FP64 line length (the sum of the accepted steps) pins the sequence tighter
+                assert d.max() <= 1e-7
+                np.testing.assert_allclose(r["line_len"], ref["length"], rtol=1e-9)
 
 
 def test_hyperbolic_closed_form_gpu(fq):

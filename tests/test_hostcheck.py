@@ -157,3 +157,71 @@ def test_plain_tracer_edge_seeds(oracle, b0, method):
             np.testing.assert_allclose(r["end"], ref["end"], rtol=0, atol=1e-7)
             np.testing.assert_allclose(r["start"], ref["start"], rtol=0, atol=1e-7)
             np.testing.assert_array_equal(r["evals"], ref["evals"])
+
+
+ROT_TOLS = (1e-6, 1e-8)
+
+
+@pytest.fixture(scope="module")
+def rotation_case():
+    """RKF45 where the controller cannot fork: a linear rotating field, which
+    the trilinear interpolant reproduces exactly, with no kinks at cell
+    faces, so the error estimate varies smoothly along every line and stays
+    far above round-off; seeds on z = 0 and inside the box."""
+    f = synth.rotation((40, 40, 48), omega=0.05, b0=1.0)
+    rng = np.random.default_rng(3)
+    n = 2000
+    seeds = np.column_stack([rng.uniform(8, 31, n), rng.uniform(8, 31, n),
+                             np.r_[np.zeros(n // 2), rng.uniform(0, 40, n - n // 2)]]).astype(np.float32)
+    return f, seeds
+
+
+@pytest.mark.parametrize("tol", ROT_TOLS)
+def test_fp64_rkf45_line_by_line(oracle, rotation_case, tol):
+    """The FP64 build takes the oracle's RKF45 step sequence on every line
+    (accepted and rejected attempts: the same evaluation count) and lands on
+    the same Q."""
+    f, seeds = rotation_case
+    ref = oracle.trace_scott(f, seeds, method="rkf45", s0=0.25, tol=tol)
+    r = hc.trace_scott(f, seeds, precision=1, method=0, s0=0.25, tol=tol)
+    np.testing.assert_array_equal(r["evals"], ref["evals"])
+    assert np.array_equal(r["flag_end"], ref["flag_end"]) and np.array_equal(r["flag_start"], ref["flag_start"])
+    d = parity.log_q_diff(r["q0"], ref["q0"])
+    assert np.median(d) <= 1e-9 and d.max() <= 1e-8, (np.median(d), d.max())
+
+
+def test_mixed_tracer_follows_the_oracles_controller(oracle):
+    """The mixed variant (FP64 position path, FP32 Jacobian) runs the FP64
+    controller on an FP64 error norm: on a random multipole its evaluation
+    counts follow the oracle's (the FP32 path's do not: its error estimate's
+    round-off floor restrains step growth) and its Q is as close to the
+    oracle's as the FP64 path."""
+    f = synth.multipole((64, 64, 64), seed=4, device="cpu")
+    rng = np.random.default_rng(9)
+    seeds = np.column_stack([rng.uniform(0, 63, 3000), rng.uniform(0, 63, 3000), np.zeros(3000)]).astype(np.float32)
+    ref = oracle.trace_scott(f, seeds, method="rkf45", s0=0.25, tol=1e-6)
+    mixed = hc.trace_scott(f, seeds, precision=2, method=0, s0=0.25, tol=1e-6)
+    fp32 = hc.trace_scott(f, seeds, precision=0, method=0, s0=0.25, tol=1e-6)
+    ev_m = mixed["evals"].sum() / ref["evals"].sum()
+    ev_f = fp32["evals"].sum() / ref["evals"].sum()
+    print("evals vs oracle: mixed %.4f fp32c %.4f" % (ev_m, ev_f))
+    assert 0.99 < ev_m < 1.01
+    assert np.mean(mixed["evals"] == ref["evals"]) > 0.5
+    cls = parity.class_match(mixed["flag_start"], mixed["flag_end"], ref["flag_start"], ref["flag_end"])
+    d = parity.log_q_diff(mixed["q0"], ref["q0"])
+    assert cls.mean() >= 0.995 and np.median(d[cls]) < 1e-4
+
+
+def test_mixed_tracer_closed_form():
+    """Q = 2 cosh(2 k N / B0) on the linear X-point field through the mixed
+    variant (RK4 and RKF45)."""
+    shape, k = (41, 41, 33), 0.03
+    f = synth.hyperbolic(shape, k=k, b0=1.0)
+    g = np.linspace(14, 26, 9, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]).astype(np.float32)
+    for method, tol in ((1, 0.0), (0, 1e-10)):   # RKF45 at a tolerance FP32 could not resolve
+        r = hc.trace_scott(f, seeds, precision=2, method=method, s0=0.25, tol=tol)
+        top = (r["flag_end"] == 6) & np.isfinite(r["q0"])   # the seed on the axis has no frame (NaN)
+        assert top.mean() > 0.9
+        np.testing.assert_allclose(r["q0"][top], 2 * np.cosh(2 * k * shape[2]), rtol=2e-3)

@@ -296,3 +296,57 @@ def test_edge_cases(oracle):
     assert r["flag_start"][0] == 0 and r["flag_end"][0] == 0 and np.isnan(r["q0"][0])
     assert r["length"][0] >= 2 * sum(shape)
     np.testing.assert_allclose(r["end"][0], [xc + 4, yc, 5.0])    # never traced forward
+
+
+# ------------------------------------------ the reference-precision (float) Scott tracer
+def test_f32_model_closed_form_and_uniform(oracle):
+    """The float restatement of TraceBlineScott (trace_scott_f32): Q = 2 cosh(2kN/B0)
+    on the linear X-point field, and Q = 2 with the closed-form exit points
+    of a uniform oblique field, to float accuracy."""
+    from fastqsl_amd import synth
+    shape, k = (41, 41, 33), 0.03
+    f = synth.hyperbolic(shape, k=k, b0=1.0)
+    g = np.linspace(14, 26, 9, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]).astype(np.float32)
+    r = oracle.trace_scott_f32(f, seeds, s0=0.25, tol=1e-6)
+    # the seed on the axis (b0 || z) has no frame: Q = NaN there (decision register)
+    top = (r["flag_end"] == 6) & np.isfinite(r["q0"])
+    assert top.mean() > 0.9 and (r["flag_start"] == 5).all()
+    # the float seven-stencil quotient carries ~1e-4 relative noise per step
+    rel = np.abs(r["q0"][top] / (2 * np.cosh(2 * k * shape[2])) - 1)
+    assert np.median(rel) < 2e-3 and rel.max() < 1e-2
+    # closed-form footpoints: x - xc = (x0 - xc) exp(k z / b0) at z = N (the landing plane)
+    xc = (shape[0] - 1) / 2.0
+    np.testing.assert_allclose(r["end"][top, 0] - xc, (seeds[top, 0] - xc) * np.exp(k * shape[2]), rtol=1e-2)
+    np.testing.assert_allclose(r["end"][top, 2], shape[2], atol=1e-5)
+    u = synth.uniform((20, 20, 20), (0.3, -0.2, 1.0))
+    su = np.array([[5.0, 7.0, 0.0], [10.0, 12.5, 0.0], [3.0, 15.0, 0.0]], np.float32)
+    ru = oracle.trace_scott_f32(u, su, s0=0.25, tol=1e-6)
+    assert (ru["flag_end"] == 6).all()
+    np.testing.assert_allclose(ru["q0"], 2.0, rtol=1e-3)   # float weights: B flat to ~1e-7, J to ~5e-5
+    np.testing.assert_allclose(ru["end"][:, 0], su[:, 0] + 0.3 * 20, atol=1e-4)
+    np.testing.assert_allclose(ru["end"][:, 1], su[:, 1] - 0.2 * 20, atol=1e-4)
+
+
+def test_f32_model_is_the_fp64_tracer_in_float(oracle):
+    """On a random multipole the float model stays within float-arithmetic
+    distance of the FP64 oracle (a dropped term or a wrong index would show
+    as a gross difference): the same classes, footpoints within 1e-3 cells,
+    Q within the noise of a float seven-stencil quotient (~1e-3); and its
+    RKF45 evaluation counts sit below the oracle's, as an error norm with a
+    float round-off floor predicts."""
+    from fastqsl_amd import synth
+    from tests import parity
+    f = synth.multipole((48, 48, 48), seed=2, device="cpu")
+    rng = np.random.default_rng(4)
+    seeds = np.column_stack([rng.uniform(2, 45, 2000), rng.uniform(2, 45, 2000), np.zeros(2000)]).astype(np.float32)
+    ref = oracle.trace_scott(f, seeds, method="rkf45", s0=0.25, tol=1e-6)
+    r = oracle.trace_scott_f32(f, seeds, s0=0.25, tol=1e-6)
+    cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+    assert cls.mean() > 0.99
+    e = np.abs(r["end"] - ref["end"]).max(axis=1)[cls]
+    assert np.median(e) < 1e-3
+    d = parity.log_q_diff(r["q0"], ref["q0"])[cls]
+    assert np.median(d) < 1e-2
+    assert 0.5 < r["evals"].sum() / ref["evals"].sum() < 1.0
