@@ -76,6 +76,8 @@ PROTOTYPES = {
     "fqsl_copy_boundary": (i32, [vp, i32, i32, i32, vp]),
     "fqsl_trace_scott": (i32, [vp, ctypes.POINTER(fqsl_scott_params), u64, vp, vp, vp,
                                ctypes.POINTER(fqsl_scott_out), vp]),
+    "fqsl_trace_scott_noalloc": (i32, [vp, ctypes.POINTER(fqsl_scott_params), u64, vp, vp, vp,
+                                       ctypes.POINTER(fqsl_scott_out), vp]),
     "fqsl_trace_plain": (i32, [vp, ctypes.POINTER(fqsl_plain_params), u64, vp, vp, vp,
                                ctypes.POINTER(fqsl_plain_out), vp]),
     "fqsl_qcalc_plane": (i32, [i32, i32] + [vp] * 13 + [ctypes.c_double, vp, vp, vp]),

@@ -1193,6 +1193,22 @@ static hipError_t scott_persistent_launches(const F& f, const ScottParams& P, co
     return e;
 }
 
+// ---------------------------------------------- scratch of the no-alloc entry
+// fqsl_trace_scott_noalloc runs the persistent schedule over this array
+// instead of allocated memory: the queue counters in its first 256 bytes,
+// then the workspace of up to 2^20 FP32C or 2^19 FP64 lines (ordered queue
+// included).  Each device holds its own copy, at its own address, from the
+// library's load until the process ends.
+constexpr unsigned long long kNoAllocLinesFp32c = 1ull << 20;
+constexpr unsigned long long kNoAllocLinesFp64 = 1ull << 19;
+constexpr size_t kNoAllocQueueBytes = 256;
+constexpr size_t kNoAllocWsBytes =
+    scott_ws_layout<float>(kNoAllocLinesFp32c, true).bytes > scott_ws_layout<double>(kNoAllocLinesFp64, true).bytes
+        ? scott_ws_layout<float>(kNoAllocLinesFp32c, true).bytes
+        : scott_ws_layout<double>(kNoAllocLinesFp64, true).bytes;
+constexpr size_t kNoAllocScratchBytes = kNoAllocQueueBytes + align256(kNoAllocWsBytes);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 // X: the position path's type (R, or double for the mixed variant).
 template <typename R, class F, class FJ = NoCurrent, typename X = R>
 static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsigned long long n, const ScottOut& o,
@@ -1207,6 +1223,18 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         if (rc) return rc;
         const bool ordered = scott_ordered<R>(ds, n_hint, grid);
         const ScottWs w = scott_ws_layout<R>(n_hint, ordered);
+        if (noalloc) {  // device address of this device's scratch array: no allocation, no events
+            if (w.bytes > kNoAllocScratchBytes - kNoAllocQueueBytes)
+                return set_err(FQSL_ERR_INVALID, "%llu lines need %zu scratch bytes, the array holds %zu",
+                               n_hint, w.bytes, kNoAllocScratchBytes - kNoAllocQueueBytes);
+            g_ev_valid = false;
+            g_evs_valid = false;
+            const hipError_t e = scott_persistent_launches<R, F, FJ, X>(
+                f, P, s, n, o, ds, st, n_hint, fj, noalloc + kNoAllocQueueBytes, w, ordered,
+                (unsigned long long*)noalloc, kern, grid, false);
+            if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "Scott trace launch: %s", hipGetErrorString(e));
+            return FQSL_OK;
+        }
         unsigned long long* q = nullptr;
         rc = queue_slot(&q);
         if (rc) return rc;
@@ -1222,6 +1250,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
     if (schedule == FQSL_SCHED_PER_LINE) {
         int rc = record_start(st);
         if (rc) return rc;
+        if (noalloc) return set_err(FQSL_ERR_INVALID, "the no-alloc entry runs the persistent schedule only");
         auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F, FJ, X>
                                                                         : k_scott_per_line<R, F, FJ, X>;
         kern<<<per_line_grid(n_hint, 256, ds), 256, 0, st>>>(f, P, s, n, o, ds, fj);
@@ -1352,6 +1381,40 @@ static int trace_scott_impl(const fqsl_field* field, const fqsl_scott_params* p,
 extern "C" int fqsl_trace_scott(const fqsl_field* field, const fqsl_scott_params* p, uint64_t n, const float* x,
                                 const float* y, const float* z, const fqsl_scott_out* out, void* stream) {
     return trace_scott_impl(field, p, n, x, y, z, out, stream, nullptr);
+}
+
+// Per device: the address of its scratch array, looked up once with that
+// device current (the array's host-side name is not a device address).
+static std::mutex g_noalloc_mu;
+static char* g_noalloc_ptr[64];
+
+extern "C" int fqsl_trace_scott_noalloc(const fqsl_field* field, const fqsl_scott_params* p, uint64_t n,
+                                        const float* x, const float* y, const float* z, const fqsl_scott_out* out,
+                                        void* stream) {
+    if (!p) return set_err(FQSL_ERR_INVALID, "NULL params");
+    if (p->schedule != FQSL_SCHED_PERSISTENT)
+        return set_err(FQSL_ERR_INVALID, "the no-alloc entry runs the persistent schedule only");
+    const unsigned long long cap = p->precision == FQSL_PREC_FP64 ? kNoAllocLinesFp64 : kNoAllocLinesFp32c;  // mixed: float records
+    if (n > cap)
+        return set_err(FQSL_ERR_INVALID, "%llu lines exceed the no-alloc maximum of %llu for this precision",
+                       (unsigned long long)n, cap);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_err(FQSL_ERR_UNSUPPORTED, "device index %d", dev);
+    char* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_noalloc_ptr[dev]) {
+            void* a = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !a)
+                return set_err(FQSL_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            (void)cu_count();  // cache the CU count now: later calls only read it
+            g_noalloc_ptr[dev] = (char*)a;
+        }
+        ptr = g_noalloc_ptr[dev];
+    }
+    return trace_scott_impl(field, p, n, x, y, z, out, stream, ptr);
 }
 
 // ---------------------------------------------------- current density (curl)

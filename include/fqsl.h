@@ -155,6 +155,22 @@ int fqsl_trace_scott(const fqsl_field *field, const fqsl_scott_params *params, u
                      const float *x, const float *y, const float *z, const fqsl_scott_out *out,
                      void *stream);
 
+/* fqsl_trace_scott that allocates nothing.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The persistent schedule's workspace (queue counters, seed
+ * frames, finished-line records, queue order) lives in a fixed array the
+ * library holds per device from its load until the process ends (about
+ * 148 MiB): n is at most 2^20 lines with FQSL_PREC_FP32C or FQSL_PREC_MIXED
+ * and 2^19 with FQSL_PREC_FP64, larger n fails with FQSL_ERR_INVALID, and params->schedule
+ * must be FQSL_SCHED_PERSISTENT.  Results are the same bits as
+ * fqsl_trace_scott's.  All calls on one device share that array: calls that
+ * overlap on one device (other streams or host threads) race and must be
+ * serialised by the caller.  It records no timing events
+ * (fqsl_last_kernel_ms / fqsl_last_step_ms then fail). */
+int fqsl_trace_scott_noalloc(const fqsl_field *field, const fqsl_scott_params *params, uint64_t n,
+                             const float *x, const float *y, const float *z,
+                             const fqsl_scott_out *out, void *stream);
 
 /* ----------------------------------------------------------------- batches */
 /* Many Scott requests -- each with its own field, method, step, tolerance,
