@@ -28,8 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP32_PEAK = 157.3e12       # FLOP/s, MI355X FP32 vector
+N_SIMDS = 1024             # 256 CUs x 4 SIMD16
+SCLK_PEAK_MHZ = 2400.0     # MI355X peak engine clock
 BYTES_PER_EVAL_F32 = 96    # 8 corners x 3 components x 4 B per RHS evaluation
 BYTES_PER_EVAL_F64 = 192
+FLOPS_PER_EVAL = 350       # algorithmic flops of one Scott stage evaluation (analytic grad b)
 
 
 def parse():
@@ -42,7 +46,7 @@ def parse():
     ap.add_argument("--duration", type=float, default=2.0,
                     help="serve: seconds of arrivals per rate")
     ap.add_argument("--workload", default="plane",
-                    choices=["plane", "volume", "batch", "serve", "stretch", "neighbour", "twist"],
+                    choices=["plane", "volume", "batch", "serve", "stretch", "neighbour", "twist", "c1"],
                     help="plane: the 1024^2 Scott Q map (default); volume: 3-D Q on every node "
                          "of the 512x256x256 rope-arcade field; batch: 256 uneven plane requests "
                          "over 3 resident fields in one graph-captured run; serve: Poisson "
@@ -62,6 +66,10 @@ def parse():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="plane workload: skip the C1/C3/C4/C5 blocks of the default record")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="plane workload: skip the mixed-precision block")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB device-to-device copy (known bytes) for counter scaling")
     return ap.parse_args()
@@ -124,9 +132,13 @@ def cpu_baseline(field_np, seeds_at, nlines, sample, log):
     return rec, idx, ref
 
 
-def error_vs_oracle(out, idx, ref):
+def error_vs_oracle(out, idx, ref, floors=None, controller="fp32"):
     """Per-line error of the GPU map against the FP64 oracle on the sampled
-    lines: class match, |dlog10 Q| distribution, endpoint error in cells."""
+    lines: class mismatches, |dlog10 Q| distribution, endpoint error in
+    cells; with ``floors`` (class_floors) the class-mismatch budget the path
+    is held to: the oracle's one-ulp spread + 1e-4 of the lines for a path
+    on the FP64 controller, the oracle's float model's + 1e-4 for the FP32
+    error norm."""
     g = {k: out[k][idx].cpu().numpy() for k in ("flag_start", "flag_end", "q_0", "end_x", "end_y",
                                                  "end_z")}
     cls = (g["flag_start"] == ref["flag_start"]) & (g["flag_end"] == ref["flag_end"])
@@ -140,13 +152,20 @@ def error_vs_oracle(out, idx, ref):
     end = np.column_stack([g["end_x"], g["end_y"], g["end_z"]]).astype(np.float64)
     e = np.abs(end - ref["end"]).max(axis=1)[cls]
     e = e[np.isfinite(e)]
-    return {"lines": int(len(idx)), "class_match": float(cls.mean()),
-            "frac_dlog10q_le_1e-3": float(np.mean(dm <= 1e-3)),
-            "dlog10q_median": float(np.median(fin)), "dlog10q_p999": float(np.quantile(fin, 0.999)),
-            "endpoint_err_cells_median": float(np.median(e)),
-            "endpoint_err_cells_p99": float(np.quantile(e, 0.99)),
-            "note": "RKF45 step sequences fork on round-off; the oracle against itself with "
-                    "tol x (1 + 1e-12) gives class match ~0.999 and ~0.67 within 1e-3"}
+    rec = {"lines": int(len(idx)), "class_match": float(cls.mean()), "class_mismatches": int((~cls).sum()),
+           "frac_dlog10q_le_1e-3": float(np.mean(dm <= 1e-3)),
+           "dlog10q_median": float(np.median(fin)), "dlog10q_p999": float(np.quantile(fin, 0.999)),
+           "endpoint_err_cells_median": float(np.median(e)),
+           "endpoint_err_cells_p99": float(np.quantile(e, 0.99))}
+    if floors:
+        budget = floors["budget_fp64_controller" if controller == "fp64" else "budget_fp32_error_norm"]
+        rec["class_mismatch_budget"] = budget
+        rec["within_budget"] = bool(rec["class_mismatches"] <= budget)
+        rec["budget_is"] = ("the oracle's own class mismatches under one-float32-ulp seed moves + 1e-4 N"
+                            if controller == "fp64" else
+                            "the class mismatches of the oracle's float model (the reference's own float "
+                            "arithmetic) + 1e-4 N")
+    return rec
 
 
 def bench_batch(args, fq, torch, log):
@@ -296,10 +315,10 @@ def bench_batch(args, fq, torch, log):
         "tuning": tuning,
         **extra,
     }
-    print(json.dumps(rec), flush=True)
     b.close()
     for f in fields:
         f.close()
+    return rec
 
 
 def bench_serve(args, fq, torch, log):
@@ -363,9 +382,9 @@ def bench_serve(args, fq, torch, log):
         "sweep": sweep,
         "fp64_plain_serial": base,
     }
-    print(json.dumps(rec), flush=True)
     for f in fields:
         f.close()
+    return rec
 
 
 def bench_neighbour(args, fq, torch, log):
@@ -500,7 +519,8 @@ def bench_neighbour(args, fq, torch, log):
         "cpu_baseline": cpu,
         **extra,
     }
-    print(json.dumps(rec), flush=True)
+    field.close()
+    return rec
 
 
 def bench_twist(args, fq, torch, log):
@@ -608,7 +628,8 @@ def bench_twist(args, fq, torch, log):
         "cpu_baseline": cpu,
         **extra,
     }
-    print(json.dumps(rec), flush=True)
+    field.close()
+    return rec
 
 
 def calibration_copy(torch, dev, log):
@@ -737,42 +758,85 @@ def bench_stretch(args, fq, torch, log):
         "classes": {str(int(c)): int(v) for c, v in zip(classes, counts)},
         **extra,
     }
-    print(json.dumps(rec), flush=True)
     field.close()
+    return rec
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(lines are independent, so a multi-GPU run would shard the seeds over one process per GPU)",
-              file=sys.stderr)
-        sys.exit(2)
-    import torch
-    import fastqsl_amd as fq
+def profile_values(workload):
+    """key = value lines of the workload's profile summary (HBM traffic,
+    VALU instructions and engine clock of its dominant kernel, as measured
+    with rocprofv3 on the current build)."""
+    vals = {}
+    try:
+        with open(os.path.join(ROOT, PROFILES[workload])) as fh:
+            for line in fh:
+                if "=" in line and not line.startswith((" ", "|", "#")):
+                    k, v = line.split("=", 1)
+                    try:
+                        vals[k.strip()] = float(v.split()[0])
+                    except (ValueError, IndexError):
+                        pass
+    except (OSError, KeyError):
+        pass
+    return vals
 
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
+
+def valu_roofline(workload, kernel_ms, evals, bpe, oracle_evals=None, kernel="", flops_per_eval=FLOPS_PER_EVAL):
+    """Roofline of a stepping kernel bound by VALU issue.  achieved / peak is
+    the issue rate of VALU wave instructions (4 cycles per wave64 instruction
+    on a SIMD16) against the chip's (1,024 SIMDs at the engine clock); the
+    instruction count per launch comes from the workload's rocprofv3 profile
+    (SQ_INSTS_VALU), the kernel time is measured live.  Beside it: HBM
+    traffic (profile) and algorithmic flops against their peaks, and the
+    no-reuse stencil model (96 B per stage evaluation)."""
+    pv = profile_values(workload)
+    t = kernel_ms * 1e-3
+    insts = pv.get("valu_insts_per_launch")
+    clk = pv.get("sclk_mhz", SCLK_PEAK_MHZ) * 1e6
+    peak = N_SIMDS * clk / 4.0
+    traffic = pv.get("traffic_bytes_per_launch")
+    rec = {"bound": "valu", "kernel": kernel, "kernel_ms": kernel_ms,
+           "unit": "G VALU wave-instructions/s",
+           "achieved": insts / t / 1e9 if insts else None, "peak": peak / 1e9,
+           "frac": insts / t / peak if insts else None,
+           "valu_insts_per_launch": insts, "sclk_mhz": clk / 1e6, "simds": N_SIMDS,
+           "traffic": traffic,
+           "hbm": {"achieved_GBs": traffic / t / 1e9 if traffic else None, "peak_GBs": HBM_PEAK / 1e9,
+                   "frac": traffic / t / HBM_PEAK if traffic else None},
+           "flops": {"per_eval": flops_per_eval, "achieved_TFLOPs": evals * flops_per_eval / t / 1e12,
+                     "peak_TFLOPs": FP32_PEAK / 1e12, "frac": evals * flops_per_eval / t / FP32_PEAK},
+           "stencil_model": {"bytes_per_eval": bpe, "evals_per_launch": evals,
+                             "achieved_GBs": evals * bpe / t / 1e9, "frac": evals * bpe / t / HBM_PEAK},
+           "source": PROFILES.get(workload)}
+    if oracle_evals is not None:
+        rec["stencil_model"]["frac_at_oracle_evals"] = oracle_evals * bpe / t / HBM_PEAK
+    return rec
+
+
+def class_floors(field_np, seeds_sample, ref, hi, log):
+    """The oracle's own spread on the sample: from seeds moved by one
+    float32 ulp (the floor of an FP32 path) and in its float model (the
+    reference's own float arithmetic)."""
+    from oracle import oracle as O
+    from tests import parity
+    t0 = time.perf_counter()
+    ulp = O.trace_scott(field_np, parity.ulp_moved(seeds_sample, hi, np.random.default_rng(5)), method="rkf45",
+                        s0=0.25, tol=1e-6)
+    f32 = O.trace_scott_f32(field_np, seeds_sample, s0=0.25, tol=1e-6)
+    log(f"oracle floors in {time.perf_counter() - t0:.1f} s")
+    cm = lambda a: int((~parity.class_match(a["flag_start"], a["flag_end"], ref["flag_start"],
+                                            ref["flag_end"])).sum())
+    fr = lambda a: float(np.mean(parity.log_q_diff(a["q0"], ref["q0"]) <= 1e-3))
+    n = len(seeds_sample)
+    return {"lines": n, "one_ulp_oracle": {"class_mismatches": cm(ulp), "frac_dlog10q_le_1e-3": fr(ulp)},
+            "float_model": {"class_mismatches": cm(f32), "frac_dlog10q_le_1e-3": fr(f32),
+                            "evals_per_line": float(f32["evals"].mean())},
+            "budget_fp64_controller": cm(ulp) + 1e-4 * n, "budget_fp32_error_norm": cm(f32) + 1e-4 * n}
+
+
+def bench_trace(args, fq, torch, log, volume):
+    """The plane map (C2, the headline) or the 3-D cube (C3)."""
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-
-    if args.schedule is None:
-        args.schedule = "per_line" if args.workload == "stretch" else "persistent"
-    if args.workload == "serve":
-        return bench_serve(args, fq, torch, log)
-    if args.workload == "stretch":
-        return bench_stretch(args, fq, torch, log)
-    if args.workload == "neighbour":
-        return bench_neighbour(args, fq, torch, log)
-    if args.workload == "twist":
-        return bench_twist(args, fq, torch, log)
-    if args.workload == "batch":
-        if args.steps is None:
-            args.steps = 3
-        if args.warmup is None:
-            args.warmup = 1
-        return bench_batch(args, fq, torch, log)
-    volume = args.workload == "volume"
     if args.steps is None:
         args.steps = 2 if volume else 20
     if args.warmup is None:
@@ -790,11 +854,12 @@ def main():
         nx_, ny_, nz_ = shape
 
         def seeds_at(idx):   # line idx traces voxel brick[idx] (cube order: x slowest, z fastest)
-            v = brick_host[idx]
+            v = brick_host[idx].astype(np.int64)
             return np.column_stack([v // (ny_ * nz_), (v // nz_) % ny_, v % nz_]).astype(np.float32)
         nlines = int(sx.numel())
         workload = ("Gold-Hoyle rope (1.5 turns, r 0.2, axis z 0.4) + 2 bipoles, 512x256x256 "
                     "float32, 3-D Scott Q on every grid node (traced in 4^3 bricks), RKF45 tol 1e-6")
+        hi = (shape[0] - 1, shape[1] - 1)
     else:
         bx, by, bz = fq.synth.multipole_256(seed=0, device="cuda")
         n = args.pixels
@@ -806,6 +871,7 @@ def main():
         nlines = seeds.shape[0]
         workload = ("random multipole (40 sources, default_rng(0)) 256^3 float32, "
                     "Scott Q, RKF45 tol 1e-6, z=0 plane %dx%d" % (n, n))
+        hi = (255, 255)
     log(f"field generated in {time.perf_counter() - t0:.1f} s; {nlines} lines per step")
 
     layouts = ("soa64", "cell32")
@@ -835,38 +901,47 @@ def main():
     if args.calibrate:
         calibration_copy(torch, dev, log)
 
-    for _ in range(args.warmup):
-        run(args.precision, args.schedule)
-    torch.cuda.synchronize()
-    # throughput: K maps back to back on the stream, synchronised on both sides
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run(args.precision, args.schedule)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    # latency and kernel times: the same K maps, each waited for
-    kernel_ms = []
-    trace_ms = []
-    step_ms = []
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        run(args.precision, args.schedule)
-        trace_ms.append(fq.last_kernel_ms())         # waits for the map's end event
-        # the stepping kernel alone (persistent schedule); the whole trace otherwise
-        kernel_ms.append(fq._lib.last_step_ms() if args.schedule == "persistent" else trace_ms[-1])
-        step_ms.append((time.perf_counter() - ts) * 1e3)
+    def timed(precision, schedule, steps, warmup):
+        for _ in range(warmup):
+            run(precision, schedule)
+        torch.cuda.synchronize()
+        # throughput: K maps back to back on the stream, synchronised on both sides
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run(precision, schedule)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # latency and kernel times: the same K maps, each waited for
+        kernel_ms, trace_ms, step_ms = [], [], []
+        for _ in range(steps):
+            ts = time.perf_counter()
+            run(precision, schedule)
+            trace_ms.append(fq.last_kernel_ms())         # waits for the map's end event
+            # the stepping kernel alone (persistent schedule); the whole trace otherwise
+            kernel_ms.append(fq._lib.last_step_ms() if schedule == "persistent" else trace_ms[-1])
+            step_ms.append((time.perf_counter() - ts) * 1e3)
+        out = outs[(precision, schedule)]
+        evals = int(out["evals"].to(torch.int64).sum().item())
+        return dict(wall=wall, kms=float(np.mean(kernel_ms)), trace_ms=float(np.mean(trace_ms)),
+                    step_ms=step_ms, evals=evals, value=nlines * steps / wall, ms_step=wall / steps * 1e3)
+
+    head = timed(args.precision, args.schedule, args.steps, args.warmup)
     out = outs[(args.precision, args.schedule)]
-    evals = int(out["evals"].to(torch.int64).sum().item())
     flags = torch.stack([out["flag_start"], out["flag_end"]]).cpu().numpy()
-    ms_step = wall / args.steps * 1e3
-    value = nlines * args.steps / wall
-    kms = float(np.mean(kernel_ms))
+    value, kms, evals = head["value"], head["kms"], head["evals"]
     bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
-    achieved = evals * bpe / (kms * 1e-3)
-    log(f"{args.precision}/{args.schedule}: {value:.4g} lines/s, {ms_step:.2f} ms/step, "
-        f"trace {np.mean(trace_ms):.2f} ms, stepping kernel {kms:.2f} ms, {evals / nlines:.1f} evals/line")
+    log(f"{args.precision}/{args.schedule}: {value:.4g} lines/s, {head['ms_step']:.2f} ms/step, "
+        f"trace {head['trace_ms']:.2f} ms, stepping kernel {kms:.2f} ms, {evals / nlines:.1f} evals/line")
 
     extra = {}
+    if args.precision == "fp32c" and args.schedule == "persistent" and not args.no_mixed:
+        m = timed("mixed", "persistent", max(3, args.steps // 4), 1)
+        extra["mixed"] = {"lines_per_s": m["value"], "ms_per_step": m["ms_step"], "kernel_ms": m["kms"],
+                          "evals_per_line": m["evals"] / nlines,
+                          "what": "FP64 position path and error norm, FP32 Jacobian: the FP64 "
+                                  "controller's step sequences (reference-parity precision)"}
+        log(f"mixed/persistent: {m['value']:.4g} lines/s, {m['ms_step']:.2f} ms/step, stepping kernel "
+            f"{m['kms']:.2f} ms, {m['evals'] / nlines:.1f} evals/line")
     if (volume or args.per_line_fp32) and (args.precision, args.schedule) == ("fp32c", "persistent"):
         run("fp32c", "per_line")
         torch.cuda.synchronize()
@@ -888,25 +963,38 @@ def main():
         extra["fp64_per_line"] = {"lines_per_s": nlines / dt, "ms_per_step": dt * 1e3,
                                   "kernel_ms": fq.last_kernel_ms()}
         extra["speedup_vs_fp64_per_line"] = value / (nlines / dt)
+        if "mixed" in extra:
+            extra["mixed"]["speedup_vs_fp64_per_line"] = extra["mixed"]["lines_per_s"] / (nlines / dt)
         log(f"fp64/per_line: {nlines / dt:.4g} lines/s ({dt * 1e3:.1f} ms)")
 
     cpu = None
+    oracle_evals = None
     if not args.no_cpu_baseline:
-        field_np = tuple(np.asarray(b, dtype=np.float64) for b in (bx, by, bz))
+        field_np = tuple(np.asfortranarray(np.asarray(b.cpu().numpy() if hasattr(b, "cpu") else b, np.float64))
+                         for b in (bx, by, bz))
         cpu, idx, ref = cpu_baseline(field_np, seeds_at, nlines,
                                      args.cpu_sample if not volume else min(args.cpu_sample, 1 << 15),
                                      log)
-        extra["error_vs_oracle"] = error_vs_oracle(out, idx, ref)
+        oracle_evals = cpu["oracle_evals_per_line"] * nlines
+        floors = class_floors(field_np, seeds_at(idx), ref, hi, log) if not volume else None
+        extra["error_vs_oracle"] = error_vs_oracle(out, idx, ref, floors, "fp32")
+        if ("mixed", "persistent") in outs:
+            extra["mixed"]["error_vs_oracle"] = error_vs_oracle(outs[("mixed", "persistent")], idx, ref, floors,
+                                                                "fp64")
         if ("fp64", "per_line") in outs:
-            extra["fp64_per_line"]["error_vs_oracle"] = error_vs_oracle(outs[("fp64", "per_line")],
-                                                                        idx, ref)
-        # the same map's bytes counted with the oracle's evaluations per line
-        # (the reference algorithm's work; the FP32 controller spends fewer)
-        ev_orc = cpu["oracle_evals_per_line"] * nlines
-        extra["roofline_at_oracle_evals"] = {"achieved": ev_orc * bpe / (kms * 1e-3) / 1e9,
-                                             "frac": ev_orc * bpe / (kms * 1e-3) / HBM_PEAK}
+            extra["fp64_per_line"]["error_vs_oracle"] = error_vs_oracle(outs[("fp64", "per_line")], idx, ref,
+                                                                        floors, "fp64")
+        if floors:
+            extra["oracle_floors"] = floors
 
     classes, counts = np.unique(flags, return_counts=True)
+    prof = "volume" if volume else "plane"
+    roof = valu_roofline(prof, kms, evals, bpe, oracle_evals,
+                         kernel=("stepping kernel k_scott_persistent_w2" if args.schedule == "persistent"
+                                 else "k_scott_per_line"))
+    if not (args.precision == "fp32c" and args.schedule == "persistent" and (volume or args.pixels == 1024)):
+        roof.update(achieved=None, frac=None, valu_insts_per_launch=None, traffic=None,
+                    note="the profile covers the default FP32c persistent workload only")
     rec = {
         "metric": "Scott-Q field lines/s",
         "value": value,
@@ -914,36 +1002,161 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
-                       "p99": float(np.percentile(step_ms, 99)),
+        "ms_per_step": head["ms_step"],
+        "latency_ms": {"p50": float(np.percentile(head["step_ms"], 50)),
+                       "p99": float(np.percentile(head["step_ms"], 99)),
                        "what": "one full Q map, launch to results resident, each map waited for "
                                "(a second pass of the same K maps)"},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32c" else "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": workload,
                    "lines_per_step": nlines, "precision": args.precision,
                    "schedule": args.schedule, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": (measured_traffic(args.workload) if args.precision == "fp32c"
-                                 and args.schedule == "persistent" and (volume or args.pixels == 1024)
-                                 else None),
-                     "traffic_source": "%s (%s)" % (PROFILES[args.workload], TRAFFIC_HOW),
-                     "kernel": ("stepping kernel k_scott_persistent_w2" if args.schedule == "persistent"
-                                else "k_scott_per_line"),
-                     "kernel_ms": kms, "trace_ms": float(np.mean(trace_ms)),
-                     "evals_per_launch": evals, "bytes_per_eval": bpe},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "tuning": tuning,
         "classes": {str(int(c)): int(k) for c, k in zip(classes, counts)},
         **extra,
     }
-    print(json.dumps(rec), flush=True)
     field.close()
+    return rec
+
+
+def bench_c1(args, fq, torch, log):
+    """The quadrupole workload: 64^3 float64 field, Scott Q with RK4 at a fixed
+    0.25-cell step, 256^2 seeds on z = 0; FP64 on the GPU (persistent and one
+    thread per line), the FP32c and mixed paths beside it, the FP64 oracle on
+    all 65,536 lines on the host.  Parity is counted off the field's two
+    mirror planes x = y and x + y = 63 (512 pixels whose lines run into the
+    central null in exact arithmetic)."""
+    from tests import parity
+    field_np = fq.synth.quadrupole_64(dtype=np.float64, device="cpu")
+    g = np.linspace(0, 63, 256, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)]).astype(np.float32)
+    sx, sy, sz = (torch.as_tensor(np.ascontiguousarray(seeds[:, i]), device="cuda") for i in range(3))
+    n = seeds.shape[0]
+    row, col = np.divmod(np.arange(n), 256)
+    off = (row != col) & (row + col != 255)
+    res = {}
+    with fq.Field(*field_np) as f:
+        for prec, sched in (("fp64", "persistent"), ("fp64", "per_line"), ("fp32c", "persistent"),
+                            ("mixed", "persistent")):
+            r = None
+            for _ in range(2):
+                r = fq.trace_scott(f, sx, sy, sz, method="rk4", step=0.25, precision=prec, schedule=sched)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 5
+            for _ in range(reps):
+                r = fq.trace_scott(f, sx, sy, sz, method="rk4", step=0.25, precision=prec, schedule=sched,
+                                   out=r)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            res[f"{prec}_{sched}"] = {"lines_per_s": n / dt, "ms_per_map": dt * 1e3,
+                                      "_r": {k: r[k].cpu().numpy() for k in ("flag_start", "flag_end", "q_0")}}
+    from oracle import oracle as O
+    nt = O.num_threads()
+    t0 = time.perf_counter()
+    ref = O.trace_scott(field_np, seeds, method="rk4", s0=0.25, nthreads=nt)
+    dt = time.perf_counter() - t0
+    for k, v in res.items():
+        r = v.pop("_r")
+        cls = parity.class_match(r["flag_start"], r["flag_end"], ref["flag_start"], ref["flag_end"])
+        d = parity.log_q_diff(r["q_0"], ref["q0"])
+        v["error_vs_oracle"] = {"lines_off_mirror_planes": int(off.sum()), "class_match": float(cls[off].mean()),
+                                "frac_dlog10q_le_1e-3": float(np.mean(d[off] <= 1e-3)),
+                                "dlog10q_max": float(d[off][np.isfinite(d[off])].max()),
+                                "class_match_all_pixels": float(cls.mean())}
+    log(f"C1: fp64 persistent {res['fp64_persistent']['ms_per_map']:.2f} ms, fp32c "
+        f"{res['fp32c_persistent']['ms_per_map']:.2f} ms; oracle {dt:.2f} s on {nt} threads")
+    return {"workload": "quadrupole (q = +-1 alternating at (+-0.3, +-0.3, -0.15)) 64^3 float64, Scott Q, RK4 "
+                        "step 0.25, z=0 plane 256x256",
+            "lines_per_step": n, "value": res["fp64_persistent"]["lines_per_s"], "unit": "lines/s",
+            "paths": res, "cpu_baseline": {"value": n / dt, "unit": "lines/s", "cores": nt, "kind": "port",
+                                           "sample": "all 65,536 lines, FP64 C oracle"}}
+
+
+def summary_of(rec, keys=("value", "unit", "ms_per_step", "latency_ms", "speedup_vs_fp64_per_line")):
+    out = {k: rec[k] for k in keys if k in rec}
+    out["workload"] = rec.get("config", {}).get("workload")
+    for k in ("fp64_per_line", "fp32c_per_line", "persistent_vs_per_line_fp32c", "one_call_per_request_fp32c",
+              "error_vs_oracle", "cpu_baseline", "sweep", "fp64_plain_serial", "mixed"):
+        if rec.get(k) is not None:
+            out[k] = rec[k]
+    if isinstance(rec.get("roofline"), dict):
+        out["roofline"] = {k: rec["roofline"].get(k) for k in ("bound", "frac", "kernel_ms", "stencil_model",
+                                                               "hbm", "achieved", "peak", "unit")
+                           if k in rec["roofline"]}
+    return out
+
+
+def other_configs(args, fq, torch, log):
+    """C1, C3, C4 and C5 at their stated sizes, each shortened to a few
+    timed repetitions, for the default run's record."""
+    import copy
+    out = {}
+    t0 = time.perf_counter()
+    out["C1_quadrupole_rk4"] = bench_c1(args, fq, torch, log)
+    a = copy.copy(args)
+    a.workload, a.steps, a.warmup, a.no_autotune, a.cpu_sample = "volume", 1, 1, True, 1 << 14
+    a.no_mixed = True
+    out["C3_volume"] = summary_of(bench_trace(a, fq, torch, log, True))
+    torch.cuda.empty_cache()
+    a = copy.copy(args)
+    a.workload, a.steps, a.warmup, a.no_autotune = "batch", 2, 1, True
+    out["C4_batch"] = summary_of(bench_batch(a, fq, torch, log))
+    torch.cuda.empty_cache()
+    a = copy.copy(args)
+    a.workload, a.rates, a.duration = "serve", "20", 2.0
+    out["C5_serve"] = summary_of(bench_serve(a, fq, torch, log))
+    torch.cuda.empty_cache()
+    log(f"other configs in {time.perf_counter() - t0:.1f} s")
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(lines are independent, so a multi-GPU run would shard the seeds over one process per GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fastqsl_amd as fq
+
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+
+    if args.schedule is None:
+        args.schedule = "per_line" if args.workload == "stretch" else "persistent"
+    if args.workload == "serve":
+        rec = bench_serve(args, fq, torch, log)
+    elif args.workload == "stretch":
+        rec = bench_stretch(args, fq, torch, log)
+    elif args.workload == "neighbour":
+        rec = bench_neighbour(args, fq, torch, log)
+    elif args.workload == "twist":
+        rec = bench_twist(args, fq, torch, log)
+    elif args.workload == "batch":
+        if args.steps is None:
+            args.steps = 3
+        if args.warmup is None:
+            args.warmup = 1
+        rec = bench_batch(args, fq, torch, log)
+    elif args.workload == "c1":
+        rec = bench_c1(args, fq, torch, log)
+    else:
+        rec = bench_trace(args, fq, torch, log, args.workload == "volume")
+        if args.workload == "plane" and not args.headline_only:
+            torch.cuda.empty_cache()
+            rec["configs"] = other_configs(args, fq, torch, log)
+    print(json.dumps(rec), flush=True)
 
 
 if __name__ == "__main__":

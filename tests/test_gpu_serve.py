@@ -68,3 +68,40 @@ def test_serial_fp64_baseline_matches_alone(snaps):
         a = _alone(fq, snaps[r.snapshot], r, "fp64")
         for k in serve.OUTPUTS:
             assert np.array_equal(r.result[k].cpu().numpy(), a[k].cpu().numpy(), equal_nan=True), (r.rid, k)
+
+
+def test_c5_at_full_size():
+    """The serving workload at its stated size: 16 resident 320x320x256
+    snapshots of the random multipole (1-5 % Fourier-mode perturbations),
+    Poisson arrivals of z = 0 maps of 128^2..2048^2 pixels at 20 requests/s
+    (below saturation) for 1 s, continuous batching on two streams; every
+    request's result equals its trace alone, bit for bit."""
+    import torch
+    import fastqsl_amd as fq
+    from fastqsl_amd import serve
+    shape = (320, 320, 256)
+    fs = []
+    try:
+        for i in range(16):
+            fs.append(fq.Field(*synth.snapshot(i, shape=shape, device="cuda"), layouts=("cell32",)))
+        print(f"16 snapshots resident: {sum(f.nbytes for f in fs) / 1e9:.1f} GB", flush=True)
+        reqs = serve.poisson_requests(rate=20, duration=1.0, nsnap=16, shape=shape, seed=20, min_count=12)
+        srv = serve.QServer(fs, precision="fp32c", streams=2, keep_results=True)
+        st = srv.run(reqs)
+        print({k: v for k, v in st.items() if k != "latency_ms"}, flush=True)
+        assert st["requests"] == len(reqs) and st["lines"] > 1e7
+        assert min(r.n for r in reqs) >= 128 and max(r.n for r in reqs) <= 2048
+        for r in reqs:
+            x, y, z = r.seeds()
+            a = fq.trace_scott(fs[r.snapshot], x, y, z, method=r.method, step=r.step, tol=r.tol,
+                               precision="fp32c", outputs=serve.OUTPUTS)
+            for k in serve.OUTPUTS:
+                u, v = r.result[k], a[k]
+                if u.dtype == torch.float32:
+                    u, v = u.view(torch.int32), v.view(torch.int32)
+                assert bool(torch.equal(u, v)), (r.rid, k)
+            r.result = None
+    finally:
+        for f in fs:
+            f.close()
+        torch.cuda.empty_cache()
