@@ -41,6 +41,13 @@ class fqsl_scott_out(ctypes.Structure):
                                   "q_0", "q_perp", "line_len", "evals", "twist")]
 
 
+PLANE_Z, PLANE_ABC = 0, 1  # FQSL_PLANE_*
+
+
+class fqsl_plane(ctypes.Structure):
+    _fields_ = [("kind", i32), ("n_u", i32), ("n_v", i32), ("reserved", i32), ("p", ctypes.c_double * 9)]
+
+
 class fqsl_plain_params(ctypes.Structure):
     _fields_ = [("method", i32), ("precision", i32), ("schedule", i32), ("reserved", i32),
                 ("step", ctypes.c_double), ("tol_coef", ctypes.c_double),
@@ -103,6 +110,13 @@ PROTOTYPES = {
     "fqsl_scott_batch_run": (i32, [vp, vp]),
     "fqsl_scott_batch_times": (i32, [vp, ctypes.POINTER(ctypes.c_double), vp]),
     "fqsl_scott_batch_destroy": (i32, [vp]),
+    "fqsl_qmap_plane": (i32, [vp, ctypes.POINTER(fqsl_plane), ctypes.POINTER(fqsl_scott_params),
+                              ctypes.POINTER(fqsl_scott_out), vp]),
+    "fqsl_server_create": (i32, [i32, i32, u64, ctypes.POINTER(vp)]),
+    "fqsl_submit": (i32, [vp, ctypes.POINTER(fqsl_scott_request), vp, ctypes.POINTER(u64)]),
+    "fqsl_poll": (i32, [vp, u64, ctypes.POINTER(i32)]),
+    "fqsl_wait": (i32, [vp, u64]),
+    "fqsl_server_destroy": (i32, [vp]),
     "fqsl_last_kernel_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
     "fqsl_last_step_ms": (i32, [ctypes.POINTER(ctypes.c_float)]),
     "fqsl_tune_set": (i32, [i32, i32]),
@@ -133,7 +147,7 @@ def check(rc: int) -> None:
 
 
 TUNE_KNOBS = {"scott_waves": 0, "grid_pct": 1, "queues": 2, "refill_min": 3, "order": 4,
-              "order_buckets": 5, "plain_refill": 6}
+              "order_buckets": 5, "plain_refill": 6, "stretch_lds": 7}
 
 
 def tune_set(knob: str, value: int) -> None:

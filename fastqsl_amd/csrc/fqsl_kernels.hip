@@ -19,7 +19,13 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "fqsl.h"
@@ -70,6 +76,7 @@ struct fqsl_field {
     bool stretched;
     const StretchGrid<float>* sg32;   // device copies, beside the tables
     const StretchGrid<double>* sg64;
+    int sg_lds_bytes;                 // bytes of sg32's compact table (0: none)
 };
 
 __global__ void k_convert_to_f32(const double* in, float* out, long n) {
@@ -281,7 +288,7 @@ extern "C" int fqsl_field_shape(const fqsl_field* f, int* nx, int* ny, int* nz) 
 // and double views.  Landing planes: 0 / n (the reference rules) or the box
 // faces (exact rule).
 static int build_stretch(const std::vector<double> (&a)[3], const int* n, int rule, void** mem, size_t* bytes,
-                         const StretchGrid<float>** g32, const StretchGrid<double>** g64) {
+                         const StretchGrid<float>** g32, const StretchGrid<double>** g64, int* lds_bytes) {
     if (rule != kLookupBisect && rule != kLookupUniform && rule != kLookupExact)
         return set_err(FQSL_ERR_INVALID, "lookup rule %d", rule);
     AxisTables t[3];
@@ -298,6 +305,11 @@ static int build_stretch(const std::vector<double> (&a)[3], const int* n, int ru
     for (int q = 0; q < 3; ++q) { off32[q] = total; total = up(total + t[q].rec32.size() * sizeof(float4)); }
     for (int q = 0; q < 3; ++q) { offr[q] = total; total = up(total + t[q].ref.size() * sizeof(int)); }
     for (int q = 0; q < 3; ++q) { offb[q] = total; total = up(total + t[q].bins.size() * sizeof(int)); }
+    std::vector<uint2> compact;
+    int lbin[3] = {0, 0, 0}, lcell[3] = {0, 0, 0};
+    if (!build_compact(t, compact, lbin, lcell)) compact.clear();
+    const size_t offc = total;
+    total = up(total + compact.size() * sizeof(uint2));
     const size_t offg32 = total;
     total = up(total + sizeof(StretchGrid<float>));
     const size_t offg64 = total;
@@ -311,6 +323,7 @@ static int build_stretch(const std::vector<double> (&a)[3], const int* n, int ru
         memcpy(host.data() + offr[q], t[q].ref.data(), t[q].ref.size() * sizeof(int));
         memcpy(host.data() + offb[q], t[q].bins.data(), t[q].bins.size() * sizeof(int));
     }
+    memcpy(host.data() + offc, compact.data(), compact.size() * sizeof(uint2));
     char* d = nullptr;
     HIP_TRY(hipMalloc(&d, total));
     const void* rec64[3] = {d + off64[0], d + off64[1], d + off64[2]};
@@ -323,6 +336,14 @@ static int build_stretch(const std::vector<double> (&a)[3], const int* n, int ru
     StretchGrid<float> h32;
     fill_grid<double>(t, rule, br64, rec64, ref, bins, h64);
     fill_grid<float>(t, rule, br32, rec32, ref, bins, h32);
+    h64.compact = nullptr;
+    h64.compact_n = 0;
+    h32.compact = compact.empty() ? nullptr : (const uint2*)(d + offc);
+    h32.compact_n = (int)compact.size();
+    for (int q = 0; q < 3; ++q) {
+        h32.ax[q].lbin = lbin[q]; h32.ax[q].lcell = lcell[q];
+        h64.ax[q].lbin = 0; h64.ax[q].lcell = 0;
+    }
     memcpy(host.data() + offg32, &h32, sizeof(h32));
     memcpy(host.data() + offg64, &h64, sizeof(h64));
     hipError_t e = hipMemcpy(d, host.data(), total, hipMemcpyHostToDevice);
@@ -332,6 +353,7 @@ static int build_stretch(const std::vector<double> (&a)[3], const int* n, int ru
     }
     *g32 = (const StretchGrid<float>*)(d + offg32);
     *g64 = (const StretchGrid<double>*)(d + offg64);
+    *lds_bytes = (int)(compact.size() * sizeof(uint2));
     *mem = d;
     *bytes = total;
     return FQSL_OK;
@@ -369,11 +391,12 @@ extern "C" int fqsl_field_set_axes(fqsl_field* f, const void* xa, const void* ya
     size_t bytes = 0;
     const StretchGrid<float>* g32 = nullptr;
     const StretchGrid<double>* g64 = nullptr;
-    int rc = build_stretch(a, f->n, lookup, &mem, &bytes, &g32, &g64);
+    int lds_bytes = 0;
+    int rc = build_stretch(a, f->n, lookup, &mem, &bytes, &g32, &g64, &lds_bytes);
     if (rc) return rc;
     if (f->axes_mem) { (void)hipFree(f->axes_mem); f->bytes -= f->axes_bytes; }
     f->axes_mem = mem; f->axes_bytes = bytes; f->bytes += bytes;
-    f->sg32 = g32; f->sg64 = g64; f->stretched = true;
+    f->sg32 = g32; f->sg64 = g64; f->sg_lds_bytes = lds_bytes; f->stretched = true;
     return FQSL_OK;
 }
 
@@ -381,7 +404,7 @@ extern "C" int fqsl_field_stretched(const fqsl_field* f) { return f && f->stretc
 
 // ------------------------------------------------------------- tuning knobs
 // Read by every launch, written by fqsl_tune_set from any host thread.
-static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 64};
+static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 64, 1};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
@@ -397,6 +420,8 @@ extern "C" int fqsl_tune_set(int knob, int value) {
         return set_err(FQSL_ERR_INVALID, "order buckets must be a power of two in [2, 256] (got %d)", value);
     if (knob == FQSL_TUNE_GRID_PCT && (value < 1 || value > 1000))
         return set_err(FQSL_ERR_INVALID, "grid percentage %d not in 1..1000", value);
+    if (knob == FQSL_TUNE_STRETCH_LDS && value != 0 && value != 1)
+        return set_err(FQSL_ERR_INVALID, "stretch_lds must be 0 or 1 (got %d)", value);
     g_tune[knob] = value;
     return FQSL_OK;
 }
@@ -896,7 +921,7 @@ __device__ __forceinline__ void plain_write(const PlainLine<R>& L, const F& f, c
 
 // The stretched tracer writes B_start / B_end / B_this by the stretched
 // Interp3dxyzn (raw, TraceBlineAdaptiveStretch.cu:463-496).
-template <typename R, class F>
+template <typename R, class F, bool kL = false>
 __device__ __forceinline__ void stretch_write(const PlainLine<R>& L, const F& f, const FQSL_CONST StretchGrid<R>& g,
                                               const PlainParams& P, unsigned long long i, const PlainOut& o) {
     const float xs[3] = {(float)L.xs[0], (float)L.xs[1], (float)L.xs[2]};
@@ -911,16 +936,16 @@ __device__ __forceinline__ void stretch_write(const PlainLine<R>& L, const F& f,
     double B[3];
     if (o.bsx) {
         const double p[3] = {xs[0], xs[1], xs[2]};
-        stretch_raw_at<R, F>(f, g, p, B);
+        stretch_raw_at<R, F, kL>(f, g, p, B);
         o.bsx[i] = (float)B[0]; o.bsy[i] = (float)B[1]; o.bsz[i] = (float)B[2];
     }
     if (o.bex) {
         const double p[3] = {xe[0], xe[1], xe[2]};
-        stretch_raw_at<R, F>(f, g, p, B);
+        stretch_raw_at<R, F, kL>(f, g, p, B);
         o.bex[i] = (float)B[0]; o.bey[i] = (float)B[1]; o.bez[i] = (float)B[2];
     }
     const double p0[3] = {L.p0[0], L.p0[1], L.p0[2]};
-    stretch_raw_at<R, F>(f, g, p0, B);
+    stretch_raw_at<R, F, kL>(f, g, p0, B);
     const float b[3] = {(float)B[0], (float)B[1], (float)B[2]};
     if (o.btx) { o.btx[i] = b[0]; o.bty[i] = b[1]; o.btz[i] = b[2]; }
     if (o.bflag) {
@@ -970,6 +995,49 @@ __device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, co
     stretch_write<R>(L, f, grid_of(g), P, i, o);
 }
 
+// A float stretched grid whose compact lookup table each workgroup copies to
+// LDS at the start (`lds_bytes` of dynamic shared memory): the bin record of
+// a stage point is then an LDS read instead of a global load that the cell
+// load has to wait for.
+struct StretchLds {
+    const StretchGrid<float>* g;
+    int lds_bytes;
+};
+__device__ __forceinline__ const FQSL_CONST StretchGrid<float>& grid_of(const StretchLds& g) {
+    return *(const FQSL_CONST StretchGrid<float>*)g.g;
+}
+template <typename R, class F>
+__device__ __forceinline__ void line_init(PlainLine<R>& L, const F& f, const StretchLds& g, const PlainParams& P,
+                                          float x, float y, float z) {
+    stretch_init<R, F, true>(L, f, grid_of(g), P, x, y, z);
+}
+template <typename R, class F, class FJ>
+__device__ __forceinline__ void line_attempt(PlainLine<R>& L, const F& f, const StretchLds& g,
+                                             const PlainParams& P, const FJ& fj) {
+    stretch_attempt<R, F, FJ, true>(L, f, grid_of(g), P, fj);
+}
+template <typename R, class F>
+__device__ __forceinline__ void line_write(const PlainLine<R>& L, const F& f, const StretchLds& g,
+                                           const PlainParams& P, unsigned long long i, const PlainOut& o,
+                                           const DevScalars&) {
+    stretch_write<R, F, true>(L, f, grid_of(g), P, i, o);
+}
+
+// Kernel prologue per geometry: the LDS grid copies its table (every thread
+// of the block reaches it before any lookup).
+template <class G>
+__device__ __forceinline__ void grid_prologue(const G&) {}
+__device__ __forceinline__ void grid_prologue(const StretchLds& g) {
+    const FQSL_CONST StretchGrid<float>& sg = grid_of(g);
+    const int m = sg.compact_n;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) fqsl_stretch_lds[i] = sg.compact[i];
+    __syncthreads();
+}
+static int grid_lds_bytes(const UniformGrid&) { return 0; }
+template <typename R>
+static int grid_lds_bytes(const StretchRef<R>&) { return 0; }
+static int grid_lds_bytes(const StretchLds& g) { return g.lds_bytes; }
+
 template <class F>
 __device__ __forceinline__ void apply_plain_scalars(F& f, PlainParams& P, unsigned long long& n,
                                                     const DevScalars& ds, const float* ncross) {
@@ -995,6 +1063,7 @@ __global__ void __launch_bounds__(256) k_plain_persistent(F f, PlainParams P, Se
                                                           unsigned long long* queue, int refill_min, FJ fj, G g) {
     apply_plain_scalars(f, P, n, ds, ncross);
     fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
+    grid_prologue(g);
     PlainLine<R> L;
     bool active = false;
     bool drained = false;
@@ -1045,6 +1114,7 @@ k_plain_per_line(F f, PlainParams P, Seeds s, unsigned long long n,
                                                         G g) {
     apply_plain_scalars(f, P, n, ds, ncross);
     fj.n[0] = f.n[0]; fj.n[1] = f.n[1]; fj.n[2] = f.n[2];  // J lives on B's grid
+    grid_prologue(g);
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n; i += stride) {
         PlainLine<R> L;
@@ -1103,9 +1173,9 @@ static int per_line_grid(unsigned long long n_hint, int block, const DevScalars&
 }
 
 template <typename K>
-static int persistent_grid(K kernel, int* grid) {
+static int persistent_grid(K kernel, int* grid, int lds_bytes = 0) {
     int per_cu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds_bytes));
     if (per_cu < 1) per_cu = 1;
     *grid = per_cu * cu_count();
     return FQSL_OK;
@@ -1268,6 +1338,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
 static int per_line_block(const UniformGrid&) { return 64; }
 template <typename R>
 static int per_line_block(const StretchRef<R>&) { return 256; }
+static int per_line_block(const StretchLds&) { return 256; }
 
 template <typename R, class F, class FJ = NoCurrent, class G = UniformGrid>
 static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsigned long long n, const PlainOut& o,
@@ -1278,13 +1349,14 @@ static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsign
         int rc = queue_slot(&q);
         if (rc) return rc;
         int grid = 0;
-        rc = persistent_grid(k_plain_persistent<R, F, FJ, G>, &grid);
+        const int lds = grid_lds_bytes(g);
+        rc = persistent_grid(k_plain_persistent<R, F, FJ, G>, &grid, lds);
         if (rc) return rc;
         k_zero_u64<<<1, 64, 0, st>>>(q, kQueueSlotWidth);
         HIP_TRY(hipGetLastError());
         rc = record_start(st);
         if (rc) return rc;
-        k_plain_persistent<R, F, FJ, G><<<grid, 256, 0, st>>>(f, P, s, n, o, ds, ncross, q,
+        k_plain_persistent<R, F, FJ, G><<<grid, 256, lds, st>>>(f, P, s, n, o, ds, ncross, q,
                                                              g_tune[FQSL_TUNE_PLAIN_REFILL], fj, g);
         HIP_TRY(hipGetLastError());
         return record_end(st);
@@ -1293,8 +1365,8 @@ static int launch_plain(const F& f, const PlainParams& P, const Seeds& s, unsign
         int rc = record_start(st);
         if (rc) return rc;
         const int block = per_line_block(g);
-        k_plain_per_line<R, F, FJ, G><<<per_line_grid(n_hint, block, ds), block, 0, st>>>(f, P, s, n, o, ds, ncross,
-                                                                                        fj, g);
+        k_plain_per_line<R, F, FJ, G><<<per_line_grid(n_hint, block, ds), block, grid_lds_bytes(g), st>>>(
+            f, P, s, n, o, ds, ncross, fj, g);
         HIP_TRY(hipGetLastError());
         return record_end(st);
     }
@@ -2021,6 +2093,272 @@ extern "C" int fqsl_scott_batch_destroy(fqsl_scott_batch* b) {
     return FQSL_OK;
 }
 
+// ------------------------------------------------------------- plane maps
+// Seeds of a plane (fqsl_plane), one thread per pixel, row-major.  NumPy's
+// linspace(a, b, n)[i] is i * ((b - a) / (n - 1)) + a in double with the last
+// point b, and plane_seeds_abc forms A + (B - A) u + (C - A) v in double; both
+// without fused multiply-adds, so the rounding to float gives NumPy's seeds.
+#pragma clang fp contract(off)
+__device__ __forceinline__ double np_linspace(double a, double b, int n, int i) {
+    if (n <= 1) return 0.0 * (b - a) + a;  // one point: arange(1) * delta + a
+    if (i == n - 1) return b;
+    const double step = (b - a) / (double)(n - 1);
+    return (double)i * step + a;
+}
+
+__global__ void __launch_bounds__(256) k_plane_seeds(fqsl_plane pl, float* __restrict__ x, float* __restrict__ y,
+                                                     float* __restrict__ z) {
+    const long n = (long)pl.n_u * pl.n_v;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % pl.n_u), j = (int)(t / pl.n_u);
+        const double* q = pl.p;
+        if (pl.kind == FQSL_PLANE_Z) {
+            x[t] = (float)np_linspace(q[0], q[1], pl.n_u, i);
+            y[t] = (float)np_linspace(q[2], q[3], pl.n_v, j);
+            z[t] = (float)q[4];
+        } else {
+            const double u = np_linspace(0.0, 1.0, pl.n_u, i), v = np_linspace(0.0, 1.0, pl.n_v, j);
+            float* o[3] = {x, y, z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double a = q[c], ba = q[3 + c] - a, ca = q[6 + c] - a;
+                o[c][t] = (float)((a + ba * u) + ca * v);
+            }
+        }
+    }
+}
+#pragma clang fp contract(on)
+
+extern "C" int fqsl_qmap_plane(const fqsl_field* field, const fqsl_plane* plane, const fqsl_scott_params* params,
+                               const fqsl_scott_out* out, void* stream) {
+    if (!field || !plane || !params || !out) return set_err(FQSL_ERR_INVALID, "NULL field/plane/params/out");
+    if (plane->kind != FQSL_PLANE_Z && plane->kind != FQSL_PLANE_ABC)
+        return set_err(FQSL_ERR_INVALID, "plane kind %d", plane->kind);
+    if (plane->n_u < 0 || plane->n_v < 0) return set_err(FQSL_ERR_INVALID, "plane of %d x %d pixels", plane->n_u, plane->n_v);
+    for (int c = 0; c < 9; ++c)
+        if (!(plane->p[c] - plane->p[c] == 0.0)) return set_err(FQSL_ERR_INVALID, "plane coordinate %d is not finite", c);
+    const unsigned long long n = (unsigned long long)plane->n_u * (unsigned long long)plane->n_v;
+    if (n == 0) return FQSL_OK;
+    hipStream_t st = (hipStream_t)stream;
+    float* seeds = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&seeds, 3 * sizeof(float) * n, st));
+    k_plane_seeds<<<grid_for((long)n, 256), 256, 0, st>>>(*plane, seeds, seeds + n, seeds + 2 * n);
+    hipError_t e = hipGetLastError();
+    int rc = e == hipSuccess ? fqsl_trace_scott(field, params, n, seeds, seeds + n, seeds + 2 * n, out, stream)
+                             : set_err(FQSL_ERR_HIP, "plane seeds: %s", hipGetErrorString(e));
+    e = hipFreeAsync(seeds, st);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "hipFreeAsync: %s", hipGetErrorString(e));
+    return FQSL_OK;
+}
+
+// ---------------------------------------------------------------- serving
+// One consumer thread per server drains the queue in batches: it takes the
+// waiting requests in submission order up to max_lines lines, traces them as
+// one fqsl_scott_batch on its stream, waits for the stream and marks them
+// done with the batch's status.  Its workspace grows to the largest batch.
+struct ServerReq {
+    uint64_t id;
+    fqsl_scott_request req;
+    hipEvent_t ready;  // recorded on the submitter's stream
+};
+
+struct ServerDone {
+    int status;
+    std::string err;
+};
+
+struct fqsl_server {
+    int device = 0;
+    int precision = FQSL_PREC_FP32C;
+    uint64_t max_lines = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServerReq> queue;
+    std::unordered_map<uint64_t, ServerDone> done;   // finished, not yet waited for
+    std::unordered_set<uint64_t> pending;            // queued or running
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    hipStream_t stream = nullptr;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+};
+
+static int server_trace(fqsl_server* S, const std::vector<ServerReq>& take) {
+    std::vector<fqsl_scott_request> reqs;
+    for (const ServerReq& r : take) {
+        reqs.push_back(r.req);
+        HIP_TRY(hipStreamWaitEvent(S->stream, r.ready, 0));
+    }
+    const size_t need = fqsl_scott_batch_workspace_bytes(reqs.data(), (int)reqs.size(), S->precision);
+    if (need > S->ws_bytes) {
+        if (S->ws) HIP_TRY(hipFree(S->ws));
+        S->ws = nullptr;
+        S->ws_bytes = 0;
+        HIP_TRY(hipMalloc(&S->ws, need));
+        S->ws_bytes = need;
+    }
+    fqsl_scott_batch* b = nullptr;
+    int rc = fqsl_scott_batch_create(reqs.data(), (int)reqs.size(), S->precision, S->ws, S->ws_bytes, S->stream, &b);
+    if (rc) return rc;
+    rc = fqsl_scott_batch_run(b, S->stream);
+    const hipError_t e = hipStreamSynchronize(S->stream);
+    fqsl_scott_batch_destroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "server batch: %s", hipGetErrorString(e));
+    return FQSL_OK;
+}
+
+static void server_loop(fqsl_server* S) {
+    (void)hipSetDevice(S->device);
+    for (;;) {
+        std::vector<ServerReq> take;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [S] { return S->stop || !S->queue.empty(); });
+            if (S->queue.empty()) return;  // stopping and drained
+            uint64_t lines = 0;
+            while (!S->queue.empty() && (take.empty() || lines + S->queue.front().req.n <= S->max_lines)) {
+                lines += S->queue.front().req.n;
+                take.push_back(S->queue.front());
+                S->queue.pop_front();
+            }
+        }
+        const int rc = server_trace(S, take);
+        const std::string err = rc ? std::string(g_err) : std::string();
+        if (rc) (void)hipStreamSynchronize(S->stream);  // the waits on the events have ended
+        for (const ServerReq& r : take) (void)hipEventDestroy(r.ready);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (const ServerReq& r : take) {
+                S->pending.erase(r.id);
+                S->done[r.id] = ServerDone{rc, err};
+            }
+        }
+        S->cv_done.notify_all();
+    }
+}
+
+extern "C" int fqsl_server_create(int device, int precision, uint64_t max_lines, fqsl_server** server) {
+    if (!server) return set_err(FQSL_ERR_INVALID, "server is NULL");
+    *server = nullptr;
+    if (precision != FQSL_PREC_FP32C && precision != FQSL_PREC_FP64 && precision != FQSL_PREC_MIXED)
+        return set_err(FQSL_ERR_INVALID, "precision %d", precision);
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return set_err(FQSL_ERR_INVALID, "device %d of %d", device, count);
+    fqsl_server* S = new fqsl_server();
+    S->device = device;
+    S->precision = precision;
+    S->max_lines = max_lines ? max_lines : (1ull << 24);
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    (void)hipSetDevice(prev);
+    if (e != hipSuccess) {
+        delete S;
+        return set_err(FQSL_ERR_HIP, "server stream: %s", hipGetErrorString(e));
+    }
+    S->worker = std::thread(server_loop, S);
+    *server = S;
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_submit(fqsl_server* S, const fqsl_scott_request* req, void* stream, uint64_t* id) {
+    if (!S || !req || !id) return set_err(FQSL_ERR_INVALID, "NULL server/request/id");
+    int rc = check_request(*req, S->precision, 0);
+    if (rc) return rc;
+    hipEvent_t ready = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    const hipError_t e = hipEventRecord(ready, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(ready);
+        return set_err(FQSL_ERR_HIP, "hipEventRecord: %s", hipGetErrorString(e));
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) {
+            (void)hipEventDestroy(ready);
+            return set_err(FQSL_ERR_INVALID, "server is stopping");
+        }
+        *id = S->next_id++;
+        S->queue.push_back(ServerReq{*id, *req, ready});
+        S->pending.insert(*id);
+    }
+    S->cv_work.notify_one();
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_poll(fqsl_server* S, uint64_t id, int* done) {
+    if (!S || !done) return set_err(FQSL_ERR_INVALID, "NULL server/done");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->done.count(id)) { *done = 1; return FQSL_OK; }
+    if (S->pending.count(id)) { *done = 0; return FQSL_OK; }
+    return set_err(FQSL_ERR_INVALID, "unknown request id %llu", (unsigned long long)id);
+}
+
+extern "C" int fqsl_wait(fqsl_server* S, uint64_t id) {
+    if (!S) return set_err(FQSL_ERR_INVALID, "server is NULL");
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (!S->done.count(id) && !S->pending.count(id))
+        return set_err(FQSL_ERR_INVALID, "unknown request id %llu", (unsigned long long)id);
+    S->cv_done.wait(lk, [S, id] { return S->done.count(id) > 0; });
+    const ServerDone d = S->done[id];
+    S->done.erase(id);
+    if (d.status) return set_err(d.status, "%s", d.err.c_str());
+    return FQSL_OK;
+}
+
+extern "C" int fqsl_server_destroy(fqsl_server* S) {
+    if (!S) return FQSL_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(S->device);
+    if (S->ws) (void)hipFree(S->ws);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
+    (void)hipSetDevice(prev);
+    delete S;
+    return FQSL_OK;
+}
+
+// Largest LDS table the stretched kernels copy per workgroup: two 256-thread
+// workgroups (two waves per SIMD) still fit a CU's 160 KiB.
+constexpr int kStretchLdsMax = 64 * 1024;
+static bool stretch_lds_fits(int bytes) {
+    return g_tune[FQSL_TUNE_STRETCH_LDS] && bytes > 0 && bytes <= kStretchLdsMax;
+}
+
+// FP32 stretched launches over geometry G (StretchRef<float> or StretchLds).
+template <class G>
+static int trace_stretched_f32(const fqsl_field* field, const PlainParams& P, const Seeds& s, unsigned long long n,
+                               const PlainOut& o, int schedule, hipStream_t st, const DevScalars& ds, const G& g) {
+    const int* nn = field->n;
+    const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
+    if (field->cell32) {
+        FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
+        if (o.tw) {
+            if (!field->curcell) return set_err(FQSL_ERR_INVALID, "current density has no cell layout");
+            const FieldCell32 fjc{field->curcell, {nn[0], nn[1], nn[2]}};
+            return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fjc, g);
+        }
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
+    }
+    if (field->soa32[0]) {
+        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
+        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, g);
+        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
+    }
+    return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+}
+
 // The stretched tracer on a resident field (RKF45 only, as the reference).
 static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, const Seeds& s, unsigned long long n,
                                  const PlainOut& o, int precision, int schedule, hipStream_t st,
@@ -2038,23 +2376,9 @@ static int trace_plain_stretched(const fqsl_field* field, const PlainParams& P, 
         }
         return launch_plain<double>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
     }
-    const StretchRef<float> g{field->sg32};
-    const FieldSoA<float> fj{field->cur32[0], field->cur32[1], field->cur32[2], {nn[0], nn[1], nn[2]}};
-    if (field->cell32) {
-        FieldCell32 f{field->cell32, {nn[0], nn[1], nn[2]}};
-        if (o.tw) {
-            if (!field->curcell) return set_err(FQSL_ERR_INVALID, "current density has no cell layout");
-            const FieldCell32 fjc{field->curcell, {nn[0], nn[1], nn[2]}};
-            return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fjc, g);
-        }
-        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
-    }
-    if (field->soa32[0]) {
-        FieldSoA<float> f{field->soa32[0], field->soa32[1], field->soa32[2], {nn[0], nn[1], nn[2]}};
-        if (o.tw) return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, fj, g);
-        return launch_plain<float>(f, P, s, n, o, ds, nullptr, schedule, st, n, NoCurrent{}, g);
-    }
-    return set_err(FQSL_ERR_INVALID, "field has no FP32 layout");
+    if (stretch_lds_fits(field->sg_lds_bytes))
+        return trace_stretched_f32(field, P, s, n, o, schedule, st, ds, StretchLds{field->sg32, field->sg_lds_bytes});
+    return trace_stretched_f32(field, P, s, n, o, schedule, st, ds, StretchRef<float>{field->sg32});
 }
 
 static int trace_plain_impl(const fqsl_field* field, const fqsl_plain_params* p, uint64_t n,
@@ -2222,6 +2546,7 @@ struct DropinAxes {
     int rule = -1;
     void* mem = nullptr;
     const StretchGrid<float>* g = nullptr;
+    int lds_bytes = 0;
 };
 static std::mutex g_dropin_mu;
 static DropinAxes g_dropin_axes[64];
@@ -2270,6 +2595,7 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
     // then covers this launch, before it frees them.
     std::lock_guard<std::mutex> lock(g_dropin_mu);
     StretchRef<float> g{nullptr};
+    int lds_bytes = 0;
     {
         DropinAxes& c = g_dropin_axes[dev];
         const bool same = c.mem && c.rule == rule && c.a[0] == a[0] && c.a[1] == a[1] && c.a[2] == a[2];
@@ -2278,7 +2604,8 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
             size_t bytes = 0;
             const StretchGrid<double>* g64 = nullptr;
             const StretchGrid<float>* g32 = nullptr;
-            rc = build_stretch(a, shape, rule, &mem, &bytes, &g32, &g64);
+            int lb = 0;
+            rc = build_stretch(a, shape, rule, &mem, &bytes, &g32, &g64, &lb);
             if (rc) return rc;
             if (c.mem) {  // launches enqueued earlier, on any stream, may still read the old tables
                 HIP_TRY(hipDeviceSynchronize());
@@ -2286,10 +2613,12 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
             }
             c.mem = mem;
             c.g = g32;
+            c.lds_bytes = lb;
             c.rule = rule;
             for (int q = 0; q < 3; ++q) c.a[q] = a[q];
         }
         g.g = c.g;
+        lds_bytes = c.lds_bytes;
     }
     PlainParams P;
     P.method = FQSL_METHOD_RKF45; P.s0 = 0.0; P.tol_coef = 1.0;
@@ -2301,10 +2630,13 @@ extern "C" int fqsl_TraceAllBlineStretch(float* Bx, float* By, float* Bz, int* B
     DevScalars ds{BshapeN, s_len, N, tol_coef, curB_flag, twist};
     FieldSoA<float> f{Bx, By, Bz, {shape[0], shape[1], shape[2]}};
     const unsigned long long n_hint = (unsigned long long)kPerLineMaxBlocks * 256;
-    if (twist_on) {
-        const FieldSoA<float> fj{curB_x, curB_y, curB_z, {shape[0], shape[1], shape[2]}};
-        return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, fj, g);
+    const FieldSoA<float> fj{curB_x, curB_y, curB_z, {shape[0], shape[1], shape[2]}};
+    if (stretch_lds_fits(lds_bytes)) {
+        const StretchLds gl{g.g, lds_bytes};
+        if (twist_on) return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, fj, gl);
+        return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, NoCurrent{}, gl);
     }
+    if (twist_on) return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, fj, g);
     return launch_plain<float>(f, P, s, 0ull, o, ds, inp_cross_dir, schedule, st, n_hint, NoCurrent{}, g);
 }
 

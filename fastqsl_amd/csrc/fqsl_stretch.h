@@ -106,6 +106,7 @@ struct StretchAxis {
     T bin_scale;                        // nbins / (hi - lo)
     T uni_width;                        // (hi - lo) / n in T, GetIndexUniStretch's cell width
     int n, nbins;
+    int lbin, lcell;                    // this axis's first entries in the compact table (below)
 };
 
 template <typename T>
@@ -115,6 +116,14 @@ struct StretchGrid {
     double lo[3], hi[3];            // the box, for the exit test
     double land_lo[3], land_hi[3];  // landing planes of min / max face exits
     double len_lim;                 // 2 x (sum of the three spans), :330-332
+    // Compact copy of the float bin records for workgroup-local (LDS) lookup,
+    // 8 bytes an entry: per bin {split bits, m_lo | m_hi << 16} (m_lo =
+    // 0xffff for a bin holding several breakpoints), then per cell m
+    // {a[m], 1 / (a[m+1] - a[m])} as float bits -- the a_lo/iw_lo, a_hi/iw_hi
+    // of a BinRec are the cell values of its m_lo, m_hi.  compact_n = 0 when
+    // an axis has more than 65535 cells.
+    const uint2* compact;
+    int compact_n;
 };
 
 // What the kernels take: the grid lives in device memory beside its tables
@@ -173,6 +182,38 @@ FQSL_HD void stretch_locate(const FQSL_CONST StretchAxis<T>& A, int rule, T phi,
     w = s.w;
 }
 
+FQSL_HD float float_of_bits(unsigned u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// The workgroup's copy of StretchGrid<float>::compact (dynamic LDS), filled
+// by the kernels that trace with kL = true before their first lookup.
+extern __shared__ uint2 fqsl_stretch_lds[];
+
+// stretch_locate<float> from the LDS copy: the same cell and weight bit for
+// bit (the same record fields, the same expressions).
+FQSL_HD void stretch_locate_lds(const FQSL_CONST StretchAxis<float>& A, int rule, float phi, float plo, int& c,
+                                float& w) {
+    float p = phi + plo;
+    if (!(p > A.lo)) { p = A.clo; phi = A.clo; plo = 0.f; }
+    if (!(p < A.hi)) { p = A.chi; phi = A.chi; plo = 0.f; }
+    int b = to_int((p - A.lo) * A.bin_scale);
+    b = b < 0 ? 0 : (b >= A.nbins ? A.nbins - 1 : b);
+    const uint2 br = fqsl_stretch_lds[A.lbin + b];
+    const int m_lo = (int)(br.y & 0xffffu), m_hi = (int)(br.y >> 16);
+    const bool multi = m_lo == 0xffff;
+    const bool up = p > float_of_bits(br.x);  // on a node: the cell below it
+    c = (up || multi) ? m_hi : m_lo;
+    const uint2 cw = fqsl_stretch_lds[A.lcell + c];
+    w = ((phi - float_of_bits(cw.x)) + plo) * float_of_bits(cw.y);
+    if (!multi) return;
+    const CellWeight<float> s = stretch_search<float>(&A, b, rule, p, phi, plo);
+    c = s.c;
+    w = s.w;
+}
+
 // Leading part and remainder of a stage point x + delta.
 FQSL_HD void stage_parts(const Pos<float>& x, const float* d, float* phi, float* plo) {
 #pragma unroll
@@ -188,39 +229,43 @@ FQSL_HD void point_parts(const double* x, R* phi, R* plo) {
     for (int a = 0; a < 3; ++a) { phi[a] = (R)x[a]; plo[a] = (R)(x[a] - (double)phi[a]); }
 }
 
-// Raw B (Interp3dxyzn(..., false), :82-194) at a split stage point.
-template <typename R, class F>
+// Raw B (Interp3dxyzn(..., false), :82-194) at a split stage point.  kL:
+// look the cells up in the workgroup's LDS copy of the tables (float only).
+template <typename R, class F, bool kL = false>
 FQSL_HD void stretch_raw(const F& f, const FQSL_CONST StretchGrid<R>& g, const R* phi, const R* plo, R B[3]) {
     int c[3];
     R w[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) stretch_locate<R>(g.ax[a], g.rule, phi[a], plo[a], c[a], w[a]);
+    for (int a = 0; a < 3; ++a) {
+        if constexpr (kL) stretch_locate_lds(g.ax[a], g.rule, phi[a], plo[a], c[a], w[a]);
+        else stretch_locate<R>(g.ax[a], g.rule, phi[a], plo[a], c[a], w[a]);
+    }
     R cc[3][8];
     f.template load<R>(c[0], c[1], c[2], cc);
     field_value<F, R>(cc, w[0], w[1], w[2], B);
 }
 
-template <typename R, class F>
+template <typename R, class F, bool kL = false>
 FQSL_HD void stretch_unit(const F& f, const FQSL_CONST StretchGrid<R>& g, const R* phi, const R* plo, R b[3]) {
     R B[3];
-    stretch_raw<R, F>(f, g, phi, plo, B);
+    stretch_raw<R, F, kL>(f, g, phi, plo, B);
     const R inb = rsq(dot3(B, B));
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
 }
 
-template <typename R, class F>
+template <typename R, class F, bool kL = false>
 FQSL_HD void stretch_unit_at(const F& f, const FQSL_CONST StretchGrid<R>& g, const double* x, R b[3]) {
     R phi[3], plo[3];
     point_parts<R>(x, phi, plo);
-    stretch_unit<R, F>(f, g, phi, plo, b);
+    stretch_unit<R, F, kL>(f, g, phi, plo, b);
 }
 
-template <typename R, class F>
+template <typename R, class F, bool kL = false>
 FQSL_HD void stretch_raw_at(const F& f, const FQSL_CONST StretchGrid<R>& g, const double* x, double B[3]) {
     R phi[3], plo[3], Bv[3];
     point_parts<R>(x, phi, plo);
-    stretch_raw<R, F>(f, g, phi, plo, Bv);
+    stretch_raw<R, F, kL>(f, g, phi, plo, Bv);
     B[0] = Bv[0]; B[1] = Bv[1]; B[2] = Bv[2];
 }
 
@@ -246,7 +291,7 @@ FQSL_HD int check_flag_box(const double* lo, const double* hi, const double* p) 
 // step-count limit, len_lim from the spans, and a landing twist term
 // J.b/|b|^2 (b the normalised B at p1, :403-406) that the uniform RKF45
 // tracer has commented out.
-template <typename R, class F>
+template <typename R, class F, bool kL = false>
 FQSL_HD void stretch_init(PlainLine<R>& L, const F& f, const FQSL_CONST StretchGrid<R>& g, const PlainParams& P, float px,
                           float py, float pz) {
     L.p0[0] = px; L.p0[1] = py; L.p0[2] = pz;
@@ -259,7 +304,7 @@ FQSL_HD void stretch_init(PlainLine<R>& L, const F& f, const FQSL_CONST StretchG
     L.flag_e = L.flag_s = kClassLimit;
     const double p0d[3] = {px, py, pz};
     R b0[3];
-    stretch_unit_at<R, F>(f, g, p0d, b0);
+    stretch_unit_at<R, F, kL>(f, g, p0d, b0);
     const double bn = fabs((double)b0[0] * P.ncross[0] + (double)b0[1] * P.ncross[1] + (double)b0[2] * P.ncross[2]);
     L.tol = (bn <= 0.05 ? kTolBase / 8e3 : kTolBase * pow(bn, 1.5)) * P.tol_coef;  // :345-352
     L.dir = 1;
@@ -267,7 +312,7 @@ FQSL_HD void stretch_init(PlainLine<R>& L, const F& f, const FQSL_CONST StretchG
 }
 
 // RK4_boundary (:274-296) on the stretched interpolant.
-template <typename R, class F>
+template <typename R, class F, bool kL = false>
 FQSL_HD void stretch_rk4_boundary(const F& f, const FQSL_CONST StretchGrid<R>& g, const double* p, double s, int dim,
                                     double out[3]) {
     double k[4][3], t[3];
@@ -277,7 +322,7 @@ FQSL_HD void stretch_rk4_boundary(const F& f, const FQSL_CONST StretchGrid<R>& g
 #pragma unroll
         for (int a = 0; a < 3; ++a) t[a] = st == 0 ? p[a] : p[a] + s * k[st - 1][a] * cf[st];
         R b[3];
-        stretch_unit_at<R, F>(f, g, t, b);
+        stretch_unit_at<R, F, kL>(f, g, t, b);
         const double bd = (double)b[dim];
 #pragma unroll
         for (int a = 0; a < 3; ++a) k[st][a] = (double)b[a] / bd;
@@ -306,7 +351,7 @@ FQSL_HD void rkf45_offset(int st, const R k[6][3], R t[3]) {
     }
 }
 
-template <typename R, class F, class FJ = NoCurrent>
+template <typename R, class F, class FJ = NoCurrent, bool kL = false>
 FQSL_HD void stretch_attempt(PlainLine<R>& L, const F& f, const FQSL_CONST StretchGrid<R>& g, const PlainParams& P,
                              const FJ& fj = FJ{}) {
     const R sc = (R)(L.s * L.dir);
@@ -319,7 +364,7 @@ FQSL_HD void stretch_attempt(PlainLine<R>& L, const F& f, const FQSL_CONST Stret
         rkf45_offset<R>(st, k, t);
         R phi[3], plo[3];
         stage_parts(L.x, t, phi, plo);
-        stretch_unit<R, F>(f, g, phi, plo, k[st]);
+        stretch_unit<R, F, kL>(f, g, phi, plo, k[st]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) k[st][a] *= sc;
     }
@@ -357,8 +402,8 @@ FQSL_HD void stretch_attempt(PlainLine<R>& L, const F& f, const FQSL_CONST Stret
     double tw_rate = 0.0, Jp1[3] = {0.0, 0.0, 0.0};
     if constexpr (has_current<FJ>()) {
         double Bp1[3];
-        stretch_raw_at<R, FJ>(fj, g, xa, Jp1);
-        stretch_raw_at<R, F>(f, g, xa, Bp1);
+        stretch_raw_at<R, FJ, kL>(fj, g, xa, Jp1);
+        stretch_raw_at<R, F, kL>(f, g, xa, Bp1);
         tw_rate = dot3(Jp1, Bp1) / dot3(Bp1, Bp1) * kQuarterOverPi;
         L.tw += tw_rate * dl;
     }
@@ -372,14 +417,14 @@ FQSL_HD void stretch_attempt(PlainLine<R>& L, const F& f, const FQSL_CONST Stret
             if (fabs(pa - pb) > 1e-3) {  // :378-406
                 const double pm = (flag % 2 == 1) ? g.land_lo[dim] : g.land_hi[dim];
                 R b1[3], b2[3];
-                stretch_unit_at<R, F>(f, g, xa, b1);
-                stretch_unit_at<R, F>(f, g, xbv, b2);
+                stretch_unit_at<R, F, kL>(f, g, xa, b1);
+                stretch_unit_at<R, F, kL>(f, g, xbv, b2);
                 double out[3];
                 if (fabs((double)b1[dim]) < 0.2 || fabs((double)b2[dim]) < 0.2) {
 #pragma unroll
                     for (int a = 0; a < 3; ++a) out[a] = (xa[a] * (pb - pm) + xbv[a] * (pm - pa)) / (pb - pa);
                 } else {
-                    stretch_rk4_boundary<R, F>(f, g, xa, pm - pa, dim, out);
+                    stretch_rk4_boundary<R, F, kL>(f, g, xa, pm - pa, dim, out);
                     L.evals += 4;
                 }
                 plain_record(L, out, flag);
@@ -579,6 +624,38 @@ inline const char* build_axis_tables(const double* a, int n, int rule, AxisTable
     const int m32 = fill_bins<float>(s32, t.ref, a, n, t.lo, span, t.nbins, t.brec32);
     t.multi = m64 > m32 ? m64 : m32;
     return nullptr;
+}
+
+// Entries of the compact table (StretchGrid::compact) for three axes: the
+// float bin records, then the float cell records; lbin/lcell get each axis's
+// first entry.  Returns false when an axis has more than 65535 cells.
+inline bool build_compact(const AxisTables (&t)[3], std::vector<uint2>& out, int (&lbin)[3], int (&lcell)[3]) {
+    out.clear();
+    for (int q = 0; q < 3; ++q)
+        if (t[q].n - 1 > 0xfffe) return false;
+    for (int q = 0; q < 3; ++q) {
+        lbin[q] = (int)out.size();
+        for (const BinRec<float>& r : t[q].brec32) {
+            unsigned split;
+            memcpy(&split, &r.split, 4);
+            const unsigned lo = r.m_lo < 0 ? 0xffffu : (unsigned)r.m_lo;
+            out.push_back(make_uint2(split, lo | ((unsigned)r.m_hi << 16)));
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        lcell[q] = (int)out.size();
+        for (int m = 0; m + 1 < t[q].n; ++m) {
+            // the float values fill_bins stores as a_lo / iw_lo for cell m
+            const float a = (float)t[q].rec64[m].x;
+            const double a0 = t[q].rec64[m].x, a1 = t[q].rec64[m].y;
+            const float iw = (float)(1.0 / (a1 - a0));
+            unsigned ua, uw;
+            memcpy(&ua, &a, 4);
+            memcpy(&uw, &iw, 4);
+            out.push_back(make_uint2(ua, uw));
+        }
+    }
+    return true;
 }
 
 // The grid view over tables placed at brec[q] (BinRec<T> per bin of axis q),

@@ -186,10 +186,14 @@ struct FieldCell32 {
     static constexpr bool kCoef = true;  // load() yields coefficients
     const float4* cells;
     int n[3];
+    // first float of cell (i, j, k): 3 components x 8 coefficients, then padding
+    FQSL_HD const FQSL_GLOBAL float* cell(int i, int j, int k) const {
+        const long cx = n[0] - 1, cxy = (long)(n[0] - 1) * (n[1] - 1);
+        return (const FQSL_GLOBAL float*)(cells + 8 * ((long)k * cxy + (long)j * cx + i));
+    }
     template <typename R>
     FQSL_HD void load(int i, int j, int k, R c[3][8]) const {
-        const long cx = n[0] - 1, cxy = (long)(n[0] - 1) * (n[1] - 1);
-        const FQSL_GLOBAL float4* p = (const FQSL_GLOBAL float4*)(cells + 8 * ((long)k * cxy + (long)j * cx + i));
+        const FQSL_GLOBAL float4* p = (const FQSL_GLOBAL float4*)cell(i, j, k);
 #pragma unroll
         for (int comp = 0; comp < 3; ++comp) {
             const float4 lo = p[2 * comp], hi = p[2 * comp + 1];
@@ -239,6 +243,37 @@ FQSL_HD Axis<R> axis_setup(int i, R r, int n) {
     else { a.am = r; if (i >= 1) a.amp = h - r; }
     if (r + h <= (R)1) a.ap = h;
     else { a.ap = (R)1 - r; if (i + 1 <= n - 2) a.apn = (r + h) - (R)1; }
+    return a;
+}
+
+// axis_setup as selects, no branches: the same values bit for bit (every
+// length is the same expression on the same operands), for code where only a
+// few lanes of a wavefront need it and a branch tree would cost each of them
+// the exec-mask bookkeeping of every arm.
+template <typename R>
+FQSL_HD Axis<R> axis_setup_sel(int i, R r, int n) {
+    const R h = (R)kFdH;
+    const bool nan = r != r;
+    const bool low = i < 0 || (i == 0 && r == (R)0);  // x <= 0 (NaN positions have i < 0)
+    const bool high = i >= n - 1;                      // x >= N-1
+    const bool mid = !nan && !low && !high;
+    // 0 < x < N-1
+    const R am_mid = (r >= h) ? h : r;
+    const R amp_mid = (!(r >= h) && i >= 1) ? h - r : (R)0;
+    const bool fits = r + h <= (R)1;
+    const R ap_mid = fits ? h : (R)1 - r;
+    const R apn_mid = (!fits && i + 1 <= n - 2) ? (r + h) - (R)1 : (R)0;
+    // x <= 0 and x >= N-1
+    const R ap_low = (i == 0) ? h : ((i == -1 && r > (R)1 - h) ? (r - (R)1) + h : (R)0);
+    const R e = (R)(i - (n - 1)) + r;
+    const R am_high = (e < h) ? (h - e) : (R)0;
+    Axis<R> a;
+    a.c = (nan || low) ? 0 : (high ? n - 2 : i);
+    a.w = (nan || mid) ? r : (low ? (R)0 : (R)1);
+    a.ap = nan ? h : (low ? ap_low : (high ? (R)0 : ap_mid));
+    a.am = nan ? h : (low ? (R)0 : (high ? am_high : am_mid));
+    a.apn = mid ? apn_mid : (R)0;
+    a.amp = mid ? amp_mid : (R)0;
     return a;
 }
 
@@ -386,20 +421,72 @@ FQSL_HD Pt<R> pt_from_double(const double* x) {
     return p;
 }
 
+// Neighbour-cell data fetched together with a stage point's own cell, for
+// the coefficient-cell float path: the first axis (lowest index) whose +-h
+// sample crosses a grid plane -- the same test axis_setup applies, apn != 0 or
+// amp != 0 -- and, per component, the four coefficients that the edge
+// derivative along that axis reads in the cell across the plane:
+//   d_J = (kA + kB s) + t (kC + kD s)
+// with (A, B, C, D; s, t) = (1, 4, 6, 7; y, z), (2, 4, 5, 7; x, z),
+// (3, 5, 6, 7; y, x) for J = 0, 1, 2 (coef_edge).  Issued before the own
+// cell is consumed, so its latency overlaps that load's instead of following
+// it; an axis that crosses when a lower one already does loads in its own
+// branch.
+struct NbCoef {
+    int j;  // axis, or -1
+    float k[3][4];
+};
+
+// Stand-in for the paths that load their crossing neighbours in the branch.
+struct NbNone {
+    static constexpr int j = -1;
+    float k[3][4];
+};
+
+template <class F>
+FQSL_HD void nb_fetch(const F& f, const Pt<float>& p, const int* c, NbCoef& nb) {
+    const float h = (float)kFdH;
+    nb.j = -1;
+    int step = 0;
+#pragma unroll
+    for (int a = 2; a >= 0; --a) {
+        const int i = p.i[a], n = f.n[a];
+        const float r = p.r[a];
+        const bool mid = !(r != r) && !(i < 0 || (i == 0 && r == 0.f)) && !(i >= n - 1);
+        const bool up = mid && !(r + h <= 1.f) && i + 1 <= n - 2;
+        const bool dn = mid && !(r >= h) && i >= 1;
+        if (up || dn) { nb.j = a; step = up ? 1 : -1; }
+    }
+    if (nb.j >= 0) {
+        int ci[3] = {c[0], c[1], c[2]};
+        ci[nb.j] += step;
+        const FQSL_GLOBAL float* q = f.cell(ci[0], ci[1], ci[2]);
+        const int kA = 1 + nb.j, kB = nb.j == 2 ? 5 : 4, kC = nb.j == 1 ? 5 : 6;
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) {
+            nb.k[comp][0] = q[8 * comp + kA];
+            nb.k[comp][1] = q[8 * comp + kB];
+            nb.k[comp][2] = q[8 * comp + kC];
+            nb.k[comp][3] = q[8 * comp + 7];
+        }
+    }
+}
+
 // Row J of grad b at p from the cell's B and edge derivative dj along J:
 // the closed form on a simple axis, else the exact difference quotient, which
 // takes the neighbour cell's edge derivative where the +-h sample crosses a
 // grid plane (crossing on both sides at once would need 2h > 1 cell).
-template <int J, typename R, class F>
+template <int J, typename R, class F, class Nb>
 FQSL_HD void scott_axis(const F& f, const Pt<R>& p, const int* c, const R* w, bool simple, const R B[3],
-                        const R b[3], R inb, const R dj[3], R out[3]) {
+                        const R b[3], R inb, const R dj[3], const Nb& nb, R out[3]) {
     if (simple) {
         const R bd = dot3(b, dj);
 #pragma unroll
         for (int q = 0; q < 3; ++q) out[q] = (dj[q] - b[q] * bd) * inb;
         return;
     }
-    const Axis<R> ax = axis_setup<R>(p.i[J], p.r[J], f.n[J]);
+    constexpr bool kPre = std::is_same<Nb, NbCoef>::value;
+    const Axis<R> ax = kPre ? axis_setup_sel<R>(p.i[J], p.r[J], f.n[J]) : axis_setup<R>(p.i[J], p.r[J], f.n[J]);
     R ap[3], am[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -409,11 +496,18 @@ FQSL_HD void scott_axis(const F& f, const Pt<R>& p, const int* c, const R* w, bo
     if (ax.apn != (R)0 || ax.amp != (R)0) {  // the +-h sample crosses a grid plane
         const bool up = ax.apn != (R)0;
         const R coef = up ? ax.apn : ax.amp;
-        int ci[3] = {c[0], c[1], c[2]};
-        ci[J] += up ? 1 : -1;
-        R cn[3][8], dn[3];
-        f.template load<R>(ci[0], ci[1], ci[2], cn);
-        field_edge<F, J, R>(cn, w[0], w[1], w[2], dn);
+        R dn[3];
+        if (kPre && nb.j == J) {
+            const R s = J == 1 ? w[0] : w[1], t = J == 2 ? w[0] : w[2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dn[q] = (nb.k[q][0] + nb.k[q][1] * s) + t * (nb.k[q][2] + nb.k[q][3] * s);
+        } else {
+            int ci[3] = {c[0], c[1], c[2]};
+            ci[J] += up ? 1 : -1;
+            R cn[3][8];
+            f.template load<R>(ci[0], ci[1], ci[2], cn);
+            field_edge<F, J, R>(cn, w[0], w[1], w[2], dn);
+        }
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             if (up) ap[q] += coef * dn[q];
@@ -453,14 +547,17 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     }
     R cc[3][8];
     f.template load<R>(c[0], c[1], c[2], cc);
+    constexpr bool kPre = F::kCoef && !kExact;
+    typename std::conditional<kPre, NbCoef, NbNone>::type nb;
+    if constexpr (kPre) nb_fetch<F>(f, p, c, nb);
     R B[3], d[3][3];
     field_eval<F, R>(cc, w[0], w[1], w[2], B, d);
     const R inb = rsq(dot3(B, B));
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
-    scott_axis<0, R, F>(f, p, c, w, simple[0], B, b, inb, d[0], J[0]);
-    scott_axis<1, R, F>(f, p, c, w, simple[1], B, b, inb, d[1], J[1]);
-    scott_axis<2, R, F>(f, p, c, w, simple[2], B, b, inb, d[2], J[2]);
+    scott_axis<0, R, F>(f, p, c, w, simple[0], B, b, inb, d[0], nb, J[0]);
+    scott_axis<1, R, F>(f, p, c, w, simple[1], B, b, inb, d[1], nb, J[1]);
+    scott_axis<2, R, F>(f, p, c, w, simple[2], B, b, inb, d[2], nb, J[2]);
 }
 
 // The mixed variant's right-hand side at a double stage point: b in FP64
@@ -490,6 +587,7 @@ FQSL_HD void scott_eval_mixed(const F& f, const Pt<double>& pd, double bd[3], fl
     }
     float cc[3][8];
     f.template load<float>(c[0], c[1], c[2], cc);
+    const NbNone nb{};  // fetching the neighbour with the cell costs this variant registers it lacks
     double Bd[3];
     float B[3], d[3][3];
     if constexpr (F::kCoef) {
@@ -526,9 +624,9 @@ FQSL_HD void scott_eval_mixed(const F& f, const Pt<double>& pd, double bd[3], fl
         B[q] = (float)Bd[q];
     }
     const float inb = (float)inbd;
-    scott_axis<0, float, F>(f, p, c, w, simple[0], B, b, inb, d[0], J[0]);
-    scott_axis<1, float, F>(f, p, c, w, simple[1], B, b, inb, d[1], J[1]);
-    scott_axis<2, float, F>(f, p, c, w, simple[2], B, b, inb, d[2], J[2]);
+    scott_axis<0, float, F>(f, p, c, w, simple[0], B, b, inb, d[0], nb, J[0]);
+    scott_axis<1, float, F>(f, p, c, w, simple[1], B, b, inb, d[1], nb, J[1]);
+    scott_axis<2, float, F>(f, p, c, w, simple[2], B, b, inb, d[2], nb, J[2]);
 }
 
 // Unit vector b only (plain tracer RHS: Interp3dxyzn(..., true)).

@@ -1,7 +1,8 @@
 """Q maps on a plane of seeds: the two pipelines the reference's notebooks run.
 
 * ``scott_qmap``     -- Scott method, one trace per pixel, q_0 / q_perp
-  (demo/Q_RKF45CuPyScott.ipynb cells [5]-[7]).
+  (demo/Q_RKF45CuPyScott.ipynb cells [5]-[7]); ``scott_qmap_plane`` the same
+  through the library's C plane entry, seeds formed on the device.
 * ``neighbour_qmap`` -- position-only traces, QCalcPlane, pseudo lines for
   pixels where B is nearly tangent to the plane, Qcut and merge
   (demo/Q_RKF45CuPy.ipynb cells [6]-[11]).
@@ -53,6 +54,40 @@ def scott_qmap(field: Field, seeds, shape, method="rkf45", step=0.25, tol=1e-6,
     if perm is not None:
         r = _unpermute(r, perm)
     return {k: v.reshape(shape) for k, v in r.items() if k != "_seeds"}
+
+
+def scott_qmap_plane(field: Field, plane, n_u, n_v, method="rkf45", step=0.25, tol=1e-6,
+                     precision="fp32c", schedule="persistent", outputs=None, stream=None):
+    """Scott Q map of a plane through the library's own plane entry
+    (``fqsl_qmap_plane``): the seeds are formed on the device as
+    ``synth.plane_seeds_z`` (``plane = ("z", x0, x1, y0, y1, z0)``) or
+    ``synth.plane_seeds_abc`` (``plane = ("abc", A, B, C)``) form them, bit for
+    bit, and traced in row order.  Returns a dict of (n_v, n_u) tensors."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .trace import SCOTT_OUTPUTS, _dtype_of, _stream
+    kind = plane[0]
+    pl = _lib.fqsl_plane()
+    pl.n_u, pl.n_v = int(n_u), int(n_v)
+    if kind == "z":
+        pl.kind = _lib.PLANE_Z
+        vals = [float(v) for v in plane[1:6]]
+    elif kind == "abc":
+        pl.kind = _lib.PLANE_ABC
+        vals = [float(c) for pt in plane[1:4] for c in pt]
+    else:
+        raise ValueError(f"plane kind {kind!r}: 'z' or 'abc'")
+    for k, v in enumerate(vals):
+        pl.p[k] = v
+    n = pl.n_u * pl.n_v
+    res = {k: torch.empty(n, dtype=_dtype_of(k), device="cuda") for k in (outputs or SCOTT_OUTPUTS)}
+    o = _lib.fqsl_scott_out(**{k: (res[k].data_ptr() if k in res else None) for k in SCOTT_OUTPUTS})
+    prm = _lib.fqsl_scott_params(method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision],
+                                 schedule=_lib.SCHEDULES[schedule], reserved=0, step=float(step), tol=float(tol))
+    _lib.check(_lib.lib().fqsl_qmap_plane(field.handle, ctypes.byref(pl), ctypes.byref(prm), ctypes.byref(o),
+                                          _stream(stream)))
+    return {k: v.reshape(pl.n_v, pl.n_u) for k, v in res.items()}
 
 
 CUBE_OUTPUTS = ("q_0", "q_perp", "flag_start", "flag_end", "evals")

@@ -210,3 +210,65 @@ class QServer:
                           stream=stream)
         r.result = res if self.keep else None
         return None
+
+
+class NativeServer:
+    """The library's own request server (``fqsl_server_*``, ``fqsl_submit``,
+    ``fqsl_poll``, ``fqsl_wait``): a consumer thread in the library traces
+    the requests that have arrived as one batch whenever the device is free.
+    ``submit`` returns an id; the request's seeds and outputs are kept alive
+    here until ``wait`` returns them."""
+
+    def __init__(self, precision="fp32c", device=None, max_lines=0):
+        import ctypes
+        import torch
+        from . import _lib
+        self._lib = _lib
+        dev = torch.cuda.current_device() if device is None else int(device)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().fqsl_server_create(dev, _lib.PRECISIONS[precision], int(max_lines),
+                                                 ctypes.byref(h)))
+        self._h = h
+        self._live = {}
+
+    def submit(self, field, x, y, z, method="rkf45", step=0.25, tol=1e-6, outputs=OUTPUTS, stream=None):
+        import ctypes
+        import torch
+        from .trace import SCOTT_OUTPUTS, _dtype_of, _seed_tensors, _stream
+        _lib = self._lib
+        xs, ys, zs = _seed_tensors(x, y, z)
+        n = xs.numel()
+        res = {k: torch.empty(n, dtype=_dtype_of(k), device=xs.device) for k in outputs}
+        o = _lib.fqsl_scott_out(**{k: (res[k].data_ptr() if k in res else None) for k in SCOTT_OUTPUTS})
+        rq = _lib.fqsl_scott_request(field=field.handle, method=_lib.METHODS[method], reserved=0,
+                                     step=float(step), tol=float(tol), n=n, x=xs.data_ptr(),
+                                     y=ys.data_ptr(), z=zs.data_ptr(), out=o)
+        rid = ctypes.c_uint64()
+        _lib.check(_lib.lib().fqsl_submit(self._h, ctypes.byref(rq), _stream(stream), ctypes.byref(rid)))
+        self._live[rid.value] = (res, (xs, ys, zs), field)
+        return rid.value
+
+    def poll(self, rid) -> bool:
+        import ctypes
+        done = ctypes.c_int()
+        self._lib.check(self._lib.lib().fqsl_poll(self._h, int(rid), ctypes.byref(done)))
+        return bool(done.value)
+
+    def wait(self, rid) -> dict:
+        rc = self._lib.lib().fqsl_wait(self._h, int(rid))
+        res = self._live.pop(int(rid), (None,))[0]
+        self._lib.check(rc)
+        return res
+
+    def close(self):
+        if self._h:
+            self._lib.check(self._lib.lib().fqsl_server_destroy(self._h))
+            self._h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

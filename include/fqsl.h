@@ -215,6 +215,65 @@ int fqsl_scott_batch_times(const fqsl_scott_batch *batch, double *ms, void *stre
 
 int fqsl_scott_batch_destroy(fqsl_scott_batch *batch);
 
+/* -------------------------------------------------------------- plane maps */
+#define FQSL_PLANE_Z 0   /* x = linspace(x0, x1, n_u), y = linspace(y0, y1, n_v), z = z0: the
+                            notebooks' constant-z grid (demo/Q_RKF45CuPy.ipynb cell [6]) */
+#define FQSL_PLANE_ABC 1 /* A + u (B - A) + v (C - A), u = linspace(0, 1, n_u),
+                            v = linspace(0, 1, n_v): a plane through three points
+                            (demo/Q_RKF45CuPyPlane.ipynb cell [4]) */
+typedef struct {
+    int kind;      /* FQSL_PLANE_* */
+    int n_u, n_v;  /* pixels per row and rows; maps are n_v x n_u, row-major */
+    int reserved;
+    double p[9];   /* FQSL_PLANE_Z: x0, x1, y0, y1, z0 (p[5..8] unused); FQSL_PLANE_ABC: A, B, C */
+} fqsl_plane;
+
+/* Scott Q map of a plane of seeds, formed on the device: linspace as NumPy
+ * evaluates it (i * ((b - a) / (n - 1)) + a in double, the last point exactly
+ * b) and, for FQSL_PLANE_ABC, A + (B - A) u + (C - A) v in double without
+ * fused multiply-adds, each coordinate then rounded to float -- the seeds of
+ * fastqsl_amd.synth.plane_seeds_z / plane_seeds_abc bit for bit -- traced as
+ * fqsl_trace_scott traces them.  Every output array holds n_v * n_u values,
+ * row-major (rows follow y, or C - A).  Replaces the notebook host code
+ * around one TraceAllBlineScott launch (demo/Q_RKF45CuPyScott.ipynb cells
+ * [5]-[7]); the seeds live in stream-ordered scratch freed on `stream`. */
+int fqsl_qmap_plane(const fqsl_field *field, const fqsl_plane *plane, const fqsl_scott_params *params,
+                    const fqsl_scott_out *out, void *stream);
+
+/* ----------------------------------------------------------------- serving */
+/* A request queue with one consumer thread per server that traces what has
+ * arrived as batches (continuous batching): whenever the device is free, the
+ * consumer takes the waiting requests in submission order, up to max_lines
+ * lines (at least one request), traces them as one fqsl_scott_batch on its
+ * own stream and marks them done.  The reference has no serving layer; its
+ * notebooks launch one kernel per map.  Submission is thread-safe.  The
+ * caller keeps a request's field, seeds and outputs alive until fqsl_poll
+ * reports it done or fqsl_wait returns; each request's outputs are the bits
+ * fqsl_trace_scott gives for it alone. */
+typedef struct fqsl_server fqsl_server;
+
+/* A server on `device` tracing with `precision` (FQSL_PREC_*); max_lines
+ * bounds one batch (0: 2^24 lines). */
+int fqsl_server_create(int device, int precision, uint64_t max_lines, fqsl_server **server);
+
+/* Queue one request (copied); *id identifies it (ids start at 1).  The
+ * request is ordered after the work enqueued on `stream` (a stream of the
+ * server's device, NULL the null stream) up to this call, so its seeds may
+ * still be in flight there. */
+int fqsl_submit(fqsl_server *server, const fqsl_scott_request *req, void *stream, uint64_t *id);
+
+/* *done = 1 once request `id` has been traced (its status is then kept for
+ * fqsl_wait), 0 while it is queued or running.  Unknown ids fail. */
+int fqsl_poll(fqsl_server *server, uint64_t id, int *done);
+
+/* Blocks until request `id` is done, returns its status (FQSL_OK or the
+ * batch's error, with fqsl_last_error set on this thread) and forgets it. */
+int fqsl_wait(fqsl_server *server, uint64_t id);
+
+/* Traces what is still queued, stops the consumer and frees the server;
+ * requests not yet waited for are forgotten. */
+int fqsl_server_destroy(fqsl_server *server);
+
 /* ------------------------------------------------------------ plain tracer */
 typedef struct {
     int method;      /* FQSL_METHOD_RKF45 (TraceBlineAdaptive.cu) or FQSL_METHOD_RK4 (TraceBline.cu) */
@@ -405,7 +464,9 @@ int fqsl_TraceAllBlineStretch(float *Bx, float *By, float *Bz, int *BshapeN, flo
 #define FQSL_TUNE_PLAIN_REFILL 6  /* idle lanes a wavefront of the persistent plain tracer gathers before it
                                      refills them (1..64, default 64: a wavefront claims new lines
                                      once all its lanes are idle) */
-#define FQSL_TUNE_COUNT 7
+#define FQSL_TUNE_STRETCH_LDS 7   /* FP32 stretched tracer: 1 (default) = each workgroup copies the axis
+                                     lookup table to LDS when it fits 64 KiB; 0 = global-memory tables */
+#define FQSL_TUNE_COUNT 8
 int fqsl_tune_set(int knob, int value);
 int fqsl_tune_get(int knob, int *value);
 

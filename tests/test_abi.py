@@ -81,12 +81,13 @@ def test_constants_match_the_header():
     assert (mac["FQSL_LAYOUT_SOA32"], mac["FQSL_LAYOUT_SOA64"], mac["FQSL_LAYOUT_CELL32"],
             mac["FQSL_LAYOUT_ALL"]) == (_lib.LAYOUT_SOA32, _lib.LAYOUT_SOA64, _lib.LAYOUT_CELL32,
                                         _lib.LAYOUT_ALL)
+    assert (mac["FQSL_PLANE_Z"], mac["FQSL_PLANE_ABC"]) == (_lib.PLANE_Z, _lib.PLANE_ABC)
     knobs = {k: mac["FQSL_TUNE_" + k.upper()] for k in _lib.TUNE_KNOBS}
     assert knobs == _lib.TUNE_KNOBS and mac["FQSL_TUNE_COUNT"] == len(knobs)
 
 
 STRUCTS = ["fqsl_scott_params", "fqsl_scott_out", "fqsl_plain_params", "fqsl_plain_out",
-           "fqsl_scott_request"]
+           "fqsl_scott_request", "fqsl_plane"]
 
 
 def test_struct_layouts_match_a_c_compiler(tmp_path):
@@ -122,6 +123,8 @@ def test_host_only_calls(L):
     assert _lib.tune_get("queues") == 1
     assert _lib.tune_get("refill_min") == 8
     assert _lib.tune_get("order") == 1 and _lib.tune_get("order_buckets") == 64
+    assert _lib.tune_get("stretch_lds") == 1
+    assert L.fqsl_tune_set(_lib.TUNE_KNOBS["stretch_lds"], 2) != _lib.FQSL_OK
     assert L.fqsl_tune_set(99, 1) != _lib.FQSL_OK
     assert L.fqsl_last_error().decode()
     v = ctypes.c_int(-1)

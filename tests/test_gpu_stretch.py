@@ -10,7 +10,8 @@
   the oracle's own 10 %-tolerance spread with the reference rules.
 * B_this / B_start / B_end and B_flag against the oracle's stretched
   Interp3dxyzn.
-* Persistent and per-line schedules bit for bit; the 39-argument drop-in
+* Persistent and per-line schedules bit for bit; the LDS copy of the axis
+  lookup against the global tables bit for bit; the 39-argument drop-in
   (flag_uni false / true, curB_flag) bit for bit against the handle API.
 * Twist: a unit-magnitude force-free field gives a L / 4 pi.
 """
@@ -184,6 +185,33 @@ def test_schedules_bitwise_identical(fq, precision):
         b = _run(fq, field, seeds, precision, "per_line", step=0.01, tol_coef=1e-2)
     for k in a:
         assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("rule", ["bisect", "uniform", "exact"])
+def test_lds_tables_bitwise_identical(fq, rule):
+    """The FP32 kernels read the axis lookup from a per-workgroup LDS copy of
+    a compact table (knob stretch_lds = 1, the default); every output,
+    twist included, equals the global-table lookup bit for bit on both
+    schedules and both float layouts."""
+    from fastqsl_amd import _lib
+    from fastqsl_amd.trace import PLAIN_OUTPUTS
+    f, axes, seeds = _case(rule, 3000)
+    cur = tuple(0.3 * b for b in f[::-1])
+    outs = tuple(PLAIN_OUTPUTS) + ("twist",)
+    before = _lib.tune_get("stretch_lds")
+    try:
+        for layouts in (("cell32", "soa64"), ("soa32", "soa64")):
+            with fq.Field(*f, axes=axes, lookup=rule, layouts=layouts, current=cur) as field:
+                for schedule in ("persistent", "per_line"):
+                    res = []
+                    for lds in (1, 0):
+                        _lib.tune_set("stretch_lds", lds)
+                        res.append(_run(fq, field, seeds, "fp32c", schedule, outputs=outs, step=0.01,
+                                        tol_coef=1e-2))
+                    for k in outs:
+                        assert np.array_equal(res[0][k], res[1][k], equal_nan=True), (layouts, schedule, k)
+    finally:
+        _lib.tune_set("stretch_lds", before)
 
 
 @pytest.mark.parametrize("uni", [False, True])
