@@ -77,14 +77,32 @@ def parse():
 
 PROFILES = {"plane": "profiles/scott_rkf45_multipole256_1024sq.md",
             "volume": "profiles/scott_rkf45_rope512x256x256_volume.md",
-            "stretch": "profiles/plain_rkf45_stretched_multipole256_1024sq.md"}
+            "stretch": "profiles/plain_rkf45_stretched_multipole256_1024sq.md",
+            "neighbour": "profiles/neighbour_qmap_plain_rkf45_multipole256_1024sq.md"}
 TRAFFIC_HOW = "rocprofv3 FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated"
+
+
+def hbm_roofline(workload, kernel, kernel_ms, evals, bpe, traffic_applies=True):
+    """HBM roofline of a tracing kernel: achieved = measured HBM bytes per
+    launch (the workload's rocprofv3 profile) over the live kernel time;
+    beside it the no-reuse stencil model (bytes per stage evaluation x
+    evaluations), which is a model of the cache-less traffic, not a bound."""
+    t = kernel_ms * 1e-3
+    traffic = measured_traffic(workload) if traffic_applies else None
+    return {"bound": "hbm", "kernel": kernel, "kernel_ms": kernel_ms, "unit": "GB/s",
+            "achieved": traffic / t / 1e9 if traffic else None, "peak": HBM_PEAK / 1e9,
+            "frac": traffic / t / HBM_PEAK if traffic else None, "traffic": traffic,
+            "traffic_source": "%s (%s)" % (PROFILES.get(workload), TRAFFIC_HOW) if traffic else None,
+            "stencil_model": {"bytes_per_eval": bpe, "evals_per_launch": evals,
+                              "achieved_GBs": evals * bpe / t / 1e9, "frac": evals * bpe / t / HBM_PEAK}}
 
 
 def measured_traffic(workload):
     """HBM bytes per launch of the workload's dominant kernel as measured
     with rocprofv3 (FETCH_SIZE + WRITE_SIZE, copy-calibrated) and recorded in
     the profile summary for the workload; None if absent."""
+    if workload not in PROFILES:
+        return None
     try:
         with open(os.path.join(ROOT, PROFILES[workload])) as fh:
             for line in fh:
@@ -306,11 +324,9 @@ def bench_batch(args, fq, torch, log):
                                "multipole 320x320x256 (seed 7); one graph-captured batch",
                    "lines_per_step": nlines, "precision": args.precision,
                    "schedule": "batch", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": evals * bpe / (ms_step * 1e-3) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": evals * bpe / (ms_step * 1e-3) / HBM_PEAK, "traffic": None,
-                     "evals_per_launch": evals, "bytes_per_eval": bpe,
-                     "note": "time per replay (host-synchronised), all groups"},
+        "roofline": dict(hbm_roofline("batch", "persistent batch groups (one replay)", ms_step, evals, bpe),
+                         note="HBM traffic of a replay not profiled; time per replay (host-synchronised), "
+                              "all groups"),
         "cpu_baseline": cpu,
         "tuning": tuning,
         **extra,
@@ -454,7 +470,6 @@ def bench_neighbour(args, fq, torch, log):
     evals = int(tr["evals"].to(torch.int64).sum().item())
     k = float(np.mean(kms))
     bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
-    achieved = evals * bpe / (k * 1e-3)
     log(f"neighbour {args.precision}/{args.schedule}: {value:.4g} pixels/s over 2 streams "
         f"({wall / steps * 1e3:.2f} ms per map; one at a time {wall1 / steps * 1e3:.2f} ms), {npseudo} pseudo "
         f"pixels; main trace {k:.2f} ms, {evals / (n * n):.1f} evals/line; maps identical: {same}")
@@ -512,10 +527,8 @@ def bench_neighbour(args, fq, torch, log):
                                "(tol_coef 0.1) + Qcut; K maps over 2 streams" % (n, n),
                    "pixels_per_step": n * n, "pseudo_pixels": npseudo, "precision": args.precision,
                    "schedule": args.schedule, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_plain_%s (main trace)" % args.schedule, "kernel_ms": k,
-                     "evals_per_launch": evals, "bytes_per_eval": bpe},
+        "roofline": hbm_roofline("neighbour", "k_plain_%s (main trace)" % args.schedule, k, evals, bpe,
+                                 (args.precision, args.schedule, n) == ("fp32c", "persistent", 1024)),
         "cpu_baseline": cpu,
         **extra,
     }
@@ -578,7 +591,6 @@ def bench_twist(args, fq, torch, log):
     evals = int(out["evals"].to(torch.int64).sum().item())
     value = nlines * steps / wall
     bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
-    achieved = evals * bpe / (k * 1e-3)
     log(f"twist {args.precision}/{args.schedule}: {value:.4g} lines/s ({wall / steps * 1e3:.2f} ms per map; "
         f"without twist {wall0 / steps * 1e3:.2f} ms), stepping kernel {k:.2f} ms (without twist {k0:.2f})")
     extra = {"without_twist": {"lines_per_s": nlines * steps / wall0, "ms_per_map": wall0 / steps * 1e3,
@@ -618,13 +630,10 @@ def bench_twist(args, fq, torch, log):
                                "twist, RKF45 tol 1e-6, z=0 plane %dx%d" % (n, n),
                    "lines_per_step": nlines, "precision": args.precision, "schedule": args.schedule,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": ("stepping kernel k_scott_persistent_w2 (twist)" if persistent
-                                else "k_scott_per_line (twist)"),
-                     "kernel_ms": k, "evals_per_launch": evals, "bytes_per_eval": bpe,
-                     "note": "stage evaluations only; the two J and B interpolations per accepted step "
-                             "for the twist are not counted"},
+        "roofline": dict(hbm_roofline("twist", ("stepping kernel k_scott_persistent_w2 (twist)" if persistent
+                                                 else "k_scott_per_line (twist)"), k, evals, bpe),
+                         note="HBM traffic not profiled; stage evaluations only (the two J and B "
+                              "interpolations per accepted step for the twist are not counted)"),
         "cpu_baseline": cpu,
         **extra,
     }
@@ -696,7 +705,6 @@ def bench_stretch(args, fq, torch, log):
     value = nlines * steps / wall
     k = float(np.mean(kms))
     bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
-    achieved = evals * bpe / (k * 1e-3)
     log(f"stretch {args.precision}/{args.schedule}: {value:.4g} lines/s, kernel {k:.2f} ms, "
         f"{evals / nlines:.1f} evals/line")
     extra = {}
@@ -747,13 +755,8 @@ def bench_stretch(args, fq, torch, log):
                                "%dx%d" % (n, n),
                    "lines_per_step": nlines, "precision": args.precision, "schedule": args.schedule,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK,
-                     "traffic": (measured_traffic("stretch") if (args.precision, args.schedule, n) ==
-                                 ("fp32c", "per_line", 1024) else None),
-                     "traffic_source": "%s (%s)" % (PROFILES["stretch"], TRAFFIC_HOW),
-                     "kernel": "k_plain_%s (stretched)" % args.schedule, "kernel_ms": k,
-                     "evals_per_launch": evals, "bytes_per_eval": bpe},
+        "roofline": hbm_roofline("stretch", "k_plain_%s (stretched)" % args.schedule, k, evals, bpe,
+                                 (args.precision, args.schedule, n) == ("fp32c", "per_line", 1024)),
         "cpu_baseline": cpu,
         "classes": {str(int(c)): int(v) for c, v in zip(classes, counts)},
         **extra,
