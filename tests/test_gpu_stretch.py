@@ -260,6 +260,68 @@ def test_dropin_matches_handle_api(fq, uni):
     assert np.any(twist != 0)
 
 
+def test_dropin_tables_shared_by_two_threads(fq):
+    """The stretched drop-in keeps one set of lookup tables per device and
+    rebuilds it when the axes change.  Two host threads, each on its own
+    stream with its own axes, alternate on it: every call replaces the other
+    thread's tables, whose launch must have been enqueued (and is waited for)
+    before they are freed.  Every result equals the same call made alone."""
+    import threading
+    import torch
+    cases = []
+    for strength, seed in ((1.5, 2), (0.7, 4)):
+        field, axes = synth.stretched_multipole((48, 44, 40), seed=seed, device="cpu",
+                                                strength=(strength, strength, 2.0))
+        axes = [np.asarray(a, np.float32) for a in axes]
+        bx, by, bz = (np.asfortranarray(b.astype(np.float32)) for b in field)
+        rng = np.random.default_rng(seed)
+        n = 1500
+        seeds = np.column_stack([rng.uniform(-0.9, 0.9, n), rng.uniform(-0.9, 0.9, n),
+                                 np.full(n, 1e-3)]).astype(np.float32)
+        cases.append((bx, by, bz, axes, seeds))
+
+    def call(kern, case):
+        bx, by, bz, axes, seeds = case
+        n = seeds.shape[0]
+        f32 = lambda: np.zeros(n, np.float32)
+        o = [f32(), f32(), f32(), np.zeros(n, np.int32), f32(), f32(), f32(), np.zeros(n, np.int32),
+             f32(), f32(), f32(), np.zeros(n, np.int32), f32(), f32(), f32(), f32(), f32(), f32()]
+        line_len = np.zeros(n, np.float64)
+        dummy = np.zeros(1, np.float32)
+        args = [bx, by, bz, np.array(bx.shape, np.int32), *axes, np.array([False]), dummy, dummy, dummy,
+                np.zeros(1, np.float64), np.array([False]), *(np.ascontiguousarray(seeds[:, q]) for q in range(3)),
+                np.array([0, 0, 1], np.float32), *o, np.array([0.01], np.float32), np.array([n], np.uint64),
+                line_len, np.array([0.5], np.float32)]
+        kern((28, 1), (128, 1, 1), tuple(args))
+        return o + [line_len]
+
+    alone = [call(fq.kernels.compileTraceBlineAdaptiveStretch(), c) for c in cases]
+    got = [[], []]
+    errors = []
+
+    def worker(t):
+        try:
+            kern = fq.kernels.compileTraceBlineAdaptiveStretch()
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                for _ in range(4):
+                    got[t].append(call(kern, cases[t]))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors
+    for t in range(2):
+        assert len(got[t]) == 4
+        for res in got[t]:
+            for a, b in zip(res, alone[t]):
+                assert np.array_equal(a, b, equal_nan=True)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32c"])
 def test_twist_in_a_unit_force_free_field(fq, precision):
     shape = (20, 20, 24)
