@@ -126,6 +126,19 @@ def test_host_only_calls(L):
     assert _lib.tune_get("stretch_lds") == 1
     assert L.fqsl_tune_set(_lib.TUNE_KNOBS["stretch_lds"], 2) != _lib.FQSL_OK
     assert L.fqsl_tune_set(99, 1) != _lib.FQSL_OK
+    # server and plane-map argument checks that return before any device call
+    h = ctypes.c_void_p()
+    assert L.fqsl_server_create(0, 99, 0, ctypes.byref(h)) != _lib.FQSL_OK and not h.value
+    done = ctypes.c_int(-1)
+    assert L.fqsl_poll(None, 1, ctypes.byref(done)) != _lib.FQSL_OK
+    assert L.fqsl_wait(None, 1) != _lib.FQSL_OK
+    rid = ctypes.c_uint64()
+    assert L.fqsl_submit(None, None, None, ctypes.byref(rid)) != _lib.FQSL_OK
+    assert L.fqsl_server_destroy(None) == _lib.FQSL_OK
+    pl = _lib.fqsl_plane(kind=7, n_u=4, n_v=4)
+    prm = _lib.fqsl_scott_params()
+    out = _lib.fqsl_scott_out()
+    assert L.fqsl_qmap_plane(None, ctypes.byref(pl), ctypes.byref(prm), ctypes.byref(out), None) != _lib.FQSL_OK
     assert L.fqsl_last_error().decode()
     v = ctypes.c_int(-1)
     assert L.fqsl_tune_get(-1, ctypes.byref(v)) != _lib.FQSL_OK
