@@ -48,6 +48,13 @@ class fqsl_plane(ctypes.Structure):
     _fields_ = [("kind", i32), ("n_u", i32), ("n_v", i32), ("reserved", i32), ("p", ctypes.c_double * 9)]
 
 
+class fqsl_neighbour_params(ctypes.Structure):
+    _fields_ = [("method", i32), ("precision", i32), ("schedule", i32), ("reserved", i32),
+                ("step", ctypes.c_double), ("tol_coef", ctypes.c_double), ("pseudo_tol_scale", ctypes.c_double),
+                ("stride", ctypes.c_double), ("pseudo_stride", ctypes.c_double),
+                ("normal", ctypes.c_float * 3), ("pseudo_normal", ctypes.c_float * 3)]
+
+
 class fqsl_plain_params(ctypes.Structure):
     _fields_ = [("method", i32), ("precision", i32), ("schedule", i32), ("reserved", i32),
                 ("step", ctypes.c_double), ("tol_coef", ctypes.c_double),
@@ -112,6 +119,8 @@ PROTOTYPES = {
     "fqsl_scott_batch_destroy": (i32, [vp]),
     "fqsl_qmap_plane": (i32, [vp, ctypes.POINTER(fqsl_plane), ctypes.POINTER(fqsl_scott_params),
                               ctypes.POINTER(fqsl_scott_out), vp]),
+    "fqsl_qmap_plane_neighbour": (i32, [vp, ctypes.POINTER(fqsl_plane), ctypes.POINTER(fqsl_neighbour_params),
+                                        vp, vp, vp]),
     "fqsl_server_create": (i32, [i32, i32, u64, ctypes.POINTER(vp)]),
     "fqsl_submit": (i32, [vp, ctypes.POINTER(fqsl_scott_request), vp, ctypes.POINTER(u64)]),
     "fqsl_poll": (i32, [vp, u64, ctypes.POINTER(i32)]),

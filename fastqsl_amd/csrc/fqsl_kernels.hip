@@ -2152,6 +2152,98 @@ extern "C" int fqsl_qmap_plane(const fqsl_field* field, const fqsl_plane* plane,
     return FQSL_OK;
 }
 
+// The neighbour-difference map of a plane, every step on the stream.
+extern "C" int fqsl_qmap_plane_neighbour(const fqsl_field* field, const fqsl_plane* plane,
+                                         const fqsl_neighbour_params* p, float* q_all, uint64_t* n_pseudo_dev,
+                                         void* stream) {
+    if (!field || !plane || !p || !q_all) return set_err(FQSL_ERR_INVALID, "NULL field/plane/params/q_all");
+    if (plane->kind != FQSL_PLANE_Z && plane->kind != FQSL_PLANE_ABC)
+        return set_err(FQSL_ERR_INVALID, "plane kind %d", plane->kind);
+    if (plane->n_u < 3 || plane->n_v < 3)
+        return set_err(FQSL_ERR_INVALID, "a neighbour-difference map needs at least 3 x 3 pixels (got %d x %d)",
+                       plane->n_u, plane->n_v);
+    for (int c = 0; c < 9; ++c)
+        if (!(plane->p[c] - plane->p[c] == 0.0)) return set_err(FQSL_ERR_INVALID, "plane coordinate %d is not finite", c);
+    const int nu = plane->n_u, nv = plane->n_v;
+    const uint64_t n = (uint64_t)nu * (uint64_t)nv;
+    const size_t pwb = fqsl_pseudo_workspace_bytes(n);
+    // scratch: seeds, main traces, q_inner, pseudo seeds / Bz / pixels / counts, pseudo traces
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off = align256(off + bytes); return o; };
+    const size_t o_seed = take(3 * sizeof(float) * n);
+    const size_t o_main = take(18 * sizeof(float) * n);
+    const size_t o_qin = take(sizeof(float) * (size_t)(nu - 2) * (nv - 2));
+    const size_t o_pseed = take(12 * sizeof(float) * n);
+    const size_t o_bz0 = take(2 * sizeof(float) * n);
+    const size_t o_pix = take(sizeof(uint64_t) * n);
+    const size_t o_cnt = take(2 * sizeof(uint64_t));
+    const size_t o_pws = take(pwb);
+    const size_t o_pout = take(32 * sizeof(float) * n);
+    hipStream_t st = (hipStream_t)stream;
+    char* ws = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&ws, off, st));
+    float* sd = (float*)(ws + o_seed);
+    float* m = (float*)(ws + o_main);  // 18 arrays of n: start xyz, end xyz, flags, b_this xyz, b_flag, b_start xyz, b_end xyz
+    float *sx = m, *sy = m + n, *sz = m + 2 * n, *ex = m + 3 * n, *ey = m + 4 * n, *ez = m + 5 * n;
+    int32_t *fs = (int32_t*)(m + 6 * n), *fe = (int32_t*)(m + 7 * n);
+    float *btx = m + 8 * n, *bty = m + 9 * n, *btz = m + 10 * n;
+    int32_t* bflag = (int32_t*)(m + 11 * n);
+    float *bsx = m + 12 * n, *bsy = m + 13 * n, *bsz = m + 14 * n, *bex = m + 15 * n, *bey = m + 16 * n,
+          *bez = m + 17 * n;
+    float* qin = (float*)(ws + o_qin);
+    float* ps = (float*)(ws + o_pseed);
+    float* bz0 = (float*)(ws + o_bz0);
+    uint64_t* pix = (uint64_t*)(ws + o_pix);
+    uint64_t* cnt = (uint64_t*)(ws + o_cnt);
+    float* po = (float*)(ws + o_pout);  // 4n-line traces: start xyz, end xyz, flags
+    const uint64_t n4 = 4 * n;
+    int rc = FQSL_OK;
+    k_plane_seeds<<<grid_for((long)n, 256), 256, 0, st>>>(*plane, sd, sd + n, sd + 2 * n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_err(FQSL_ERR_HIP, "plane seeds: %s", hipGetErrorString(e));
+    if (!rc) {
+        fqsl_plain_params mp{p->method, p->precision, p->schedule, 0, p->step, p->tol_coef,
+                             {p->normal[0], p->normal[1], p->normal[2]}, 0.f};
+        fqsl_plain_out mo{sx, sy, sz, ex, ey, ez, fs, fe, btx, bty, btz, bflag, bsx, bsy, bsz, bex, bey, bez,
+                          nullptr, nullptr, nullptr};
+        rc = fqsl_trace_plain(field, &mp, n, sd, sd + n, sd + 2 * n, &mo, stream);
+    }
+    if (!rc)
+        rc = fqsl_qcalc_plane(nv, nu, ex, ey, ez, fe, sx, sy, sz, fs, bsz, bez, btx, bty, btz, p->stride, p->normal,
+                              qin, stream);
+    if (!rc) {
+        e = hipMemsetAsync(q_all, 0, sizeof(float) * n, st);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(q_all + nu + 1, sizeof(float) * nu, qin, sizeof(float) * (nu - 2),
+                                 sizeof(float) * (nu - 2), nv - 2, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) rc = set_err(FQSL_ERR_HIP, "q_all interior: %s", hipGetErrorString(e));
+    }
+    const float pstride = (float)p->pseudo_stride;
+    if (!rc)
+        rc = fqsl_pseudo_seeds_async(n, bflag, sx, sy, sz, fs, bsz, bez, pstride, n, ps, ps + n4, ps + 2 * n4, bz0,
+                                     bz0 + n, pix, cnt, ws + o_pws, pwb, stream);
+    float *qsx = po, *qsy = po + n4, *qsz = po + 2 * n4, *qex = po + 3 * n4, *qey = po + 4 * n4, *qez = po + 5 * n4;
+    int32_t *qfs = (int32_t*)(po + 6 * n4), *qfe = (int32_t*)(po + 7 * n4);
+    if (!rc) {
+        fqsl_plain_params pp{p->method, p->precision, p->schedule, 0, p->step, p->tol_coef * p->pseudo_tol_scale,
+                             {p->pseudo_normal[0], p->pseudo_normal[1], p->pseudo_normal[2]}, 0.f};
+        fqsl_plain_out pout{qsx, qsy, qsz, qex, qey, qez, qfs, qfe, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        rc = fqsl_trace_plain_n(field, &pp, n4, cnt + 1, ps, ps + n4, ps + 2 * n4, &pout, stream);
+    }
+    if (!rc)
+        rc = fqsl_qcut_scatter_n(n, cnt, qsx, qsy, qsz, qfs, qex, qey, qez, qfe, bz0, bz0 + n, pstride, pix, q_all,
+                                 stream);
+    if (!rc && n_pseudo_dev) {
+        e = hipMemcpyAsync(n_pseudo_dev, cnt, sizeof(uint64_t), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) rc = set_err(FQSL_ERR_HIP, "pseudo count: %s", hipGetErrorString(e));
+    }
+    e = hipFreeAsync(ws, st);
+    if (rc) return rc;
+    if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "hipFreeAsync: %s", hipGetErrorString(e));
+    return FQSL_OK;
+}
+
 // ---------------------------------------------------------------- serving
 // One consumer thread per server drains the queue in batches: it takes the
 // waiting requests in submission order up to max_lines lines, traces them as

@@ -2,7 +2,8 @@
 
 * ``scott_qmap``     -- Scott method, one trace per pixel, q_0 / q_perp
   (demo/Q_RKF45CuPyScott.ipynb cells [5]-[7]); ``scott_qmap_plane`` the same
-  through the library's C plane entry, seeds formed on the device.
+  through the library's C plane entry, seeds formed on the device
+  (``neighbour_qmap_plane`` likewise for the neighbour-difference map).
 * ``neighbour_qmap`` -- position-only traces, QCalcPlane, pseudo lines for
   pixels where B is nearly tangent to the plane, Qcut and merge
   (demo/Q_RKF45CuPy.ipynb cells [6]-[11]).
@@ -67,9 +68,22 @@ def scott_qmap_plane(field: Field, plane, n_u, n_v, method="rkf45", step=0.25, t
     import torch
     from . import _lib
     from .trace import SCOTT_OUTPUTS, _dtype_of, _stream
-    kind = plane[0]
+    pl = _plane_struct(plane, n_u, n_v)
+    n = pl.n_u * pl.n_v
+    res = {k: torch.empty(n, dtype=_dtype_of(k), device="cuda") for k in (outputs or SCOTT_OUTPUTS)}
+    o = _lib.fqsl_scott_out(**{k: (res[k].data_ptr() if k in res else None) for k in SCOTT_OUTPUTS})
+    prm = _lib.fqsl_scott_params(method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision],
+                                 schedule=_lib.SCHEDULES[schedule], reserved=0, step=float(step), tol=float(tol))
+    _lib.check(_lib.lib().fqsl_qmap_plane(field.handle, ctypes.byref(pl), ctypes.byref(prm), ctypes.byref(o),
+                                          _stream(stream)))
+    return {k: v.reshape(pl.n_v, pl.n_u) for k, v in res.items()}
+
+
+def _plane_struct(plane, n_u, n_v):
+    from . import _lib
     pl = _lib.fqsl_plane()
     pl.n_u, pl.n_v = int(n_u), int(n_v)
+    kind = plane[0]
     if kind == "z":
         pl.kind = _lib.PLANE_Z
         vals = [float(v) for v in plane[1:6]]
@@ -80,14 +94,35 @@ def scott_qmap_plane(field: Field, plane, n_u, n_v, method="rkf45", step=0.25, t
         raise ValueError(f"plane kind {kind!r}: 'z' or 'abc'")
     for k, v in enumerate(vals):
         pl.p[k] = v
-    n = pl.n_u * pl.n_v
-    res = {k: torch.empty(n, dtype=_dtype_of(k), device="cuda") for k in (outputs or SCOTT_OUTPUTS)}
-    o = _lib.fqsl_scott_out(**{k: (res[k].data_ptr() if k in res else None) for k in SCOTT_OUTPUTS})
-    prm = _lib.fqsl_scott_params(method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision],
-                                 schedule=_lib.SCHEDULES[schedule], reserved=0, step=float(step), tol=float(tol))
-    _lib.check(_lib.lib().fqsl_qmap_plane(field.handle, ctypes.byref(pl), ctypes.byref(prm), ctypes.byref(o),
-                                          _stream(stream)))
-    return {k: v.reshape(pl.n_v, pl.n_u) for k, v in res.items()}
+    return pl
+
+
+def neighbour_qmap_plane(field: Field, plane, n_u, n_v, stride, normal=(0.0, 0.0, 1.0), method="rkf45",
+                         step=0.125, tol_coef=1.0, pseudo_stride=None, pseudo_tol_scale=0.1, pseudo_normal=None,
+                         precision="fp32c", schedule="persistent", stream=None):
+    """``neighbour_qmap`` through the library's own plane entry
+    (``fqsl_qmap_plane_neighbour``): seeds formed on the device as
+    ``scott_qmap_plane`` forms them, every step of the pipeline on one stream.
+    Returns q (n_v-2, n_u-2), q_all (n_v, n_u) and n_pseudo (a CUDA int64
+    scalar)."""
+    import ctypes
+    import torch
+    from . import _lib
+    from .trace import _stream
+    pl = _plane_struct(plane, n_u, n_v)
+    prm = _lib.fqsl_neighbour_params(
+        method=_lib.METHODS[method], precision=_lib.PRECISIONS[precision], schedule=_lib.SCHEDULES[schedule],
+        reserved=0, step=float(step), tol_coef=float(tol_coef), pseudo_tol_scale=float(pseudo_tol_scale),
+        stride=float(stride), pseudo_stride=float(stride if pseudo_stride is None else pseudo_stride))
+    pn = normal if pseudo_normal is None else pseudo_normal
+    for k in range(3):
+        prm.normal[k] = float(normal[k])
+        prm.pseudo_normal[k] = float(pn[k])
+    q_all = torch.empty((pl.n_v, pl.n_u), dtype=torch.float32, device="cuda")
+    npseudo = torch.zeros((), dtype=torch.int64, device="cuda")
+    _lib.check(_lib.lib().fqsl_qmap_plane_neighbour(field.handle, ctypes.byref(pl), ctypes.byref(prm),
+                                                    q_all.data_ptr(), npseudo.data_ptr(), _stream(stream)))
+    return dict(q=q_all[1:-1, 1:-1], q_all=q_all, n_pseudo=npseudo)
 
 
 CUBE_OUTPUTS = ("q_0", "q_perp", "flag_start", "flag_end", "evals")

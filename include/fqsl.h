@@ -240,6 +240,33 @@ typedef struct {
 int fqsl_qmap_plane(const fqsl_field *field, const fqsl_plane *plane, const fqsl_scott_params *params,
                     const fqsl_scott_out *out, void *stream);
 
+/* Neighbour-difference Q map of a plane of seeds (the pipeline of
+ * demo/Q_RKF45CuPy.ipynb cells [6]-[11]): the plane's seeds as in
+ * fqsl_qmap_plane, position-only traces (fqsl_trace_plain), QCalcPlane
+ * (FastQSL.py:65-117) into the interior of q_all, CookPseudoLine
+ * (FastQSL.py:4-57) for the pixels with B_flag set, their four pseudo lines
+ * traced with tol_coef x pseudo_tol_scale, and Qcut scattered into q_all --
+ * all on `stream`, the pseudo-line count kept on the device.  q_all holds
+ * n_v * n_u floats (row-major, border pixels 0); *n_pseudo_dev (a device
+ * word, may be NULL) receives the number of pseudo-line pixels.  Plain
+ * methods and precisions only (FQSL_PREC_FP32C or FQSL_PREC_FP64); the plane
+ * needs at least 3 x 3 pixels.  Its scratch is stream-ordered and freed on
+ * `stream`. */
+typedef struct {
+    int method, precision, schedule, reserved;  /* as fqsl_plain_params */
+    double step;             /* s_len, cells */
+    double tol_coef;         /* tolerance coefficient of the main lines */
+    double pseudo_tol_scale; /* the pseudo lines use tol_coef * pseudo_tol_scale (notebooks: 0.1) */
+    double stride;           /* pixel spacing in cells (QCalcPlane's stride_step) */
+    double pseudo_stride;    /* offset of the four pseudo seeds (CookPseudoLine's stride) */
+    float normal[3];         /* plane normal: tolerance rule, B_flag and QCalcPlane */
+    float pseudo_normal[3];  /* normal for the pseudo lines' tolerance rule */
+} fqsl_neighbour_params;
+
+int fqsl_qmap_plane_neighbour(const fqsl_field *field, const fqsl_plane *plane,
+                              const fqsl_neighbour_params *params, float *q_all, uint64_t *n_pseudo_dev,
+                              void *stream);
+
 /* ----------------------------------------------------------------- serving */
 /* A request queue with one consumer thread per server that traces what has
  * arrived as batches (continuous batching): whenever the device is free, the

@@ -87,7 +87,7 @@ def test_constants_match_the_header():
 
 
 STRUCTS = ["fqsl_scott_params", "fqsl_scott_out", "fqsl_plain_params", "fqsl_plain_out",
-           "fqsl_scott_request", "fqsl_plane"]
+           "fqsl_scott_request", "fqsl_plane", "fqsl_neighbour_params"]
 
 
 def test_struct_layouts_match_a_c_compiler(tmp_path):

@@ -4,6 +4,8 @@
   through three points on the device; every output equals ``fqsl_trace_scott``
   on the seeds ``synth.plane_seeds_z`` / ``plane_seeds_abc`` build on the host,
   bit for bit.
+* ``fqsl_qmap_plane_neighbour`` (the whole neighbour-difference pipeline on
+  one stream) equals the same steps run from Python on host-built seeds.
 * ``fqsl_submit`` / ``fqsl_poll`` / ``fqsl_wait``: requests over two fields,
   both methods and uneven sizes, submitted from two host threads, are traced
   by the server's consumer in batches; each request's outputs equal tracing it
@@ -117,3 +119,29 @@ def test_native_server_matches_alone(fq, fields):
         a = fq.trace_scott(f, s[:, 0], s[:, 1], s[:, 2], method=m, step=0.25, tol=tol, precision="fp32c")
         torch.cuda.synchronize()
         _same(r, a, ("q_0", "q_perp", "flag_start", "flag_end", "evals"))
+
+
+@pytest.mark.parametrize("kind", ["z", "abc"])
+def test_neighbour_qmap_plane_matches_the_python_pipeline(fq, fields, kind):
+    """fqsl_qmap_plane_neighbour (seeds, plain traces, QCalcPlane,
+    CookPseudoLine, pseudo traces, Qcut, all on one stream) against the same
+    steps run from Python (qmap.neighbour_qmap) on the host-built seeds."""
+    import torch
+    from fastqsl_amd import qmap
+    f = fields[1]
+    nu, nv = 90, 70
+    stride = (69.0 - 2.0) / (nu - 1)
+    if kind == "z":
+        plane = ("z", 2.0, 69.0, 3.0, 60.0, 1e-4)
+        seeds, shape = synth.plane_seeds_z(nu, nv, (2.0, 69.0), (3.0, 60.0), z=1e-4)
+    else:
+        plane = ("abc", (2.0, 3.0, 1e-4), (69.0, 3.0, 1e-4), (2.0, 60.0, 1e-4))
+        seeds, shape, _ = synth.plane_seeds_abc(*plane[1:], nu, nv)
+    got = qmap.neighbour_qmap_plane(f, plane, nu, nv, stride, step=0.125, tol_coef=0.5)
+    want = qmap.neighbour_qmap(f, seeds, shape, stride, step=0.125, tol_coef=0.5)
+    torch.cuda.synchronize()
+    assert int(got["n_pseudo"]) == int(want["n_pseudo"]) > 0
+    a, b = got["q_all"].cpu().numpy(), want["q_all"].cpu().numpy()
+    assert a.shape == (nv, nu)
+    assert np.array_equal(a, b, equal_nan=True)
+    assert np.isfinite(a[1:-1, 1:-1]).mean() > 0.5
