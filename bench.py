@@ -1087,7 +1087,7 @@ def summary_of(rec, keys=("value", "unit", "ms_per_step", "latency_ms", "speedup
     out = {k: rec[k] for k in keys if k in rec}
     out["workload"] = rec.get("config", {}).get("workload")
     for k in ("fp64_per_line", "fp32c_per_line", "persistent_vs_per_line_fp32c", "one_call_per_request_fp32c",
-              "error_vs_oracle", "cpu_baseline", "sweep", "fp64_plain_serial", "mixed"):
+              "error_vs_oracle", "cpu_baseline", "sweep", "fp64_plain_serial", "mixed", "tuning"):
         if rec.get(k) is not None:
             out[k] = rec[k]
     if isinstance(rec.get("roofline"), dict):
@@ -1105,6 +1105,7 @@ def other_configs(args, fq, torch, log):
     t0 = time.perf_counter()
     out["C1_quadrupole_rk4"] = bench_c1(args, fq, torch, log)
     a = copy.copy(args)
+    # default knobs: autotuning on the cube (10 s) picks them with the longest-first order
     a.workload, a.steps, a.warmup, a.no_autotune, a.cpu_sample = "volume", 1, 1, True, 1 << 14
     a.no_mixed = True
     out["C3_volume"] = summary_of(bench_trace(a, fq, torch, log, True))
