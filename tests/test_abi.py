@@ -139,6 +139,8 @@ def test_host_only_calls(L):
     prm = _lib.fqsl_scott_params()
     out = _lib.fqsl_scott_out()
     assert L.fqsl_qmap_plane(None, ctypes.byref(pl), ctypes.byref(prm), ctypes.byref(out), None) != _lib.FQSL_OK
+    npm = _lib.fqsl_neighbour_params()
+    assert L.fqsl_qmap_plane_neighbour(None, ctypes.byref(pl), ctypes.byref(npm), None, None, None) != _lib.FQSL_OK
     assert L.fqsl_last_error().decode()
     v = ctypes.c_int(-1)
     assert L.fqsl_tune_get(-1, ctypes.byref(v)) != _lib.FQSL_OK
