@@ -402,6 +402,15 @@ FQSL_HD void fd_quotient(const R B[3], const R ap[3], const R am[3], R out[3]) {
     for (int q = 0; q < 3; ++q) out[q] = D[q] * t1 - S[q] * t2;
 }
 
+// True when `v` holds on some active lane of the wavefront (on the host: v).
+FQSL_HD bool wave_any(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __any(v) != 0;
+#else
+    return v;
+#endif
+}
+
 // Raw stage position: x = i + r per axis (i = floor(x)).
 template <typename R>
 struct Pt {
@@ -549,7 +558,12 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     f.template load<R>(c[0], c[1], c[2], cc);
     constexpr bool kPre = F::kCoef && !kExact;
     typename std::conditional<kPre, NbCoef, NbNone>::type nb;
-    if constexpr (kPre) nb_fetch<F>(f, p, c, nb);
+    if constexpr (kPre) {
+        // only a lane with a non-simple axis can cross; most stages of a
+        // wavefront have none and skip the crossing test
+        nb.j = -1;
+        if (wave_any(!(simple[0] && simple[1] && simple[2]))) nb_fetch<F>(f, p, c, nb);
+    }
     R B[3], d[3][3];
     field_eval<F, R>(cc, w[0], w[1], w[2], B, d);
     const R inb = rsq(dot3(B, B));
