@@ -486,12 +486,13 @@ FQSL_HD void nb_fetch(const F& f, const Pt<float>& p, const int* c, NbCoef& nb) 
 // takes the neighbour cell's edge derivative where the +-h sample crosses a
 // grid plane (crossing on both sides at once would need 2h > 1 cell).
 template <int J, typename R, class F, class Nb>
-FQSL_HD void scott_axis(const F& f, const Pt<R>& p, const int* c, const R* w, bool simple, const R B[3],
-                        const R b[3], R inb, const R dj[3], const Nb& nb, R out[3]) {
+FQSL_HD void scott_axis(const F& f, const Pt<R>& p, const int* c, const R* w, bool simple, bool flat,
+                        const R B[3], const R b[3], R inb, const R dj[3], const Nb& nb, R out[3]) {
     if (simple) {
         const R bd = dot3(b, dj);
+        const R g = flat ? (R)0 : inb;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) out[q] = (dj[q] - b[q] * bd) * inb;
+        for (int q = 0; q < 3; ++q) out[q] = (dj[q] - b[q] * bd) * g;
         return;
     }
     constexpr bool kPre = std::is_same<Nb, NbCoef>::value;
@@ -536,7 +537,8 @@ FQSL_HD void scott_axis(const F& f, const Pt<R>& p, const int* c, const R* w, bo
 // of b, at most ~2e-7 relative for a field whose scale length is >= 1 cell,
 // below float resolution.  Axes whose samples cross a grid plane or the box
 // face keep the exact quotient, which there blends two cells or halves the
-// one-sided slope.
+// one-sided slope; an axis whose samples both lie beyond one face has a zero
+// row (the clamped interpolant is constant there).
 template <typename R, class F>
 FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     constexpr bool kExact = sizeof(R) == 8;
@@ -558,20 +560,35 @@ FQSL_HD void scott_eval(const F& f, const Pt<R>& p, R b[3], R J[3][3]) {
     f.template load<R>(c[0], c[1], c[2], cc);
     constexpr bool kPre = F::kCoef && !kExact;
     typename std::conditional<kPre, NbCoef, NbNone>::type nb;
-    if constexpr (kPre) {
+    if constexpr (kPre) nb.j = -1;
+    bool flat[3] = {false, false, false};
+    if (!kExact && wave_any(!(simple[0] && simple[1] && simple[2]))) {
+        // An axis whose +-h samples both lie beyond one face of the box reads
+        // the same clamped value on both sides: its quotient row is zero.
+        // Such a lane takes the closed form scaled by zero instead of the
+        // quotient (a stage point outside the box, as in a line's last
+        // attempt); only the sign of a zero entry can differ.
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int i = p.i[a], n = f.n[a];
+            const R r = p.r[a];
+            const bool below = i < -1 || (i == -1 && !(r > (R)1 - h));      // x <= -h
+            const bool above = i >= n - 1 && !(((R)(i - (n - 1)) + r) < h);  // x >= N-1+h
+            flat[a] = r == r && (below || above);
+            simple[a] = simple[a] || flat[a];
+        }
         // only a lane with a non-simple axis can cross; most stages of a
         // wavefront have none and skip the crossing test
-        nb.j = -1;
-        if (wave_any(!(simple[0] && simple[1] && simple[2]))) nb_fetch<F>(f, p, c, nb);
+        if constexpr (kPre) nb_fetch<F>(f, p, c, nb);
     }
     R B[3], d[3][3];
     field_eval<F, R>(cc, w[0], w[1], w[2], B, d);
     const R inb = rsq(dot3(B, B));
 #pragma unroll
     for (int q = 0; q < 3; ++q) b[q] = B[q] * inb;
-    scott_axis<0, R, F>(f, p, c, w, simple[0], B, b, inb, d[0], nb, J[0]);
-    scott_axis<1, R, F>(f, p, c, w, simple[1], B, b, inb, d[1], nb, J[1]);
-    scott_axis<2, R, F>(f, p, c, w, simple[2], B, b, inb, d[2], nb, J[2]);
+    scott_axis<0, R, F>(f, p, c, w, simple[0], flat[0], B, b, inb, d[0], nb, J[0]);
+    scott_axis<1, R, F>(f, p, c, w, simple[1], flat[1], B, b, inb, d[1], nb, J[1]);
+    scott_axis<2, R, F>(f, p, c, w, simple[2], flat[2], B, b, inb, d[2], nb, J[2]);
 }
 
 // The mixed variant's right-hand side at a double stage point: b in FP64
@@ -638,9 +655,9 @@ FQSL_HD void scott_eval_mixed(const F& f, const Pt<double>& pd, double bd[3], fl
         B[q] = (float)Bd[q];
     }
     const float inb = (float)inbd;
-    scott_axis<0, float, F>(f, p, c, w, simple[0], B, b, inb, d[0], nb, J[0]);
-    scott_axis<1, float, F>(f, p, c, w, simple[1], B, b, inb, d[1], nb, J[1]);
-    scott_axis<2, float, F>(f, p, c, w, simple[2], B, b, inb, d[2], nb, J[2]);
+    scott_axis<0, float, F>(f, p, c, w, simple[0], false, B, b, inb, d[0], nb, J[0]);
+    scott_axis<1, float, F>(f, p, c, w, simple[1], false, B, b, inb, d[1], nb, J[1]);
+    scott_axis<2, float, F>(f, p, c, w, simple[2], false, B, b, inb, d[2], nb, J[2]);
 }
 
 // Unit vector b only (plain tracer RHS: Interp3dxyzn(..., true)).
