@@ -410,7 +410,7 @@ def bench_neighbour(args, fq, torch, log):
     on z = 1e-4 (a quarter-cell stride), QCalcPlane (FastQSL.py:65-117), four
     pseudo lines per B_flag pixel traced with tol_coef 0.1 (CookPseudoLine,
     FastQSL.py:4-57), Qcut and the merge.  One step = one full map; inputs
-    resident; the K timed maps run as one job over two streams
+    resident; the K timed maps run as one job over four streams
     (qmap.neighbour_qmaps), and the line also carries the one-map-at-a-time
     rate.  The CPU leg runs the same pipeline (FP64 C oracle lines, the
     NumPy float32 restatements of the map steps) on the central block of the
@@ -445,14 +445,15 @@ def bench_neighbour(args, fq, torch, log):
     torch.cuda.synchronize()
     wall1 = time.perf_counter() - t1
     npseudo = int(res["n_pseudo"])
-    # the job: K maps over two streams, so one map's pseudo-line trace (a few
-    # long lines) runs beside the next map's main trace
+    # the job: K maps over four streams, so a map's pseudo-line trace (a few
+    # long lines) runs beside the other maps' main traces
+    nstreams = 4
     jobs = [(st, shape, stride)] * steps
     for _ in range(warm):
-        qmap.neighbour_qmaps(field, jobs[:2], streams=2, **kw)
+        qmap.neighbour_qmaps(field, jobs[:nstreams], streams=nstreams, **kw)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    piped = qmap.neighbour_qmaps(field, jobs, streams=2, **kw)
+    piped = qmap.neighbour_qmaps(field, jobs, streams=nstreams, **kw)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     value = n * n * steps / wall
@@ -470,11 +471,11 @@ def bench_neighbour(args, fq, torch, log):
     evals = int(tr["evals"].to(torch.int64).sum().item())
     k = float(np.mean(kms))
     bpe = BYTES_PER_EVAL_F64 if args.precision == "fp64" else BYTES_PER_EVAL_F32
-    log(f"neighbour {args.precision}/{args.schedule}: {value:.4g} pixels/s over 2 streams "
+    log(f"neighbour {args.precision}/{args.schedule}: {value:.4g} pixels/s over {nstreams} streams "
         f"({wall / steps * 1e3:.2f} ms per map; one at a time {wall1 / steps * 1e3:.2f} ms), {npseudo} pseudo "
         f"pixels; main trace {k:.2f} ms, {evals / (n * n):.1f} evals/line; maps identical: {same}")
     extra = {"one_map_at_a_time": {"pixels_per_s": n * n * steps / wall1, "ms_per_map": wall1 / steps * 1e3},
-             "streams": 2, "maps_identical_to_one_at_a_time": same}
+             "streams": nstreams, "maps_identical_to_one_at_a_time": same}
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -524,7 +525,7 @@ def bench_neighbour(args, fq, torch, log):
         "vs_baseline": None, "dtype": "f32" if args.precision == "fp32c" else "f64", "data": "synthetic",
         "config": {"workload": "random multipole (40 sources, default_rng(0)) 256^3 float32, plain RKF45 "
                                "s_len 1/8 tol_coef 0.5, z=1e-4 plane %dx%d, QCalcPlane + pseudo lines "
-                               "(tol_coef 0.1) + Qcut; K maps over 2 streams" % (n, n),
+                               "(tol_coef 0.1) + Qcut; K maps over %d streams" % (n, n, nstreams),
                    "pixels_per_step": n * n, "pseudo_pixels": npseudo, "precision": args.precision,
                    "schedule": args.schedule, "parallelism": "single GPU"},
         "roofline": hbm_roofline("neighbour", "k_plain_%s (main trace)" % args.schedule, k, evals, bpe,

@@ -12,7 +12,7 @@
   (demo/Q_RKF45MixCuPyDiffScott.ipynb cells [7]-[9]).
 * ``neighbour_qmaps`` -- several ``neighbour_qmap`` jobs over HIP streams, so
   one map's pseudo-line trace (a few long lines that leave most of the GPU
-  idle) runs beside the next map's main trace.
+  idle) runs beside the other maps' main traces.
 
 Seed grids use the notebooks' conventions (``linspace`` x ``meshgrid``, rows
 = the second in-plane axis), see ``synth.plane_seeds_z`` / ``plane_seeds_abc``.
@@ -300,41 +300,27 @@ def neighbour_qmap(field: Field, seeds, shape, stride, normal=(0.0, 0.0, 1.0), m
                 n_pseudo=c["counts"][0], _pseudo=(c, rc))
 
 
-def neighbour_qmaps(field: Field, jobs, streams=2, **kw):
-    """``neighbour_qmap`` for each (seeds, shape, stride) of ``jobs``, the
-    jobs dealt round-robin to ``streams`` host threads, each issuing its
-    jobs in order on its own HIP stream (default priority).  Keyword
-    arguments go to every ``neighbour_qmap``.  Returns the results in job
-    order; they are complete when this returns (the streams are
-    synchronised)."""
-    import threading
+def neighbour_qmaps(field: Field, jobs, streams=4, **kw):
+    """``neighbour_qmap`` for each (seeds, shape, stride) of ``jobs``, job j
+    issued on HIP stream j mod ``streams`` (default priority) from the
+    calling thread.  A map holds no host synchronisation (its pseudo-line
+    count stays on the device), so one thread keeps every stream fed, and a
+    map's pseudo-line trace runs beside the main traces of the maps on the
+    other streams; four streams match the four hardware queues a process
+    gets by default.  Keyword arguments go to every ``neighbour_qmap``.
+    Returns the results in job order; they are complete on the caller's
+    current stream (it waits for every pool stream)."""
     import torch
     jobs = list(jobs)
-    out = [None] * len(jobs)
-    errors = []
     nstreams = max(1, min(int(streams), len(jobs)))
     pool = [torch.cuda.Stream() for _ in range(nstreams)]
     base = torch.cuda.current_stream()
     for s in pool:
         s.wait_stream(base)   # inputs written on the caller's stream are visible
-
-    def worker(k):
-        try:
-            with torch.cuda.stream(pool[k]):
-                for j in range(k, len(jobs), nstreams):
-                    seeds, shape, stride = jobs[j]
-                    out[j] = neighbour_qmap(field, seeds, shape, stride, **kw)
-                pool[k].synchronize()
-        except BaseException as e:  # re-raised on the calling thread
-            errors.append(e)
-
-    threads = [threading.Thread(target=worker, args=(k,)) for k in range(nstreams)]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    if errors:
-        raise errors[0]
+    out = []
+    for j, (seeds, shape, stride) in enumerate(jobs):
+        with torch.cuda.stream(pool[j % nstreams]):
+            out.append(neighbour_qmap(field, seeds, shape, stride, **kw))
     for s in pool:
         base.wait_stream(s)
 

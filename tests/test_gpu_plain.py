@@ -327,10 +327,11 @@ def test_empty_inputs(fq):
     torch.cuda.synchronize()
 
 
-def test_neighbour_qmaps_over_streams_match_serial(fq):
-    """Maps run as one job over two streams (qmap.neighbour_qmaps) equal the
-    same maps run one at a time, bit for bit (a line's result does not
-    depend on the launch or stream that traces it)."""
+@pytest.mark.parametrize("streams", [2, 4])
+def test_neighbour_qmaps_over_streams_match_serial(fq, streams):
+    """Maps run as one job over two or four streams (qmap.neighbour_qmaps)
+    equal the same maps run one at a time, bit for bit (a line's result does
+    not depend on the launch or stream that traces it)."""
     import torch
     field = synth.quadrupole_64(dtype=np.float64, device="cpu", n=48)
     jobs = []
@@ -341,7 +342,7 @@ def test_neighbour_qmaps_over_streams_match_serial(fq):
     with fq.Field(*field) as f:
         serial = [fq.qmap.neighbour_qmap(f, s, sh, st, **kw) for s, sh, st in jobs]
         torch.cuda.synchronize()
-        piped = fq.qmap.neighbour_qmaps(f, jobs, streams=2, **kw)
+        piped = fq.qmap.neighbour_qmaps(f, jobs, streams=streams, **kw)
         torch.cuda.synchronize()
         for a, b in zip(serial, piped):
             assert int(a["n_pseudo"]) == int(b["n_pseudo"])
