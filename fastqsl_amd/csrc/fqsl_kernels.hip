@@ -2396,7 +2396,12 @@ extern "C" int fqsl_wait(fqsl_server* S, uint64_t id) {
     std::unique_lock<std::mutex> lk(S->mu);
     if (!S->done.count(id) && !S->pending.count(id))
         return set_err(FQSL_ERR_INVALID, "unknown request id %llu", (unsigned long long)id);
-    S->cv_done.wait(lk, [S, id] { return S->done.count(id) > 0; });
+    // a request leaves `pending` when it enters `done`; if it is in neither
+    // on wake-up, another thread's wait took it first
+    S->cv_done.wait(lk, [S, id] { return S->done.count(id) > 0 || !S->pending.count(id); });
+    if (!S->done.count(id))
+        return set_err(FQSL_ERR_INVALID, "request id %llu was waited for by another call",
+                       (unsigned long long)id);
     const ServerDone d = S->done[id];
     S->done.erase(id);
     if (d.status) return set_err(d.status, "%s", d.err.c_str());

@@ -294,11 +294,14 @@ int fqsl_submit(fqsl_server *server, const fqsl_scott_request *req, void *stream
 int fqsl_poll(fqsl_server *server, uint64_t id, int *done);
 
 /* Blocks until request `id` is done, returns its status (FQSL_OK or the
- * batch's error, with fqsl_last_error set on this thread) and forgets it. */
+ * batch's error, with fqsl_last_error set on this thread) and forgets it.
+ * Of several calls waiting on one id, one gets the status and the others
+ * fail with FQSL_ERR_INVALID. */
 int fqsl_wait(fqsl_server *server, uint64_t id);
 
 /* Traces what is still queued, stops the consumer and frees the server;
- * requests not yet waited for are forgotten. */
+ * requests not yet waited for are forgotten.  No other call on the server
+ * may be in progress or follow. */
 int fqsl_server_destroy(fqsl_server *server);
 
 /* ------------------------------------------------------------ plain tracer */

@@ -9,7 +9,8 @@
 * ``fqsl_submit`` / ``fqsl_poll`` / ``fqsl_wait``: requests over two fields,
   both methods and uneven sizes, submitted from two host threads, are traced
   by the server's consumer in batches; each request's outputs equal tracing it
-  alone, bit for bit.  Unknown and already-waited ids fail.
+  alone, bit for bit.  Unknown and already-waited ids fail, also for a
+  second thread waiting on the same id.
 """
 import threading
 
@@ -119,6 +120,40 @@ def test_native_server_matches_alone(fq, fields):
         a = fq.trace_scott(f, s[:, 0], s[:, 1], s[:, 2], method=m, step=0.25, tol=tol, precision="fp32c")
         torch.cuda.synchronize()
         _same(r, a, ("q_0", "q_perp", "flag_start", "flag_end", "evals"))
+
+
+def test_native_server_one_wait_per_id(fq, fields):
+    """Two threads waiting on one request id: one gets the outputs, the
+    other an error (the id is forgotten once waited for), and neither
+    blocks."""
+    import torch
+    from fastqsl_amd.serve import NativeServer
+    rng = np.random.default_rng(11)
+    seeds = np.column_stack([rng.uniform(2, 60, 4000), rng.uniform(2, 60, 4000),
+                             rng.uniform(0, 20, 4000)]).astype(np.float32)
+    srv = NativeServer("fp32c")
+    rid = srv.submit(fields[0], seeds[:, 0], seeds[:, 1], seeds[:, 2], outputs=("q_perp", "evals"))
+    got = []
+
+    def waiter():
+        try:
+            got.append(("ok", srv.wait(rid)))
+        except fq.FqslError:
+            got.append(("error", None))
+
+    th = [threading.Thread(target=waiter) for _ in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in th)
+    assert sorted(g[0] for g in got) == ["error", "ok"]
+    res = next(g[1] for g in got if g[0] == "ok")
+    srv.close()
+    want = fq.trace_scott(fields[0], seeds[:, 0], seeds[:, 1], seeds[:, 2], precision="fp32c",
+                          outputs=("q_perp", "evals"))
+    torch.cuda.synchronize()
+    _same(res, want, ("q_perp", "evals"))
 
 
 @pytest.mark.parametrize("kind", ["z", "abc"])
