@@ -225,7 +225,8 @@ typedef struct {
     int kind;      /* FQSL_PLANE_* */
     int n_u, n_v;  /* pixels per row and rows; maps are n_v x n_u, row-major */
     int reserved;
-    double p[9];   /* FQSL_PLANE_Z: x0, x1, y0, y1, z0 (p[5..8] unused); FQSL_PLANE_ABC: A, B, C */
+    double p[9];   /* FQSL_PLANE_Z: x0, x1, y0, y1, z0 (p[5..8] unused, finite, e.g. 0);
+                      FQSL_PLANE_ABC: A, B, C.  A non-finite entry is rejected. */
 } fqsl_plane;
 
 /* Scott Q map of a plane of seeds, formed on the device: linspace as NumPy
