@@ -1088,7 +1088,8 @@ def summary_of(rec, keys=("value", "unit", "ms_per_step", "latency_ms", "speedup
     out = {k: rec[k] for k in keys if k in rec}
     out["workload"] = rec.get("config", {}).get("workload")
     for k in ("fp64_per_line", "fp32c_per_line", "persistent_vs_per_line_fp32c", "one_call_per_request_fp32c",
-              "error_vs_oracle", "cpu_baseline", "sweep", "fp64_plain_serial", "mixed", "tuning"):
+              "error_vs_oracle", "cpu_baseline", "sweep", "fp64_plain_serial", "mixed", "tuning",
+              "one_map_at_a_time", "streams", "maps_identical_to_one_at_a_time", "without_twist"):
         if rec.get(k) is not None:
             out[k] = rec[k]
     if isinstance(rec.get("roofline"), dict):
@@ -1120,6 +1121,26 @@ def other_configs(args, fq, torch, log):
     out["C5_serve"] = summary_of(bench_serve(a, fq, torch, log))
     torch.cuda.empty_cache()
     log(f"other configs in {time.perf_counter() - t0:.1f} s")
+    return out
+
+
+def other_workloads(args, fq, torch, log):
+    """The workloads either side of the Scott path -- the neighbour-difference
+    map, the stretched-grid tracer and the twist number -- at their bench
+    sizes with their default repetitions, for the default run's record."""
+    import copy
+    out = {}
+    t0 = time.perf_counter()
+    for key, fn, workload in (("neighbour_qmap", bench_neighbour, "neighbour"),
+                              ("stretched_grid", bench_stretch, "stretch"),
+                              ("twist", bench_twist, "twist")):
+        a = copy.copy(args)
+        a.workload, a.steps, a.warmup = workload, None, None
+        a.schedule = "per_line" if workload == "stretch" else "persistent"
+        rec = fn(a, fq, torch, log)
+        out[key] = dict(metric=rec["metric"], **summary_of(rec))
+        torch.cuda.empty_cache()
+    log(f"other workloads in {time.perf_counter() - t0:.1f} s")
     return out
 
 
@@ -1161,6 +1182,7 @@ def main():
         if args.workload == "plane" and not args.headline_only:
             torch.cuda.empty_cache()
             rec["configs"] = other_configs(args, fq, torch, log)
+            rec["other_workloads"] = other_workloads(args, fq, torch, log)
     print(json.dumps(rec), flush=True)
 
 
