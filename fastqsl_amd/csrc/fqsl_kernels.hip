@@ -3122,10 +3122,11 @@ extern "C" int fqsl_bad_pixels(uint64_t n, const int32_t* b_flag, const float* q
     if (n > 0xffffffffull) return set_err(FQSL_ERR_INVALID, "bad-pixel compaction supports < 2^32 pixels");
     hipStream_t st = (hipStream_t)stream;
     const unsigned nblocks = (unsigned)((n + kCompactBlock - 1) / kCompactBlock);
+    // block counts and the total in one stream-ordered allocation
+    const size_t counts_bytes = align256((size_t)nblocks * sizeof(unsigned) + 16);
     unsigned* counts = nullptr;
-    unsigned long long* total = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&counts, nblocks * sizeof(unsigned) + 16, st));
-    HIP_TRY(hipMallocAsync((void**)&total, sizeof(unsigned long long), st));
+    HIP_TRY(hipMallocAsync((void**)&counts, counts_bytes + sizeof(unsigned long long), st));
+    unsigned long long* total = (unsigned long long*)((char*)counts + counts_bytes);
     k_bad_count<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, q, counts);
     k_pseudo_scan<<<1, 1024, 0, st>>>(nblocks, counts, total);
     k_bad_write<<<nblocks, kCompactBlock, 0, st>>>(n, b_flag, q, x, y, z, counts, capacity, seed_x, seed_y, seed_z,
@@ -3134,7 +3135,6 @@ extern "C" int fqsl_bad_pixels(uint64_t n, const int32_t* b_flag, const float* q
     unsigned long long cnt = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&cnt, total, sizeof(cnt), hipMemcpyDeviceToHost, st);
     (void)hipFreeAsync(counts, st);
-    (void)hipFreeAsync(total, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return set_err(FQSL_ERR_HIP, "bad-pixel compaction: %s", hipGetErrorString(e));
     *count_host = cnt;
