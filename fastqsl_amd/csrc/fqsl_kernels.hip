@@ -404,7 +404,7 @@ extern "C" int fqsl_field_stretched(const fqsl_field* f) { return f && f->stretc
 
 // ------------------------------------------------------------- tuning knobs
 // Read by every launch, written by fqsl_tune_set from any host thread.
-static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {2, 100, 1, 8, 1, 64, 64, 1};
+static std::atomic<int> g_tune[FQSL_TUNE_COUNT] = {3, 100, 1, 8, 1, 64, 64, 1};
 
 extern "C" int fqsl_tune_set(int knob, int value) {
     if (knob < 0 || knob >= FQSL_TUNE_COUNT) return set_err(FQSL_ERR_INVALID, "tuning knob %d", knob);
@@ -887,10 +887,17 @@ __global__ void __launch_bounds__(256) k_scott_per_line(F f, ScottParams P, Seed
     scott_per_line_body<R, F, FJ, X>(f, P, s, n, out, ds, fj);
 }
 
-// FP32 variant compiled for two waves per SIMD (like the persistent kernel).
+// FP32 variants compiled for two or three waves per SIMD (like the
+// persistent kernel).
 template <typename R, class F, class FJ, typename X>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_scott_per_line_w2(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds, FJ fj) {
+    scott_per_line_body<R, F, FJ, X>(f, P, s, n, out, ds, fj);
+}
+
+template <typename R, class F, class FJ, typename X>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_scott_per_line_w3(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds, FJ fj) {
     scott_per_line_body<R, F, FJ, X>(f, P, s, n, out, ds, fj);
 }
 
@@ -1203,10 +1210,19 @@ constexpr ScottWs scott_ws_layout(unsigned long long n, bool ordered) {
     return w;
 }
 
+// Waves per SIMD a float Scott kernel is compiled for: the knob for FP32c
+// (its stages rolled); the mixed variant and the twist tracer (stages
+// unrolled) fit two; FP64 is left to the compiler.
+template <typename R, typename X, class FJ>
+static int scott_waves() {
+    if (sizeof(R) != 4) return 0;
+    return rolled_stages<R, X, FJ>() ? g_tune[FQSL_TUNE_SCOTT_WAVES].load() : 2;
+}
+
 // Stepping kernel and grid of the persistent Scott schedule.
 template <typename R, class F, class FJ, typename X>
 static int scott_persistent_kernel(decltype(&k_scott_persistent<R, F, FJ, X>)* kern, int* grid) {
-    const int waves = sizeof(R) == 4 ? g_tune[FQSL_TUNE_SCOTT_WAVES].load() : 0;
+    const int waves = scott_waves<R, X, FJ>();
     *kern = waves == 2 ? k_scott_persistent_w2<R, F, FJ, X>
           : waves == 3 ? k_scott_persistent_w3<R, F, FJ, X>
                        : k_scott_persistent<R, F, FJ, X>;
@@ -1321,8 +1337,10 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         int rc = record_start(st);
         if (rc) return rc;
         if (noalloc) return set_err(FQSL_ERR_INVALID, "the no-alloc entry runs the persistent schedule only");
-        auto kern = sizeof(R) == 4 && g_tune[FQSL_TUNE_SCOTT_WAVES] == 2 ? k_scott_per_line_w2<R, F, FJ, X>
-                                                                        : k_scott_per_line<R, F, FJ, X>;
+        const int waves = scott_waves<R, X, FJ>();
+        auto kern = waves == 2 ? k_scott_per_line_w2<R, F, FJ, X>
+                  : waves == 3 ? k_scott_per_line_w3<R, F, FJ, X>
+                               : k_scott_per_line<R, F, FJ, X>;
         kern<<<per_line_grid(n_hint, 256, ds), 256, 0, st>>>(f, P, s, n, o, ds, fj);
         HIP_TRY(hipGetLastError());
         return record_end(st);
@@ -1835,6 +1853,13 @@ k_batch_step_w2(F f, int method, BatchGroupArgs g, const R* frames, ScottRec<R>*
     scott_batch_body<R, F, X>(f, method, g, frames, recs, c, qi, refill_min);
 }
 
+template <typename R, class F, typename X>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_batch_step_w3(F f, int method, BatchGroupArgs g, const R* frames, ScottRec<R>* recs, BatchCtl c, int qi,
+                int refill_min) {
+    scott_batch_body<R, F, X>(f, method, g, frames, recs, c, qi, refill_min);
+}
+
 template <typename R, class F>
 __global__ void __launch_bounds__(256) k_batch_finish(F f, BatchGroupArgs g, const R* __restrict__ frames,
                                                       const ScottRec<R>* __restrict__ recs) {
@@ -1944,7 +1969,8 @@ int batch_launch_group(const fqsl_scott_batch* b, const BatchGroup& G, int gi, c
     BatchGroupArgs g{b->reqs, b->offs, G.r0, G.r1, G.base, G.n};
     R* frames = (R*)b->frames;
     ScottRec<R>* recs = (ScottRec<R>*)b->recs;
-    auto kern = sizeof(R) == 4 && b->waves == 2 ? k_batch_step_w2<R, F, X> : k_batch_step<R, F, X>;
+    const int waves = sizeof(R) != 4 ? 0 : (sizeof(X) == 8 ? 2 : b->waves);
+    auto kern = waves == 2 ? k_batch_step_w2<R, F, X> : waves == 3 ? k_batch_step_w3<R, F, X> : k_batch_step<R, F, X>;
     int grid = 0;
     int rc = persistent_grid(kern, &grid);
     if (rc) return rc;

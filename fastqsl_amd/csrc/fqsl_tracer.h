@@ -837,8 +837,13 @@ FQSL_HD double scott_ctl(double tol) { return tol > 0 ? pow(tol / 11.09, 0.2) : 
 // y <- y (1.1 - 0.1 y^10 e2); each step maps a relative error d to 5.5 d^2,
 // so the result is exact to double rounding.  Zero, NaN, infinite and extreme
 // inputs take the library pow.
+// pow(e2, -0.1) out of line: inlined, its polynomial constants held in
+// registers cost the stepping kernels about 40 VGPRs for a path that only a
+// zero or huge error norm takes.
+FQSL_COLD double pow_minus_tenth_wide(double e2) { return pow(e2, -0.1); }
+
 FQSL_HD double pow_minus_tenth(double e2) {
-    if (!(e2 > 1e-30 && e2 < 1e30)) return pow(e2, -0.1);
+    if (!(e2 > 1e-30 && e2 < 1e30)) return pow_minus_tenth_wide(e2);
     double y = (double)exp2f(-0.1f * log2f((float)e2));
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
@@ -1024,6 +1029,256 @@ FQSL_HD int scott_side_done(ScottLine<R, X>& L, const F& f, ScottEnd<R>& e) {
     return kFinished;
 }
 
+// The stage sets of one attempt.  Each returns the increment of the state
+// (and, for RKF45, the squared position error estimate); the controller and
+// the step bookkeeping follow in scott_attempt.  The unrolled forms hold
+// every stage's derivative in registers; on the device the FP32 path runs the
+// same expressions with one copy of the stage evaluation in a loop and the
+// RKF45 stage derivatives k_1..k_5 in LDS, which lets the kernel run three
+// waves per SIMD instead of two -- the same bits.
+template <typename R, typename X, class F>
+FQSL_HD void rkf45_stages_unrolled(const F& f, const ScottLine<R, X>& L, X scx, R sc, X incx[3], R incu[6],
+                                   double& e2) {
+    // stage k_i = (kx_i, ku_i); t = the stage offset sum_j a_ij k_j
+    X x1[3], x2[3], x3[3], x4[3], x5[3], x6[3], tx[3];
+    R u1[6], u2[6], u3[6], u4[6], u5[6], u6[6], tu[6];
+    const X zx[3] = {0, 0, 0};
+    const R zu[6] = {0, 0, 0, 0, 0, 0};
+    scott_stage<R, X>(f, L, zx, zu, zu + 3, scx, sc, x1, u1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a21 * x1[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a21 * u1[c];
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x2, u2);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a31 * x1[c] + (X)Tab::a32 * x2[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a31 * u1[c] + (R)Tab::a32 * u2[c];
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x3, u3);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a41 * x1[c] + (X)Tab::a42 * x2[c] + (X)Tab::a43 * x3[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a41 * u1[c] + (R)Tab::a42 * u2[c] + (R)Tab::a43 * u3[c];
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x4, u4);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        tx[c] = (X)Tab::a51 * x1[c] + (X)Tab::a52 * x2[c] + (X)Tab::a53 * x3[c] + (X)Tab::a54 * x4[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        tu[c] = (R)Tab::a51 * u1[c] + (R)Tab::a52 * u2[c] + (R)Tab::a53 * u3[c] + (R)Tab::a54 * u4[c];
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x5, u5);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        tx[c] = (X)Tab::a61 * x1[c] + (X)Tab::a62 * x2[c] + (X)Tab::a63 * x3[c] + (X)Tab::a64 * x4[c] +
+                (X)Tab::a65 * x5[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        tu[c] = (R)Tab::a61 * u1[c] + (R)Tab::a62 * u2[c] + (R)Tab::a63 * u3[c] + (R)Tab::a64 * u4[c] +
+                (R)Tab::a65 * u5[c];
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x6, u6);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        incx[c] = (X)Tab::b1 * x1[c] + (X)Tab::b3 * x3[c] + (X)Tab::b4 * x4[c] + (X)Tab::b5 * x5[c] +
+                  (X)Tab::b6 * x6[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        incu[c] = (R)Tab::b1 * u1[c] + (R)Tab::b3 * u3[c] + (R)Tab::b4 * u4[c] + (R)Tab::b5 * u5[c] +
+                  (R)Tab::b6 * u6[c];
+    e2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double ec = Tab::e1 * (double)x1[c] + Tab::e3 * (double)x3[c] + Tab::e4 * (double)x4[c] +
+                          Tab::e5 * (double)x5[c] + Tab::e6 * (double)x6[c];
+        e2 += ec * ec;
+    }
+}
+
+template <typename R, typename X, class F>
+FQSL_HD void rk4_stages_unrolled(const F& f, const ScottLine<R, X>& L, X scx, R sc, X incx[3], R incu[6]) {
+    // RK4 (TraceBline.cu:226-234) on the 9-vector, fixed step
+    X kx[3], tx[3];
+    R ku[6], tu[6];
+    const X zx[3] = {0, 0, 0};
+    const R zu[6] = {0, 0, 0, 0, 0, 0};
+    const X halfx = (X)0.5;
+    const R half = (R)0.5;
+    scott_stage<R, X>(f, L, zx, zu, zu + 3, scx, sc, kx, ku);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { incx[c] = kx[c]; tx[c] = halfx * kx[c]; }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { incu[c] = ku[c]; tu[c] = half * ku[c]; }
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { incx[c] += (X)2 * kx[c]; tx[c] = halfx * kx[c]; }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { incu[c] += (R)2 * ku[c]; tu[c] = half * ku[c]; }
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { incx[c] += (X)2 * kx[c]; tx[c] = kx[c]; }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { incu[c] += (R)2 * ku[c]; tu[c] = ku[c]; }
+    scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) incx[c] = (incx[c] + kx[c]) * (X)(1.0 / 6.0);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) incu[c] = (incu[c] + ku[c]) * (R)(1.0 / 6.0);
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Threads per workgroup of every kernel that traces Scott lines (their
+// __launch_bounds__): the LDS array of the rolled RKF45 stages holds one
+// column per thread.
+constexpr int kScottBlock = 256;
+
+template <class F>
+__device__ void rkf45_stages_rolled(const F& f, const ScottLine<float, float>& L, float scx, float sc,
+                                    float incx[3], float incu[6], double& e2) {
+    using X = float;
+    using R = float;
+    __shared__ float klds[5 * 9 * kScottBlock];
+    const int tid = threadIdx.x;
+    auto K = [&](int j, int c) -> float& { return klds[(j * 9 + c) * kScottBlock + tid]; };
+    float x6[3], u6[6];
+#pragma clang loop unroll(disable)
+    for (int st = 0; st < 6; ++st) {
+        // the stage offset sum_j a_ij k_j, the unrolled form's expressions
+        float tx[3], tu[6];
+        switch (st) {
+        case 0:
+#pragma unroll
+            for (int c = 0; c < 3; ++c) tx[c] = 0.f;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) tu[c] = 0.f;
+            break;
+        case 1:
+#pragma unroll
+            for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a21 * K(0, c);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a21 * K(0, 3 + c);
+            break;
+        case 2:
+#pragma unroll
+            for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a31 * K(0, c) + (X)Tab::a32 * K(1, c);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a31 * K(0, 3 + c) + (R)Tab::a32 * K(1, 3 + c);
+            break;
+        case 3:
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                tx[c] = (X)Tab::a41 * K(0, c) + (X)Tab::a42 * K(1, c) + (X)Tab::a43 * K(2, c);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                tu[c] = (R)Tab::a41 * K(0, 3 + c) + (R)Tab::a42 * K(1, 3 + c) + (R)Tab::a43 * K(2, 3 + c);
+            break;
+        case 4:
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                tx[c] = (X)Tab::a51 * K(0, c) + (X)Tab::a52 * K(1, c) + (X)Tab::a53 * K(2, c) +
+                        (X)Tab::a54 * K(3, c);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                tu[c] = (R)Tab::a51 * K(0, 3 + c) + (R)Tab::a52 * K(1, 3 + c) + (R)Tab::a53 * K(2, 3 + c) +
+                        (R)Tab::a54 * K(3, 3 + c);
+            break;
+        default:
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                tx[c] = (X)Tab::a61 * K(0, c) + (X)Tab::a62 * K(1, c) + (X)Tab::a63 * K(2, c) +
+                        (X)Tab::a64 * K(3, c) + (X)Tab::a65 * K(4, c);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                tu[c] = (R)Tab::a61 * K(0, 3 + c) + (R)Tab::a62 * K(1, 3 + c) + (R)Tab::a63 * K(2, 3 + c) +
+                        (R)Tab::a64 * K(3, 3 + c) + (R)Tab::a65 * K(4, 3 + c);
+            break;
+        }
+        float xs[3], us[6];
+        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, xs, us);
+        if (st < 5) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) K(st, c) = xs[c];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) K(st, 3 + c) = us[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x6[c] = xs[c];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) u6[c] = us[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        incx[c] = (X)Tab::b1 * K(0, c) + (X)Tab::b3 * K(2, c) + (X)Tab::b4 * K(3, c) + (X)Tab::b5 * K(4, c) +
+                  (X)Tab::b6 * x6[c];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        incu[c] = (R)Tab::b1 * K(0, 3 + c) + (R)Tab::b3 * K(2, 3 + c) + (R)Tab::b4 * K(3, 3 + c) +
+                  (R)Tab::b5 * K(4, 3 + c) + (R)Tab::b6 * u6[c];
+    e2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double ec = Tab::e1 * (double)K(0, c) + Tab::e3 * (double)K(2, c) + Tab::e4 * (double)K(3, c) +
+                          Tab::e5 * (double)K(4, c) + Tab::e6 * (double)x6[c];
+        e2 += ec * ec;
+    }
+}
+
+template <class F>
+__device__ void rk4_stages_rolled(const F& f, const ScottLine<float, float>& L, float scx, float sc, float incx[3],
+                                  float incu[6]) {
+    float tx[3] = {0.f, 0.f, 0.f};
+    float tu[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma clang loop unroll(disable)
+    for (int st = 0; st < 4; ++st) {
+        float kx[3], ku[6];
+        scott_stage<float, float>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
+        if (st == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { incx[c] = kx[c]; tx[c] = 0.5f * kx[c]; }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { incu[c] = ku[c]; tu[c] = 0.5f * ku[c]; }
+        } else if (st < 3) {
+            const float g = st == 1 ? 0.5f : 1.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { incx[c] += 2.f * kx[c]; tx[c] = g * kx[c]; }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { incu[c] += 2.f * ku[c]; tu[c] = g * ku[c]; }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) incx[c] = (incx[c] + kx[c]) * (float)(1.0 / 6.0);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) incu[c] = (incu[c] + ku[c]) * (float)(1.0 / 6.0);
+        }
+    }
+}
+#endif
+
+// The rolled form serves the FP32c tracer without the twist number (whose
+// extra loads at each accepted step leave no registers for three waves).
+template <typename R, typename X, class FJ>
+constexpr bool rolled_stages() {
+    return std::is_same<R, float>::value && std::is_same<X, float>::value && !has_current<FJ>();
+}
+
+template <typename R, typename X, class FJ, class F>
+FQSL_HD void rkf45_stages(const F& f, const ScottLine<R, X>& L, X scx, R sc, X incx[3], R incu[6], double& e2) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (rolled_stages<R, X, FJ>())
+        rkf45_stages_rolled<F>(f, L, scx, sc, incx, incu, e2);
+    else
+#endif
+        rkf45_stages_unrolled<R, X, F>(f, L, scx, sc, incx, incu, e2);
+}
+
+template <typename R, typename X, class FJ, class F>
+FQSL_HD void rk4_stages(const F& f, const ScottLine<R, X>& L, X scx, R sc, X incx[3], R incu[6]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (rolled_stages<R, X, FJ>())
+        rk4_stages_rolled<F>(f, L, scx, sc, incx, incu);
+    else
+#endif
+        rk4_stages_unrolled<R, X, F>(f, L, scx, sc, incx, incu);
+}
+
 // One integrator attempt (one pass of the while loop at
 // TraceBlineScott.cu:407-470).  Returns kTracing, kFinished (both footpoints
 // known: start in L.st, end in e) or kAborted (class 42: Q = 0).
@@ -1036,58 +1291,8 @@ FQSL_HD int scott_attempt(ScottLine<R, X>& L, const F& f, const ScottParams& P, 
     R incu[6];
     L.attempts += 1;
     if (P.method == kRKF45) {
-        // stage k_i = (kx_i, ku_i); t = the stage offset sum_j a_ij k_j
-        X x1[3], x2[3], x3[3], x4[3], x5[3], x6[3], tx[3];
-        R u1[6], u2[6], u3[6], u4[6], u5[6], u6[6], tu[6];
-        const X zx[3] = {0, 0, 0};
-        const R zu[6] = {0, 0, 0, 0, 0, 0};
-        scott_stage<R, X>(f, L, zx, zu, zu + 3, scx, sc, x1, u1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a21 * x1[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a21 * u1[c];
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x2, u2);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a31 * x1[c] + (X)Tab::a32 * x2[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a31 * u1[c] + (R)Tab::a32 * u2[c];
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x3, u3);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) tx[c] = (X)Tab::a41 * x1[c] + (X)Tab::a42 * x2[c] + (X)Tab::a43 * x3[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) tu[c] = (R)Tab::a41 * u1[c] + (R)Tab::a42 * u2[c] + (R)Tab::a43 * u3[c];
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x4, u4);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            tx[c] = (X)Tab::a51 * x1[c] + (X)Tab::a52 * x2[c] + (X)Tab::a53 * x3[c] + (X)Tab::a54 * x4[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            tu[c] = (R)Tab::a51 * u1[c] + (R)Tab::a52 * u2[c] + (R)Tab::a53 * u3[c] + (R)Tab::a54 * u4[c];
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x5, u5);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            tx[c] = (X)Tab::a61 * x1[c] + (X)Tab::a62 * x2[c] + (X)Tab::a63 * x3[c] + (X)Tab::a64 * x4[c] +
-                    (X)Tab::a65 * x5[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            tu[c] = (R)Tab::a61 * u1[c] + (R)Tab::a62 * u2[c] + (R)Tab::a63 * u3[c] + (R)Tab::a64 * u4[c] +
-                    (R)Tab::a65 * u5[c];
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, x6, u6);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            incx[c] = (X)Tab::b1 * x1[c] + (X)Tab::b3 * x3[c] + (X)Tab::b4 * x4[c] + (X)Tab::b5 * x5[c] +
-                      (X)Tab::b6 * x6[c];
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            incu[c] = (R)Tab::b1 * u1[c] + (R)Tab::b3 * u3[c] + (R)Tab::b4 * u4[c] + (R)Tab::b5 * u5[c] +
-                      (R)Tab::b6 * u6[c];
-        double e2 = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const double ec = Tab::e1 * (double)x1[c] + Tab::e3 * (double)x3[c] + Tab::e4 * (double)x4[c] +
-                              Tab::e5 * (double)x5[c] + Tab::e6 * (double)x6[c];
-            e2 += ec * ec;
-        }
+        double e2;
+        rkf45_stages<R, X, FJ, F>(f, L, scx, sc, incx, incu, e2);
         L.evals += 6;
         // step controller (TraceBlineScott.cu:413-419), FP64: the FP64 path
         // evaluates the reference's expression as written; the FP32 and mixed
@@ -1109,32 +1314,7 @@ FQSL_HD int scott_attempt(ScottLine<R, X>& L, const F& f, const ScottParams& P, 
         if (L.s < 1. / 10.) L.s = 1. / 10.;
     } else {
         // RK4 (TraceBline.cu:226-234) on the 9-vector, fixed step
-        X kx[3], tx[3];
-        R ku[6], tu[6];
-        const X zx[3] = {0, 0, 0};
-        const R zu[6] = {0, 0, 0, 0, 0, 0};
-        const X halfx = (X)0.5;
-        const R half = (R)0.5;
-        scott_stage<R, X>(f, L, zx, zu, zu + 3, scx, sc, kx, ku);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { incx[c] = kx[c]; tx[c] = halfx * kx[c]; }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) { incu[c] = ku[c]; tu[c] = half * ku[c]; }
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { incx[c] += (X)2 * kx[c]; tx[c] = halfx * kx[c]; }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) { incu[c] += (R)2 * ku[c]; tu[c] = half * ku[c]; }
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { incx[c] += (X)2 * kx[c]; tx[c] = kx[c]; }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) { incu[c] += (R)2 * ku[c]; tu[c] = ku[c]; }
-        scott_stage<R, X>(f, L, tx, tu, tu + 3, scx, sc, kx, ku);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) incx[c] = (incx[c] + kx[c]) * (X)(1.0 / 6.0);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) incu[c] = (incu[c] + ku[c]) * (R)(1.0 / 6.0);
+        rk4_stages<R, X, FJ, F>(f, L, scx, sc, incx, incu);
         L.evals += 4;
     }
 

@@ -4,8 +4,8 @@ The knobs (``_lib.TUNE_KNOBS``) change speed only -- a line's result does not
 depend on the lane, wave, queue or launch that traces it -- so they can be
 picked per workload by measurement:
 
-* ``scott_waves``  waves per SIMD the FP32 Scott stepping kernel is built for
-                   (2 = 256 VGPRs per lane; 0 = the compiler's choice);
+* ``scott_waves``  waves per SIMD the FP32c Scott stepping kernel is built for
+                   (3 = 168 VGPRs per lane, 2 = 256; 0 = the compiler's choice);
 * ``refill_min``   idle lanes a wavefront gathers before it claims new lines
                    (the queue chunk: one atomic per chunk);
 * ``queues``       one seed queue, or one per XCD (8), the launch's blocks
@@ -37,12 +37,12 @@ from . import _lib
 SPACE = {
     "order": (1, 2, 0),
     "order_buckets": (64, 16, 256),
-    "scott_waves": (2, 0),
+    "scott_waves": (3, 2, 0),
     "refill_min": (8, 4, 16, 32, 1),
     "queues": (1, 8),
     "grid_pct": (100, 75, 50),
 }
-DEFAULTS = {"order": 1, "order_buckets": 64, "scott_waves": 2, "refill_min": 8, "queues": 1,
+DEFAULTS = {"order": 1, "order_buckets": 64, "scott_waves": 3, "refill_min": 8, "queues": 1,
             "grid_pct": 100}
 
 
@@ -166,7 +166,7 @@ def autotune(field, x, y, z, method="rkf45", step=0.25, tol=1e-6, precision="fp3
 BATCH_SPACE = {
     "order": (1, 2, 0),
     "order_buckets": (64, 16, 256),
-    "scott_waves": (2, 0),
+    "scott_waves": (3, 2, 0),
     "refill_min": (8, 4, 16, 32),
     "grid_pct": (100, 75),
 }

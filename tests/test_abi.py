@@ -118,7 +118,7 @@ def test_host_only_calls(L):
     """Calls that do not reach the device: version string, tuning defaults,
     argument checks of the tuning knobs and the error message."""
     assert L.fqsl_version().decode().strip()
-    assert _lib.tune_get("scott_waves") == 2
+    assert _lib.tune_get("scott_waves") == 3
     assert _lib.tune_get("grid_pct") == 100
     assert _lib.tune_get("queues") == 1
     assert _lib.tune_get("refill_min") == 8

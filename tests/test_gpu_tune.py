@@ -35,7 +35,7 @@ def test_autotune_picks_knobs_that_do_not_change_results(tmp_path, monkeypatch):
             assert len(tune.load_cache()) == 1
             # every knob away from its default, results unchanged
             for alt in ({"refill_min": 1, "queues": 8, "grid_pct": 50}, {"order": 0}, {"order": 2, "order_buckets": 256},
-                        {"scott_waves": 0, "refill_min": 32}, cfg):
+                        {"scott_waves": 0, "refill_min": 32}, {"scott_waves": 2}, cfg):
                 tune.apply(dict(tune.DEFAULTS, **alt))
                 b = fq.trace_scott(field, *s, method="rkf45", step=0.25, tol=1e-6)
                 torch.cuda.synchronize()
