@@ -993,9 +993,10 @@ def bench_trace(args, fq, torch, log, volume):
 
     classes, counts = np.unique(flags, return_counts=True)
     prof = "volume" if volume else "plane"
+    waves = fq._lib.tune_get("scott_waves") if args.precision == "fp32c" else 0
     roof = valu_roofline(prof, kms, evals, bpe, oracle_evals,
-                         kernel=("stepping kernel k_scott_persistent_w2" if args.schedule == "persistent"
-                                 else "k_scott_per_line"))
+                         kernel=("stepping kernel k_scott_persistent%s" % ("_w%d" % waves if waves else "")
+                                 if args.schedule == "persistent" else "k_scott_per_line"))
     if not (args.precision == "fp32c" and args.schedule == "persistent" and (volume or args.pixels == 1024)):
         roof.update(achieved=None, frac=None, valu_insts_per_launch=None, traffic=None,
                     note="the profile covers the default FP32c persistent workload only")

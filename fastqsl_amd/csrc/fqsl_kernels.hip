@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(256) k_scott_prep(F f, ScottParams P, Seeds s,
 // queue entries once at least `refill_min` lanes of the wavefront are idle (or
 // all are), so the setup of refilled lines runs once for the group.  A wave
 // exits once every queue is empty and all its lanes are idle.
-template <typename R, class F, class FJ, typename X>
+template <typename R, class F, class FJ, typename X, bool kRoll = false>
 __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                       const R* __restrict__ frames, ScottRec<R>* __restrict__ recs,
                                                       DevScalars ds, unsigned long long* queues, int nq,
@@ -809,7 +809,7 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
         }
         if (active) {
             ScottEnd<R> e;
-            const int st = scott_attempt<R, F, FJ, X>(L, f, P, e, fj);
+            const int st = scott_attempt<R, F, FJ, X, kRoll>(L, f, P, e, fj);
             if (st != kTracing) {
                 recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                 active = false;
@@ -825,7 +825,8 @@ __device__ __forceinline__ void scott_persistent_body(F f, ScottParams P, Seeds 
     k_scott_persistent_w##W(F f, ScottParams P, Seeds s, unsigned long long n, const R* frames,       \
                             ScottRec<R>* recs, DevScalars ds, unsigned long long* queues, int nq,     \
                             int refill_min, const unsigned* order, FJ fj) {                           \
-        scott_persistent_body<R, F, FJ, X>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, fj); \
+        scott_persistent_body<R, F, FJ, X, W == 3>(f, P, s, n, frames, recs, ds, queues, nq, refill_min, order, \
+                                                  fj);                                                \
     }
 
 template <typename R, class F, class FJ, typename X>
@@ -865,7 +866,7 @@ __global__ void __launch_bounds__(256) k_scott_finish(F f, ScottParams P, unsign
 
 // One thread per line (grid-stride), the reference's mapping
 // (TraceBlineScott.cu:530-553).
-template <typename R, class F, class FJ, typename X>
+template <typename R, class F, class FJ, typename X, bool kRoll = false>
 __device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s, unsigned long long n,
                                                     ScottOut out, DevScalars ds, FJ fj) {
     apply_scalars(f, P, n, ds);
@@ -876,7 +877,7 @@ __device__ __forceinline__ void scott_per_line_body(F f, ScottParams P, Seeds s,
         ScottLine<R, X> L;
         ScottEnd<R> e;
         int st = scott_init<R>(L, f, P, s.x[i], s.y[i], s.z[i], e);
-        while (st == kTracing) st = scott_attempt<R, F, FJ, X>(L, f, P, e, fj);
+        while (st == kTracing) st = scott_attempt<R, F, FJ, X, kRoll>(L, f, P, e, fj);
         scott_write<R>(L, e, st, f, i, out, ds);
     }
 }
@@ -898,7 +899,7 @@ k_scott_per_line_w2(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut 
 template <typename R, class F, class FJ, typename X>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_scott_per_line_w3(F f, ScottParams P, Seeds s, unsigned long long n, ScottOut out, DevScalars ds, FJ fj) {
-    scott_per_line_body<R, F, FJ, X>(f, P, s, n, out, ds, fj);
+    scott_per_line_body<R, F, FJ, X, true>(f, P, s, n, out, ds, fj);
 }
 
 // ------------------------------------------------------------ plain kernels
@@ -1210,19 +1211,29 @@ constexpr ScottWs scott_ws_layout(unsigned long long n, bool ordered) {
     return w;
 }
 
-// Waves per SIMD a float Scott kernel is compiled for: the knob for FP32c
-// (its stages rolled); the mixed variant and the twist tracer (stages
-// unrolled) fit two; FP64 is left to the compiler.
+// Lines up to which a launch runs the two-wave FP32c kernel whatever the
+// knob: one two-wave grid's lanes (2,048 waves of 64 on 256 CUs).  Such a
+// launch leaves SIMD slots idle anyway, and its time is the latency of its
+// longest lines, which the unrolled two-wave kernel runs faster.
+static unsigned long long small_launch_lines() { return (unsigned long long)cu_count() * 4 * 2 * 64; }
+
+// Waves per SIMD a float Scott kernel runs with, for a launch of n lines:
+// the knob for FP32c (three: the rolled stages; two: unrolled), two for
+// small launches; the mixed variant and the twist tracer (stages unrolled)
+// run two; FP64 is left to the compiler.
 template <typename R, typename X, class FJ>
-static int scott_waves() {
+static int scott_waves(unsigned long long n) {
     if (sizeof(R) != 4) return 0;
-    return rolled_stages<R, X, FJ>() ? g_tune[FQSL_TUNE_SCOTT_WAVES].load() : 2;
+    if (!rollable_stages<R, X, FJ>()) return 2;
+    const int w = g_tune[FQSL_TUNE_SCOTT_WAVES].load();
+    return (w == 3 && n <= small_launch_lines()) ? 2 : w;
 }
 
 // Stepping kernel and grid of the persistent Scott schedule.
 template <typename R, class F, class FJ, typename X>
-static int scott_persistent_kernel(decltype(&k_scott_persistent<R, F, FJ, X>)* kern, int* grid) {
-    const int waves = scott_waves<R, X, FJ>();
+static int scott_persistent_kernel(decltype(&k_scott_persistent<R, F, FJ, X>)* kern, int* grid,
+                                   unsigned long long n) {
+    const int waves = scott_waves<R, X, FJ>(n);
     *kern = waves == 2 ? k_scott_persistent_w2<R, F, FJ, X>
           : waves == 3 ? k_scott_persistent_w3<R, F, FJ, X>
                        : k_scott_persistent<R, F, FJ, X>;
@@ -1305,7 +1316,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         if (n_hint == 0) return FQSL_OK;
         decltype(&k_scott_persistent<R, F, FJ, X>) kern = nullptr;
         int grid = 0;
-        int rc = scott_persistent_kernel<R, F, FJ, X>(&kern, &grid);
+        int rc = scott_persistent_kernel<R, F, FJ, X>(&kern, &grid, n_hint);
         if (rc) return rc;
         const bool ordered = scott_ordered<R>(ds, n_hint, grid);
         const ScottWs w = scott_ws_layout<R>(n_hint, ordered);
@@ -1337,7 +1348,7 @@ static int launch_scott(const F& f, const ScottParams& P, const Seeds& s, unsign
         int rc = record_start(st);
         if (rc) return rc;
         if (noalloc) return set_err(FQSL_ERR_INVALID, "the no-alloc entry runs the persistent schedule only");
-        const int waves = scott_waves<R, X, FJ>();
+        const int waves = scott_waves<R, X, FJ>(n_hint);
         auto kern = waves == 2 ? k_scott_per_line_w2<R, F, FJ, X>
                   : waves == 3 ? k_scott_per_line_w3<R, F, FJ, X>
                                : k_scott_per_line<R, F, FJ, X>;
@@ -1782,7 +1793,7 @@ __global__ void __launch_bounds__(256) k_batch_prep(F f, BatchGroupArgs g, R* __
 
 // Persistent stepping over one group: as scott_persistent_body, with the
 // step and tolerance of each lane's request.
-template <typename R, class F, typename X>
+template <typename R, class F, typename X, bool kRoll = false>
 __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs g, const R* __restrict__ frames,
                                                  ScottRec<R>* __restrict__ recs, BatchCtl c, int qi,
                                                  int refill_min) {
@@ -1830,7 +1841,7 @@ __device__ __forceinline__ void scott_batch_body(F f, int method, BatchGroupArgs
         }
         if (active) {
             ScottEnd<R> e;
-            const int st = scott_attempt<R, F, NoCurrent, X>(L, f, P, e);
+            const int st = scott_attempt<R, F, NoCurrent, X, kRoll>(L, f, P, e);
             if (st != kTracing) {
                 recs[idx] = ScottRec<R>{L.st, e, L.len, st, L.evals, L.tw};
                 active = false;
@@ -1857,7 +1868,7 @@ template <typename R, class F, typename X>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_batch_step_w3(F f, int method, BatchGroupArgs g, const R* frames, ScottRec<R>* recs, BatchCtl c, int qi,
                 int refill_min) {
-    scott_batch_body<R, F, X>(f, method, g, frames, recs, c, qi, refill_min);
+    scott_batch_body<R, F, X, true>(f, method, g, frames, recs, c, qi, refill_min);
 }
 
 template <typename R, class F>
@@ -1969,7 +1980,8 @@ int batch_launch_group(const fqsl_scott_batch* b, const BatchGroup& G, int gi, c
     BatchGroupArgs g{b->reqs, b->offs, G.r0, G.r1, G.base, G.n};
     R* frames = (R*)b->frames;
     ScottRec<R>* recs = (ScottRec<R>*)b->recs;
-    const int waves = sizeof(R) != 4 ? 0 : (sizeof(X) == 8 ? 2 : b->waves);
+    const int knob = sizeof(R) != 4 ? 0 : (sizeof(X) == 8 ? 2 : b->waves);
+    const int waves = (knob == 3 && G.n <= small_launch_lines()) ? 2 : knob;
     auto kern = waves == 2 ? k_batch_step_w2<R, F, X> : waves == 3 ? k_batch_step_w3<R, F, X> : k_batch_step<R, F, X>;
     int grid = 0;
     int rc = persistent_grid(kern, &grid);

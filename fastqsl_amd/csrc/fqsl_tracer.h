@@ -1252,27 +1252,28 @@ __device__ void rk4_stages_rolled(const F& f, const ScottLine<float, float>& L, 
 }
 #endif
 
-// The rolled form serves the FP32c tracer without the twist number (whose
-// extra loads at each accepted step leave no registers for three waves).
+// The rolled form serves the three-wave FP32c kernels (kRoll) without the
+// twist number (whose extra loads at each accepted step leave no registers
+// for a third wave); at two waves the unrolled form is faster.
 template <typename R, typename X, class FJ>
-constexpr bool rolled_stages() {
+constexpr bool rollable_stages() {
     return std::is_same<R, float>::value && std::is_same<X, float>::value && !has_current<FJ>();
 }
 
-template <typename R, typename X, class FJ, class F>
+template <typename R, typename X, class FJ, bool kRoll, class F>
 FQSL_HD void rkf45_stages(const F& f, const ScottLine<R, X>& L, X scx, R sc, X incx[3], R incu[6], double& e2) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (rolled_stages<R, X, FJ>())
+    if constexpr (kRoll && rollable_stages<R, X, FJ>())
         rkf45_stages_rolled<F>(f, L, scx, sc, incx, incu, e2);
     else
 #endif
         rkf45_stages_unrolled<R, X, F>(f, L, scx, sc, incx, incu, e2);
 }
 
-template <typename R, typename X, class FJ, class F>
+template <typename R, typename X, class FJ, bool kRoll, class F>
 FQSL_HD void rk4_stages(const F& f, const ScottLine<R, X>& L, X scx, R sc, X incx[3], R incu[6]) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (rolled_stages<R, X, FJ>())
+    if constexpr (kRoll && rollable_stages<R, X, FJ>())
         rk4_stages_rolled<F>(f, L, scx, sc, incx, incu);
     else
 #endif
@@ -1282,7 +1283,7 @@ FQSL_HD void rk4_stages(const F& f, const ScottLine<R, X>& L, X scx, R sc, X inc
 // One integrator attempt (one pass of the while loop at
 // TraceBlineScott.cu:407-470).  Returns kTracing, kFinished (both footpoints
 // known: start in L.st, end in e) or kAborted (class 42: Q = 0).
-template <typename R, class F, class FJ = NoCurrent, typename X = R>
+template <typename R, class F, class FJ = NoCurrent, typename X = R, bool kRoll = false>
 FQSL_HD int scott_attempt(ScottLine<R, X>& L, const F& f, const ScottParams& P, ScottEnd<R>& e,
                           const FJ& fj = FJ{}) {
     const R sc = (R)(L.s * L.dir);
@@ -1292,7 +1293,7 @@ FQSL_HD int scott_attempt(ScottLine<R, X>& L, const F& f, const ScottParams& P, 
     L.attempts += 1;
     if (P.method == kRKF45) {
         double e2;
-        rkf45_stages<R, X, FJ, F>(f, L, scx, sc, incx, incu, e2);
+        rkf45_stages<R, X, FJ, kRoll, F>(f, L, scx, sc, incx, incu, e2);
         L.evals += 6;
         // step controller (TraceBlineScott.cu:413-419), FP64: the FP64 path
         // evaluates the reference's expression as written; the FP32 and mixed
@@ -1314,7 +1315,7 @@ FQSL_HD int scott_attempt(ScottLine<R, X>& L, const F& f, const ScottParams& P, 
         if (L.s < 1. / 10.) L.s = 1. / 10.;
     } else {
         // RK4 (TraceBline.cu:226-234) on the 9-vector, fixed step
-        rk4_stages<R, X, FJ, F>(f, L, scx, sc, incx, incu);
+        rk4_stages<R, X, FJ, kRoll, F>(f, L, scx, sc, incx, incu);
         L.evals += 4;
     }
 

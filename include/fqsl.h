@@ -483,9 +483,11 @@ int fqsl_TraceAllBlineStretch(float *Bx, float *By, float *Bz, int *BshapeN, flo
 /* ---------------------------------------------------------- tuning knobs */
 /* Process-wide launch parameters (fastqsl_amd._lib.tune_set / tune_get).
  * They change speed only, never results. */
-#define FQSL_TUNE_SCOTT_WAVES 0 /* waves/SIMD the FP32c Scott kernels are compiled for:
-                                 0 = compiler's choice, 2 or 3 (default); the mixed
-                                 variant always runs two */
+#define FQSL_TUNE_SCOTT_WAVES 0 /* waves/SIMD of the FP32c Scott kernels: 3 (default: the
+                                 stages rolled into one loop), 2 (unrolled) or 0
+                                 (compiler's choice); a launch of at most one two-wave
+                                 grid's lanes (131,072 lines on 256 CUs) runs two; the
+                                 mixed variant and the twist tracer always run two */
 #define FQSL_TUNE_GRID_PCT 1    /* persistent grid size in % of one full occupancy wave (default 100) */
 #define FQSL_TUNE_QUEUES 2      /* seed queues of the persistent Scott kernel: 1 (default) or 8 (one per XCD) */
 #define FQSL_TUNE_REFILL_MIN 3  /* idle lanes a wavefront gathers before it refills them (1..64, default 8) */
