@@ -117,6 +117,34 @@ def test_schedules_bitwise_identical(fq, c1):
                 assert np.array_equal(a[k], c[k][inv], equal_nan=True), (prec, k)
 
 
+def test_rolled_and_unrolled_stages_bitwise_identical(fq):
+    """The three-wave FP32c kernels run the RKF45 and RK4 stages as one loop
+    (stage derivatives in LDS), the two-wave ones unrolled; launches above
+    one two-wave grid's lanes take the three-wave kernel under the default
+    knob.  Both forms give the same bits, in the persistent and the
+    one-thread-per-line schedule."""
+    from fastqsl_amd import _lib
+    bx, by, bz = synth.multipole_256(seed=0, device="cuda")
+    g = np.linspace(0.0, 255.0, 512, dtype=np.float32)
+    X, Y = np.meshgrid(g, g)
+    seeds = np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size, np.float32)])   # 262,144 lines
+    assert len(seeds) > 256 * 4 * 2 * 64
+    keep = _lib.tune_get("scott_waves")
+    try:
+        with fq.Field(bx, by, bz, layouts=("soa64", "cell32")) as f:
+            for kw in (dict(method="rkf45", step=0.25, tol=1e-6, schedule="persistent"),
+                       dict(method="rk4", step=0.25, schedule="persistent"),
+                       dict(method="rkf45", step=0.25, tol=1e-6, schedule="per_line")):
+                out = {}
+                for w in (2, 3):
+                    _lib.tune_set("scott_waves", w)
+                    out[w] = _trace(fq, f, seeds, precision="fp32c", **kw)
+                for k in out[2]:
+                    assert np.array_equal(out[2][k], out[3][k], equal_nan=True), (kw, k)
+    finally:
+        _lib.tune_set("scott_waves", keep)
+
+
 C2_LINES = 1 << 16
 
 
