@@ -617,7 +617,8 @@ __device__ __forceinline__ bool claim(unsigned long long* queues, int nq, unsign
 // buckets; within a bucket lines keep roughly their seed order, so fewer
 // buckets keep more of the field locality of neighbouring seeds), which
 // removes most of that tail for ~3 % more work.  (4-cell steps, twice as
-// many, ordered no better and cost 0.25 ms more per 1024^2 map.)
+// many, ordered no better and cost 0.25 ms more per 1024^2 map; 16-cell
+// steps, half as many, save 0.15 ms and order worse by as much.)
 // With many lines per lane the tail is short anyway and the lost locality
 // costs more than it saves, so by default lines are only ordered up to 64
 // lines per persistent lane.  The order only decides which lane traces a line
